@@ -1,0 +1,226 @@
+"""ctypes binding of the C-ABI in include/cubff_hip.h (libcubff_hip.so).
+
+Plumbing for tests, the smoke check and bench.py.  The library is loaded from
+cubff_amd/lib/; if it is missing, import fails loudly: there is no CPU
+fallback for any of these entry points.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcubff_hip.so")
+
+PAIR_SHUFFLE, PAIR_FIXED, PAIR_CYCLIC, PAIR_2D, PAIR_HOST = 0, 1, 2, 3, 4
+
+# Every symbol include/cubff_hip.h declares.
+EXPORTS = [
+    "cubff_last_error", "cubff_lang_id", "cubff_lang_name", "cubff_num_langs",
+    "cubff_ctx_create", "cubff_ctx_destroy", "cubff_init_programs",
+    "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
+    "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
+    "cubff_run_epochs", "cubff_read_counters", "cubff_reset_counters",
+    "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
+    "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
+    "cubff_splitmix64", "cubff_epoch_seed",
+]
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_i = ctypes.c_int
+
+_lib = None
+
+
+class CubffError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libcubff_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CubffError(
+            f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "cubff_last_error": (ctypes.c_char_p, []),
+        "cubff_lang_id": (_i, [ctypes.c_char_p]),
+        "cubff_lang_name": (ctypes.c_char_p, [_i]),
+        "cubff_num_langs": (_i, []),
+        "cubff_ctx_create": (_i, [ctypes.POINTER(_vp), _i, _u64, _i]),
+        "cubff_ctx_destroy": (_i, [_vp]),
+        "cubff_init_programs": (_i, [_vp, _u64, _i, _u64]),
+        "cubff_write_soup": (_i, [_vp, _u64, _u8p, _u64]),
+        "cubff_read_soup": (_i, [_vp, _u64, _u8p, _u64]),
+        "cubff_set_shuffle": (_i, [_vp, _u32p, _u64]),
+        "cubff_set_interactions": (_i, [_vp, _u64p, _u32p]),
+        "cubff_pair": (_i, [_vp, _i, _u64, _u64]),
+        "cubff_run_epoch": (_i, [_vp, _u64, ctypes.c_uint32]),
+        "cubff_run_epochs": (_i, [_vp, _i, _u64, _u64, _u64, ctypes.c_uint32]),
+        "cubff_read_counters": (_i, [_vp, _u64p, _u64p]),
+        "cubff_reset_counters": (_i, [_vp]),
+        "cubff_read_pair_ops": (_i, [_vp, _u32p, _u32p, _u64]),
+        "cubff_read_shuffle": (_i, [_vp, _u32p, _u64p]),
+        "cubff_selfrep": (_i, [_vp, _u64, _u64, _u64p]),
+        "cubff_eval_tapes": (_i, [_i, _i, _u8p, _u64, _u64, _u32p, _u32p]),
+        "cubff_eval_selfrep": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p]),
+        "cubff_sync": (_i, [_vp]),
+        "cubff_splitmix64": (_u64, [_u64]),
+        "cubff_epoch_seed": (_u64, [_u64, _u64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise CubffError(load().cubff_last_error().decode())
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def lang_id(name: str) -> int:
+    i = load().cubff_lang_id(name.encode())
+    if i < 0:
+        raise CubffError(f"Unknown language `{name}`")
+    return i
+
+
+def splitmix64(x: int) -> int:
+    return int(load().cubff_splitmix64(x & (2**64 - 1)))
+
+
+def epoch_seed(params_seed: int, x: int) -> int:
+    return int(load().cubff_epoch_seed(params_seed & (2**64 - 1), x & (2**64 - 1)))
+
+
+def eval_tapes(lang: str, tapes: np.ndarray, steps: int = 8192, device: int = 0):
+    """Evaluates a (count, 128) uint8 array of tapes on the GPU.
+    Returns (final_tapes, ops, steps)."""
+    t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128).copy()
+    n = t.shape[0]
+    ops = np.zeros(n, dtype=np.uint32)
+    st = np.zeros(n, dtype=np.uint32)
+    _check(load().cubff_eval_tapes(lang_id(lang), device, _ptr(t, _u8p), n, steps,
+                                   _ptr(ops, _u32p), _ptr(st, _u32p)))
+    return t, ops, st
+
+
+def eval_selfrep(lang: str, prog: bytes | np.ndarray, epoch: int, seed: int,
+                 device: int = 0) -> int:
+    p = np.frombuffer(bytes(prog), dtype=np.uint8).copy()
+    out = ctypes.c_uint64(0)
+    _check(load().cubff_eval_selfrep(lang_id(lang), device, _ptr(p, _u8p), p.size,
+                                     epoch, seed, ctypes.byref(out)))
+    return int(out.value)
+
+
+class Soup:
+    """One device soup behind a cubff_ctx."""
+
+    def __init__(self, lang: str, num_programs: int, device: int = 0):
+        L = load()
+        self.lang, self.n = lang, num_programs
+        h = _vp()
+        _check(L.cubff_ctx_create(ctypes.byref(h), lang_id(lang), num_programs, device))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().cubff_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init_programs(self, seed_value: int, zero_init: bool = False, count=None):
+        count = self.n if count is None else count
+        _check(load().cubff_init_programs(self._h, seed_value, int(zero_init), count))
+
+    def write_soup(self, data: np.ndarray, offset: int = 0):
+        d = np.ascontiguousarray(data, dtype=np.uint8)
+        _check(load().cubff_write_soup(self._h, offset, _ptr(d, _u8p), d.size))
+
+    def read_soup(self) -> np.ndarray:
+        out = np.zeros(self.n * 64, dtype=np.uint8)
+        _check(load().cubff_read_soup(self._h, 0, _ptr(out, _u8p), out.size))
+        return out
+
+    def set_shuffle(self, s: np.ndarray, num_indices: int | None = None):
+        sa = np.ascontiguousarray(s, dtype=np.uint32)
+        k = self.n // 2 if num_indices is None else num_indices
+        _check(load().cubff_set_shuffle(self._h, _ptr(sa, _u32p), k))
+
+    def set_interactions(self, row_off: np.ndarray, nbr: np.ndarray):
+        self._ro = np.ascontiguousarray(row_off, dtype=np.uint64)
+        self._nb = np.ascontiguousarray(nbr, dtype=np.uint32)
+        if self._nb.size == 0:
+            self._nb = np.zeros(1, dtype=np.uint32)
+        _check(load().cubff_set_interactions(self._h, _ptr(self._ro, _u64p),
+                                             _ptr(self._nb, _u32p)))
+
+    def pair(self, mode: int, params_seed: int, epoch: int):
+        _check(load().cubff_pair(self._h, mode, params_seed, epoch))
+
+    def run_epoch(self, seed_value: int, mutation_prob: int):
+        _check(load().cubff_run_epoch(self._h, seed_value, mutation_prob))
+
+    def run_epochs(self, mode: int, params_seed: int, first_epoch: int,
+                   count: int, mutation_prob: int):
+        _check(load().cubff_run_epochs(self._h, mode, params_seed, first_epoch,
+                                       count, mutation_prob))
+
+    def read_counters(self):
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load().cubff_read_counters(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
+
+    def reset_counters(self):
+        _check(load().cubff_reset_counters(self._h))
+
+    def read_pair_ops(self):
+        ops = np.zeros(self.n // 2, dtype=np.uint32)
+        st = np.zeros(self.n // 2, dtype=np.uint32)
+        _check(load().cubff_read_pair_ops(self._h, _ptr(ops, _u32p), _ptr(st, _u32p),
+                                          self.n // 2))
+        return ops, st
+
+    def read_shuffle(self):
+        s = np.zeros(self.n, dtype=np.uint32)
+        k = ctypes.c_uint64(0)
+        _check(load().cubff_read_shuffle(self._h, _ptr(s, _u32p), ctypes.byref(k)))
+        return s, int(k.value)
+
+    def selfrep(self, seed_value: int, count: int | None = None) -> np.ndarray:
+        count = self.n if count is None else count
+        out = np.zeros(max(count, 1), dtype=np.uint64)
+        _check(load().cubff_selfrep(self._h, seed_value, count, _ptr(out, _u64p)))
+        return out[:count]
+
+    def sync(self):
+        _check(load().cubff_sync(self._h))

@@ -1,0 +1,442 @@
+// C-ABI of the cubff HIP library (declared in include/cubff_hip.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <string>
+
+#include "../../include/cubff_hip.h"
+#include "device_common.h"
+#include "kernels.h"
+#include "languages.h"
+
+using namespace cubff;
+
+struct cubff_ctx {
+  int lang = 0;
+  int device = 0;
+  uint64_t n = 0;
+  hipStream_t stream = nullptr;
+  uint8_t *soup = nullptr;
+  uint32_t *shuf = nullptr;
+  uint64_t *num_indices = nullptr;         // device scalar
+  unsigned long long *counters = nullptr;  // device [ops, num_runs]
+  uint32_t *pair_ops = nullptr;
+  uint32_t *pair_steps = nullptr;
+  uint64_t *selfrep = nullptr;
+  FyScratch fy{};
+  G2Scratch g2{};
+  uint64_t *row_off = nullptr;
+  uint32_t *nbr = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return fail(CUBFF_EHIP, std::string(#expr " failed: ") +              \
+                                  hipGetErrorString(_e));                    \
+  } while (0)
+
+template <typename T>
+int dev_alloc(T **p, uint64_t count) {
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p),
+                           count ? count * sizeof(T) : sizeof(T));
+  if (e != hipSuccess)
+    return fail(CUBFF_ENOMEM, std::string("hipMalloc failed: ") +
+                                  hipGetErrorString(e));
+  return CUBFF_OK;
+}
+
+template <typename T>
+void dev_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+int set_device(int device) {
+  HIP_TRY(hipSetDevice(device));
+  return CUBFF_OK;
+}
+
+int check_ctx(cubff_ctx *ctx) {
+  if (!ctx) return fail(CUBFF_EINVAL, "null context");
+  return set_device(ctx->device);
+}
+
+int ensure_2d_scratch(cubff_ctx *c) {
+  if (c->g2.order) return CUBFF_OK;
+  const uint64_t n = c->n;
+  int r;
+  if ((r = dev_alloc(&c->g2.order, n)) || (r = dev_alloc(&c->g2.eu, n)) ||
+      (r = dev_alloc(&c->g2.ew, n)) || (r = dev_alloc(&c->g2.minpri, n)) ||
+      (r = dev_alloc(&c->g2.rank, n)) || (r = dev_alloc(&c->g2.undecided, 1)) ||
+      (r = dev_alloc(&c->g2.state, n)) ||
+      (r = dev_alloc(&c->g2.used, (n + 3) / 4 * 4)) ||
+      (r = dev_alloc(&c->g2.rest_total, 1)))
+    return r;
+  return CUBFF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cubff_last_error(void) { return g_last_error.c_str(); }
+
+int cubff_num_langs(void) { return kNumLangs; }
+
+const char *cubff_lang_name(int lang) {
+  return (lang >= 0 && lang < kNumLangs) ? kLangNames[lang] : nullptr;
+}
+
+int cubff_lang_id(const char *name) {
+  if (!name) return -1;
+  for (int i = 0; i < kNumLangs; i++)
+    if (strcmp(name, kLangNames[i]) == 0) return i;
+  return -1;
+}
+
+uint64_t cubff_splitmix64(uint64_t x) { return splitmix64(x); }
+
+uint64_t cubff_epoch_seed(uint64_t params_seed, uint64_t x) {
+  return epoch_seed_pre(splitmix64(params_seed), x);
+}
+
+int cubff_ctx_create(cubff_ctx **out, int lang, uint64_t n, int device) {
+  if (!out) return fail(CUBFF_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
+  // common_language.h:365 requires an even count; indices are 32-bit.
+  if (n == 0 || n % 2 != 0) return fail(CUBFF_EINVAL, "num_programs must be even and > 0");
+  if (n >= (1ULL << 32)) return fail(CUBFF_EINVAL, "num_programs must be < 2^32");
+  int r;
+  if ((r = set_device(device))) return r;
+  cubff_ctx *c = new cubff_ctx();
+  c->lang = lang;
+  c->device = device;
+  c->n = n;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+  }
+  if ((r = dev_alloc(&c->soup, n * kTape)) || (r = dev_alloc(&c->shuf, n)) ||
+      (r = dev_alloc(&c->num_indices, 1)) || (r = dev_alloc(&c->counters, 2)) ||
+      (r = dev_alloc(&c->pair_ops, n / 2)) ||
+      (r = dev_alloc(&c->pair_steps, n / 2)) ||
+      (r = dev_alloc(&c->fy.j, n)) || (r = dev_alloc(&c->fy.cnt, n + 1)) ||
+      (r = dev_alloc(&c->fy.base, n)) || (r = dev_alloc(&c->fy.cursor, n)) ||
+      (r = dev_alloc(&c->fy.items, n)) || (r = dev_alloc(&c->fy.first, n)) ||
+      (r = dev_alloc(&c->fy.succ, n))) {
+    std::string keep = g_last_error;
+    cubff_ctx_destroy(c);
+    g_last_error = keep;
+    return r;
+  }
+  // Identity pairing with every slot evaluated until the first pairing call
+  // (the state of RunSimulation after :409-411).
+  hipError_t e2 = hipSuccess;
+  {
+    uint32_t *host = new uint32_t[n];
+    for (uint64_t i = 0; i < n; i++) host[i] = (uint32_t)i;
+    e2 = hipMemcpyAsync(c->shuf, host, n * 4, hipMemcpyHostToDevice, c->stream);
+    if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+    delete[] host;
+  }
+  if (e2 == hipSuccess) e2 = launch_set_u64(c->stream, c->num_indices, n / 2);
+  if (e2 == hipSuccess)
+    e2 = launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 4);
+  if (e2 == hipSuccess) e2 = launch_fill_u32(c->stream, c->pair_ops, 0, n / 2);
+  if (e2 == hipSuccess) e2 = launch_fill_u32(c->stream, c->pair_steps, 0, n / 2);
+  if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
+  if (e2 != hipSuccess) {
+    cubff_ctx_destroy(c);
+    return fail(CUBFF_EHIP, std::string("context init failed: ") + hipGetErrorString(e2));
+  }
+  *out = c;
+  return CUBFF_OK;
+}
+
+int cubff_ctx_destroy(cubff_ctx *c) {
+  if (!c) return CUBFF_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  dev_free(c->soup);
+  dev_free(c->shuf);
+  dev_free(c->num_indices);
+  dev_free(c->counters);
+  dev_free(c->pair_ops);
+  dev_free(c->pair_steps);
+  dev_free(c->selfrep);
+  dev_free(c->fy.j);
+  dev_free(c->fy.cnt);
+  dev_free(c->fy.base);
+  dev_free(c->fy.cursor);
+  dev_free(c->fy.items);
+  dev_free(c->fy.first);
+  dev_free(c->fy.succ);
+  dev_free(c->g2.order);
+  dev_free(c->g2.eu);
+  dev_free(c->g2.ew);
+  dev_free(c->g2.minpri);
+  dev_free(c->g2.rank);
+  dev_free(c->g2.undecided);
+  dev_free(c->g2.state);
+  dev_free(c->g2.used);
+  dev_free(c->g2.rest_total);
+  dev_free(c->row_off);
+  dev_free(c->nbr);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CUBFF_OK;
+}
+
+int cubff_init_programs(cubff_ctx *c, uint64_t seed_value, int zero_init,
+                        uint64_t count) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (count > c->n) return fail(CUBFF_EINVAL, "count exceeds num_programs");
+  HIP_TRY(launch_init(c->stream, c->soup, c->n, seed_value, count, zero_init));
+  return CUBFF_OK;
+}
+
+int cubff_write_soup(cubff_ctx *c, uint64_t offset, const uint8_t *src,
+                     uint64_t len) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (offset > c->n * kTape || len > c->n * kTape - offset)
+    return fail(CUBFF_EINVAL, "write past the end of the soup");
+  if (len == 0) return CUBFF_OK;
+  if (!src) return fail(CUBFF_EINVAL, "null source");
+  HIP_TRY(hipMemcpyAsync(c->soup + offset, src, len, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_read_soup(cubff_ctx *c, uint64_t offset, uint8_t *dst, uint64_t len) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (offset > c->n * kTape || len > c->n * kTape - offset)
+    return fail(CUBFF_EINVAL, "read past the end of the soup");
+  if (len == 0) return CUBFF_OK;
+  if (!dst) return fail(CUBFF_EINVAL, "null destination");
+  HIP_TRY(hipMemcpyAsync(dst, c->soup + offset, len, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_set_shuffle(cubff_ctx *c, const uint32_t *s, uint64_t num_indices) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (!s) return fail(CUBFF_EINVAL, "null shuffle");
+  if (num_indices > c->n / 2) return fail(CUBFF_EINVAL, "num_indices exceeds num_programs/2");
+  for (uint64_t i = 0; i < c->n; i++)
+    if (s[i] >= c->n) return fail(CUBFF_EINVAL, "shuffle index out of range");
+  HIP_TRY(hipMemcpyAsync(c->shuf, s, c->n * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_set_u64(c->stream, c->num_indices, num_indices));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_set_interactions(cubff_ctx *c, const uint64_t *row_off,
+                           const uint32_t *nbr) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (!row_off) return fail(CUBFF_EINVAL, "null row_off");
+  const uint64_t n = c->n;
+  if (row_off[0] != 0) return fail(CUBFF_EINVAL, "row_off[0] must be 0");
+  for (uint64_t i = 0; i < n; i++)
+    if (row_off[i + 1] < row_off[i]) return fail(CUBFF_EINVAL, "row_off must be non-decreasing");
+  const uint64_t e = row_off[n];
+  if (e && !nbr) return fail(CUBFF_EINVAL, "null nbr");
+  for (uint64_t x = 0; x < e; x++)
+    if (nbr[x] >= n)
+      return fail(CUBFF_EINVAL, "invalid interaction pattern: programs not in [0, n) range.");
+  if ((r = ensure_2d_scratch(c))) return r;
+  dev_free(c->row_off);
+  dev_free(c->nbr);
+  if ((r = dev_alloc(&c->row_off, n + 1)) || (r = dev_alloc(&c->nbr, e ? e : 1))) return r;
+  HIP_TRY(hipMemcpyAsync(c->row_off, row_off, (n + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  if (e) HIP_TRY(hipMemcpyAsync(c->nbr, nbr, e * 4, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_pair(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  const uint64_t n = c->n;
+  switch (mode) {
+    case CUBFF_PAIR_SHUFFLE:
+      HIP_TRY(launch_pair_shuffle(c->stream, c->fy, c->shuf, n, params_seed, epoch));
+      HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
+      return CUBFF_OK;
+    case CUBFF_PAIR_FIXED:
+      HIP_TRY(launch_pair_fixed(c->stream, c->shuf, n, epoch));
+      HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
+      return CUBFF_OK;
+    case CUBFF_PAIR_CYCLIC:
+      HIP_TRY(launch_pair_cyclic(c->stream, c->shuf, n, epoch));
+      HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
+      return CUBFF_OK;
+    case CUBFF_PAIR_2D:
+      if (!c->row_off) return fail(CUBFF_ESTATE, "2-D pairing needs cubff_set_interactions first");
+      HIP_TRY(launch_pair_2d(c->stream, c->fy, c->g2, c->row_off, c->nbr, c->shuf, n,
+                             params_seed, epoch, c->num_indices));
+      return CUBFF_OK;
+    default:
+      return fail(CUBFF_EINVAL, "unknown pairing mode");
+  }
+}
+
+int cubff_run_epoch(cubff_ctx *c, uint64_t seed_value, uint32_t prob) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  HIP_TRY(launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
+                       seed_value, prob, c->pair_ops, c->pair_steps, c->counters));
+  return CUBFF_OK;
+}
+
+int cubff_run_epochs(cubff_ctx *c, int mode, uint64_t params_seed,
+                     uint64_t first_epoch, uint64_t count, uint32_t prob) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (mode == CUBFF_PAIR_HOST) return fail(CUBFF_EINVAL, "run_epochs needs a device pairing mode");
+  const uint64_t sm = splitmix64(params_seed);
+  for (uint64_t e = first_epoch; e < first_epoch + count; e++) {
+    if ((r = cubff_pair(c, mode, params_seed, e))) return r;
+    HIP_TRY(launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
+                         epoch_seed_pre(sm, e), prob, c->pair_ops, c->pair_steps,
+                         c->counters));
+  }
+  return CUBFF_OK;
+}
+
+int cubff_read_counters(cubff_ctx *c, uint64_t *ops, uint64_t *num_runs) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  unsigned long long h[2];
+  HIP_TRY(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (ops) *ops = h[0];
+  if (num_runs) *num_runs = h[1];
+  return CUBFF_OK;
+}
+
+int cubff_reset_counters(cubff_ctx *c) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  HIP_TRY(launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 4));
+  return CUBFF_OK;
+}
+
+int cubff_read_pair_ops(cubff_ctx *c, uint32_t *ops, uint32_t *steps,
+                        uint64_t num_slots) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (num_slots > c->n / 2) return fail(CUBFF_EINVAL, "num_slots exceeds num_programs/2");
+  if (ops) HIP_TRY(hipMemcpyAsync(ops, c->pair_ops, num_slots * 4, hipMemcpyDeviceToHost, c->stream));
+  if (steps) HIP_TRY(hipMemcpyAsync(steps, c->pair_steps, num_slots * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_read_shuffle(cubff_ctx *c, uint32_t *s, uint64_t *num_indices) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (s) HIP_TRY(hipMemcpyAsync(s, c->shuf, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+  uint64_t k = 0;
+  HIP_TRY(hipMemcpyAsync(&k, c->num_indices, 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (num_indices) *num_indices = k;
+  return CUBFF_OK;
+}
+
+int cubff_selfrep(cubff_ctx *c, uint64_t seed_value, uint64_t count,
+                  uint64_t *result) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (count > c->n) return fail(CUBFF_EINVAL, "count exceeds num_programs");
+  if (count && !result) return fail(CUBFF_EINVAL, "null result");
+  if (!c->selfrep && (r = dev_alloc(&c->selfrep, c->n))) return r;
+  HIP_TRY(launch_selfrep(c->stream, c->lang, c->soup, c->n, count, seed_value, c->selfrep));
+  if (count)
+    HIP_TRY(hipMemcpyAsync(result, c->selfrep, count * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
+                     uint64_t steps, uint32_t *ops, uint32_t *steps_out) {
+  if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
+  if (count && !tapes) return fail(CUBFF_EINVAL, "null tapes");
+  if (steps > 0xffffffffULL) return fail(CUBFF_EINVAL, "steps must fit in 32 bits");
+  int r;
+  if ((r = set_device(device))) return r;
+  if (count == 0) return CUBFF_OK;
+  uint8_t *d_t = nullptr;
+  uint32_t *d_o = nullptr, *d_s = nullptr;
+  if ((r = dev_alloc(&d_t, count * kTape2)) || (r = dev_alloc(&d_o, count)) ||
+      (r = dev_alloc(&d_s, count))) {
+    dev_free(d_t);
+    dev_free(d_o);
+    return r;
+  }
+  hipError_t e = hipMemcpy(d_t, tapes, count * kTape2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_eval_tapes(nullptr, lang, d_t, count, (uint32_t)steps, d_o, d_s);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(tapes, d_t, count * kTape2, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ops) e = hipMemcpy(ops, d_o, count * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && steps_out) e = hipMemcpy(steps_out, d_s, count * 4, hipMemcpyDeviceToHost);
+  dev_free(d_t);
+  dev_free(d_o);
+  dev_free(d_s);
+  if (e != hipSuccess)
+    return fail(CUBFF_EHIP, std::string("eval_tapes failed: ") + hipGetErrorString(e));
+  return CUBFF_OK;
+}
+
+int cubff_eval_selfrep(int lang, int device, const uint8_t *prog, uint64_t len,
+                       uint64_t epoch, uint64_t seed, uint64_t *result) {
+  if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
+  if (!result) return fail(CUBFF_EINVAL, "null result");
+  int r;
+  if ((r = set_device(device))) return r;
+  uint8_t padded[kTape] = {};
+  if (len > kTape) len = kTape;
+  if (len) memcpy(padded, prog, len);
+  uint8_t *d_p = nullptr;
+  uint64_t *d_r = nullptr;
+  if ((r = dev_alloc(&d_p, kTape)) || (r = dev_alloc(&d_r, 1))) {
+    dev_free(d_p);
+    return r;
+  }
+  const uint64_t seed_value = epoch_seed_pre(splitmix64(seed), epoch);  // :334
+  hipError_t e = hipMemcpy(d_p, padded, kTape, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_selfrep(nullptr, lang, d_p, 1, 1, seed_value, d_r);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(result, d_r, 8, hipMemcpyDeviceToHost);
+  dev_free(d_p);
+  dev_free(d_r);
+  if (e != hipSuccess)
+    return fail(CUBFF_EHIP, std::string("eval_selfrep failed: ") + hipGetErrorString(e));
+  return CUBFF_OK;
+}
+
+int cubff_sync(cubff_ctx *c) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return CUBFF_OK;
+}
+
+}  // extern "C"

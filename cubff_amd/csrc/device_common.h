@@ -1,0 +1,62 @@
+// Device-side building blocks shared by every cubff kernel (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubff {
+
+constexpr int kTape = 64;            // bytes per program (common.h:55)
+constexpr int kTape2 = 2 * kTape;    // bytes per interaction tape
+constexpr uint32_t kEvalSteps = 8192;  // common_language.h:184
+constexpr uint32_t kNone = 0xffffffffu;
+
+// SplitMix64 finaliser (common_language.h:131-136).
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// Epoch seed S(x) = SM(SM(params.seed) ^ SM(x)) (common_language.h:367-369),
+// with SM(params.seed) precomputed by the caller.
+__host__ __device__ __forceinline__ uint64_t epoch_seed_pre(uint64_t sm_seed,
+                                                            uint64_t x) {
+  return splitmix64(sm_seed ^ splitmix64(x));
+}
+
+// One tape per lane, stored in LDS with a dword-interleaved layout: dword w of
+// lane l sits at LDS dword (w * LANES + l).  Every lane therefore always hits
+// bank (l mod #banks), whatever byte it reads, so data-dependent accesses from
+// 64 lanes never conflict.  Consecutive tape bytes share a dword.
+template <int LANES>
+struct LdsTape {
+  uint8_t *p;  // LDS address of this lane's dword 0
+
+  __device__ __forceinline__ static LdsTape at(uint8_t *region, int lane) {
+    return LdsTape{region + lane * 4};
+  }
+  __device__ __forceinline__ uint32_t off(uint32_t i) const {
+    return (i >> 2) * (LANES * 4) + (i & 3);
+  }
+  __device__ __forceinline__ uint8_t get(uint32_t i) const { return p[off(i)]; }
+  __device__ __forceinline__ void set(uint32_t i, uint8_t v) const {
+    p[off(i)] = v;
+  }
+  __device__ __forceinline__ uint32_t word(uint32_t w) const {
+    return *reinterpret_cast<const uint32_t *>(p + w * (LANES * 4));
+  }
+  __device__ __forceinline__ void set_word(uint32_t w, uint32_t v) const {
+    *reinterpret_cast<uint32_t *>(p + w * (LANES * 4)) = v;
+  }
+};
+
+// Sum over the 64 lanes of a wavefront.
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace cubff

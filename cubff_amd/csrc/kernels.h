@@ -1,0 +1,55 @@
+// Launch helpers for the cubff HIP kernels (host side of kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubff {
+
+constexpr int kEpochLanes = 256;      // lanes (= pair slots) per epoch block
+constexpr int kSelfrepProgs = 16;     // programs per self-rep block
+constexpr int kSelfrepLanes = 256;    // >= kSelfrepProgs * 13, multiple of 64
+
+// Device scratch of the Fisher-Yates pairing; every array has n entries
+// except cnt, which has n + 1 (the last entry is the bucket allocator).
+struct FyScratch {
+  uint32_t *j, *cnt, *base, *cursor, *items, *first, *succ;
+};
+
+// Device scratch of the 2-D greedy pairing (n entries each; used is padded
+// to a multiple of 4 bytes).
+struct G2Scratch {
+  uint32_t *order, *eu, *ew, *minpri, *rank, *undecided;
+  uint8_t *state, *used;
+  uint64_t *rest_total;
+};
+
+hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
+                       uint64_t seed_value, uint64_t count, int zero_init);
+hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
+                        const uint32_t *shuf, uint64_t n,
+                        const uint64_t *num_indices, uint64_t seed_value,
+                        uint32_t prob, uint32_t *pair_ops, uint32_t *pair_steps,
+                        unsigned long long *counters);
+hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
+                          uint64_t n, uint64_t count, uint64_t seed_value,
+                          uint64_t *result);
+hipError_t launch_eval_tapes(hipStream_t st, int lang, uint8_t *tapes,
+                             uint64_t count, uint32_t steps, uint32_t *ops,
+                             uint32_t *steps_out);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v,
+                           uint64_t count);
+hipError_t launch_set_u64(hipStream_t st, uint64_t *p, uint64_t v);
+hipError_t launch_pair_fixed(hipStream_t st, uint32_t *s, uint64_t n,
+                             uint64_t epoch);
+hipError_t launch_pair_cyclic(hipStream_t st, uint32_t *s, uint64_t n,
+                              uint64_t epoch);
+hipError_t launch_pair_shuffle(hipStream_t st, const FyScratch &w, uint32_t *s,
+                               uint64_t n, uint64_t params_seed, uint64_t epoch);
+hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,
+                          const G2Scratch &g2, const uint64_t *row_off,
+                          const uint32_t *nbr, uint32_t *s, uint64_t n,
+                          uint64_t params_seed, uint64_t epoch,
+                          uint64_t *num_indices);
+
+}  // namespace cubff

@@ -1,0 +1,150 @@
+/*
+ * cubff on MI355X -- the C-ABI between the host simulation loop and the HIP
+ * kernels of the per-epoch hot path.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  Every
+ * function returns CUBFF_OK (0) or a negative CUBFF_E* code; the message of
+ * the last failure on the calling thread is returned by cubff_last_error().
+ * Host buffers belong to the caller.  A context is used by one thread at a
+ * time.  Reads are synchronous; launches are asynchronous on the context's
+ * stream until a read or cubff_sync().
+ *
+ * Each entry point names the reference interface it replaces
+ * (paradigms-of-intelligence/cubff, file:line).  The reference host loop
+ * (common_language.h:344-597) calls CUDA directly; here the same loop calls
+ * these functions.
+ */
+#ifndef CUBFF_HIP_H_
+#define CUBFF_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CUBFF_OK 0
+#define CUBFF_EINVAL -1   /* bad argument */
+#define CUBFF_EHIP -2     /* HIP runtime error */
+#define CUBFF_ENOMEM -3   /* device allocation failed */
+#define CUBFF_ESTATE -4   /* call not valid in the context's current state */
+
+/* Pairing modes of RunSimulation (common_language.h:435-486). */
+#define CUBFF_PAIR_SHUFFLE 0  /* Fisher-Yates, :418-424 + :476 */
+#define CUBFF_PAIR_FIXED 1    /* --fixed_shuffle, :468-474 */
+#define CUBFF_PAIR_CYCLIC 2   /* --nopermute_programs, :478-486 */
+#define CUBFF_PAIR_2D 3       /* allowed_interactions greedy, :435-463 */
+#define CUBFF_PAIR_HOST 4     /* caller-supplied via cubff_set_shuffle */
+
+typedef struct cubff_ctx cubff_ctx;
+
+/* Message of the last failed call on this thread ("" if none).
+ * Replaces CUCHECK / CHECK printing (common_language.h:35-42, :125-129). */
+const char *cubff_last_error(void);
+
+/* Language registry by name.  Replaces GetLanguage's lookup over the REGISTER
+ * table (common.cc:28-40, common.h:41-45); -1 for an unknown name. */
+int cubff_lang_id(const char *name);
+const char *cubff_lang_name(int lang);
+int cubff_num_langs(void);
+
+/* Context holding one soup of num_programs x 64 bytes on `device`, plus its
+ * shuffle buffer, counters and per-slot outputs.  Replaces the DeviceMemory
+ * buffers of RunSimulation (common_language.h:362-363, :405). */
+int cubff_ctx_create(cubff_ctx **out, int lang, uint64_t num_programs,
+                     int device);
+int cubff_ctx_destroy(cubff_ctx *ctx);
+
+/* InitPrograms launch (common_language.h:138-155, :371-373, :592-593): fills
+ * programs [0, count) with SplitMix64 bytes of seed_value, or zeros. */
+int cubff_init_programs(cubff_ctx *ctx, uint64_t seed_value, int zero_init,
+                        uint64_t count);
+
+/* Soup copies (common_language.h:377, :402, :502). Offsets/lengths in bytes. */
+int cubff_write_soup(cubff_ctx *ctx, uint64_t offset, const uint8_t *src,
+                     uint64_t len);
+int cubff_read_soup(cubff_ctx *ctx, uint64_t offset, uint8_t *dst,
+                    uint64_t len);
+
+/* Caller-computed pairing for the next epoch: the shuf_idx.Write of
+ * common_language.h:488, plus num_indices (:433, :442-456). */
+int cubff_set_shuffle(cubff_ctx *ctx, const uint32_t *s, uint64_t num_indices);
+
+/* Neighbour lists for CUBFF_PAIR_2D in CSR form: program i may interact with
+ * nbr[row_off[i] .. row_off[i+1]), in file order.  Replaces
+ * SimulationParams::allowed_interactions (common.h:71, main.cc:281-299). */
+int cubff_set_interactions(cubff_ctx *ctx, const uint64_t *row_off,
+                           const uint32_t *nbr);
+
+/* Computes the pairing of `epoch` on the device in the given mode (any mode
+ * but CUBFF_PAIR_HOST).  Replaces the host pairing block,
+ * common_language.h:433-488. */
+int cubff_pair(cubff_ctx *ctx, int mode, uint64_t params_seed, uint64_t epoch);
+
+/* One MutateAndRunPrograms pass over the current pairing
+ * (common_language.h:157-193, launched at :490-493): gather, mutate with
+ * probability mutation_prob / 2^30 per byte, evaluate up to 8192 steps for
+ * slots below num_indices, scatter, accumulate ops.  seed_value is the
+ * epoch seed S(epoch) of :367-369. */
+int cubff_run_epoch(cubff_ctx *ctx, uint64_t seed_value,
+                    uint32_t mutation_prob);
+
+/* `count` whole epochs starting at first_epoch: device pairing in `mode`
+ * (not CUBFF_PAIR_HOST) followed by cubff_run_epoch with S(epoch), all on the
+ * context stream.  The loop body of common_language.h:432-494 without the
+ * callback block. */
+int cubff_run_epochs(cubff_ctx *ctx, int mode, uint64_t params_seed,
+                     uint64_t first_epoch, uint64_t count,
+                     uint32_t mutation_prob);
+
+/* Interval counters: ops summed since the last reset (insn_count,
+ * common_language.h:499-501) and the number of evaluated slots (num_runs,
+ * :494). */
+int cubff_read_counters(cubff_ctx *ctx, uint64_t *ops, uint64_t *num_runs);
+int cubff_reset_counters(cubff_ctx *ctx);  /* :585-587 */
+
+/* Per-slot results of the last epoch: ops (the Evaluate return value) and
+ * loop iterations including no-ops.  No reference counterpart: the reference
+ * only keeps the sum (common_language.h:192). */
+int cubff_read_pair_ops(cubff_ctx *ctx, uint32_t *ops, uint32_t *steps,
+                        uint64_t num_slots);
+
+/* Pairing of the last epoch (SimulationState::shuffle_idx, common.h:76,
+ * filled at common_language.h:407-486) and its num_indices. */
+int cubff_read_shuffle(cubff_ctx *ctx, uint32_t *s, uint64_t *num_indices);
+
+/* CheckSelfRep over programs [0, count) with seed_value S(epoch)
+ * (common_language.h:203-277, launched at :563-569); result is
+ * SimulationState::replication_per_prog (common.h:90). */
+int cubff_selfrep(cubff_ctx *ctx, uint64_t seed_value, uint64_t count,
+                  uint64_t *result);
+
+/* Evaluates `count` independent 128-byte tapes of language `lang` in place
+ * (tapes is count*128 bytes, host memory), up to `steps` steps each; ops and
+ * steps (nullable) receive per-tape results.  The batched form of
+ * RunOneProgram (common_language.h:195-201) / RunSingleParsedProgram
+ * (:279-296), used by --run and by parity checks. */
+int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
+                     uint64_t steps, uint32_t *ops, uint32_t *steps_out);
+
+/* Single-program self-replication check of EvalParsedSelfrep
+ * (common_language.h:326-342): prog is zero-padded to 64 bytes and scored
+ * with seed_value = S(epoch) computed from `seed`. */
+int cubff_eval_selfrep(int lang, int device, const uint8_t *prog,
+                       uint64_t len, uint64_t epoch, uint64_t seed,
+                       uint64_t *result);
+
+/* Waits for all work queued on the context (Synchronize, :65). */
+int cubff_sync(cubff_ctx *ctx);
+
+/* Host-side SplitMix64 and epoch seed, so callers compute S(x) exactly as the
+ * device does (common_language.h:131-136, :367-369). */
+uint64_t cubff_splitmix64(uint64_t x);
+uint64_t cubff_epoch_seed(uint64_t params_seed, uint64_t x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUBFF_HIP_H_ */

@@ -1,0 +1,226 @@
+"""Writes the golden fixtures under tests/golden/ from the compiled reference.
+
+Run in the build container after `make -f oracle/ref.mk` (it needs
+oracle/_ref/, which is built from the reference sources):
+
+    python tests/golden/make_golden.py [--only NAME ...]
+
+Every fixture is data: inputs (or the seeds that generate them with
+tests/tapegen.py / SplitMix64) and the reference's outputs.  Hashes are
+SHA-256 of the raw little-endian bytes (first 16 hex digits where noted).
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import zlib
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import refimpl as R  # noqa: E402
+from tests.tapegen import sm_tapes  # noqa: E402
+
+LANGS = [
+    "bff", "bff_noheads", "bff8", "bff8_noheads", "bff_noheads_4bit",
+    "bff_perm", "bff_selfmove", "forth", "forthcopy", "forthtrivial",
+    "forthtrivial_reset", "subleq", "rsubleq4",
+]
+
+
+def sha(a, n=None):
+    h = hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    return h[:n] if n else h
+
+
+def dump(name, obj):
+    path = os.path.join(HERE, name)
+    with open(path, "w") as f:
+        json.dump(obj, f, indent=1, sort_keys=True)
+    print("wrote", path, os.path.getsize(path), "bytes")
+
+
+def grid_pattern(w, h, r=2):
+    """Neighbour lists in the order make_2d_pattern.py:17-29 writes them."""
+    rows = []
+    for i in range(h):
+        for j in range(w):
+            lst = []
+            for ii in range(-r, r + 1):
+                for jj in range(-r, r + 1):
+                    y, x = i + ii, j + jj
+                    if 0 <= y < h and 0 <= x < w and (ii or jj):
+                        lst.append(y * w + x)
+            rows.append(lst)
+    row_off = np.zeros(len(rows) + 1, dtype=np.uint64)
+    row_off[1:] = np.cumsum([len(x) for x in rows])
+    return row_off, np.array([v for x in rows for v in x], dtype=np.uint32)
+
+
+def gen_eval(count=512):
+    out = {}
+    for li, lang in enumerate(LANGS):
+        seed = 1000 + li
+        tapes = sm_tapes(lang, count, seed)
+        ops, crc = [], []
+        finals = []
+        for t in tapes:
+            f, o = R.evaluate(lang, t)
+            ops.append(o)
+            crc.append(zlib.crc32(f.tobytes()))
+            finals.append(f)
+        limits = {}
+        for steps in [0, 1, 2, 7, 100, 1000, 32768]:
+            lo, lc = [], []
+            for t in tapes[:64]:
+                f, o = R.evaluate(lang, t, steps)
+                lo.append(o)
+                lc.append(zlib.crc32(f.tobytes()))
+            limits[str(steps)] = {"ops": lo, "final_crc32": lc}
+        out[lang] = {"seed": seed, "count": count, "steps": 8192, "ops": ops,
+                     "final_crc32": crc, "finals_sha256": sha(np.stack(finals)),
+                     "limits_first64": limits}
+    dump("eval_vectors.json", {
+        "generator": "tests/tapegen.py:sm_tapes(lang, count, seed)",
+        "source": "reference Language::Evaluate via oracle/_ref", "langs": out})
+
+
+def run_epochs(lang, n, seed, epochs, **kw):
+    rows = []
+
+    def cb(st):
+        soup = R.soup_of(st)
+        shuf = R.shuffle_of(st, n)
+        row = {"epoch": st.epoch, "soup": sha(soup, 16), "shuffle": sha(shuf, 16),
+               "total_ops": st.total_ops}
+        if kw.get("eval_selfrep"):
+            rep = R.replication_of(st, n)
+            row["selfrep"] = sha(rep.astype(np.uint64), 16)
+            row["selfrep_ge5"] = int((rep >= 5).sum())
+            row["selfrep_sum"] = int(rep.sum())
+        rows.append(row)
+        return st.epoch >= epochs
+
+    R.run_simulation(lang, n, seed, cb, **kw)
+    return rows
+
+
+def gen_epochs():
+    out = {}
+    for lang in LANGS:
+        out[lang] = run_epochs(lang, 1024, 10248, 48, callback_interval=1)
+    dump("epochs_small.json", {
+        "params": {"num_programs": 1024, "seed": 10248, "mutation_prob": 262144,
+                   "callback_interval": 1, "epochs": 48},
+        "source": "reference RunSimulation via oracle/_ref", "langs": out})
+
+
+def gen_selfrep():
+    out = {}
+    for lang in LANGS:
+        out[lang] = run_epochs(lang, 512, 77, 33, callback_interval=8,
+                               eval_selfrep=True)
+    dump("selfrep_small.json", {
+        "params": {"num_programs": 512, "seed": 77, "callback_interval": 8,
+                   "eval_selfrep": True, "epochs": 33},
+        "source": "reference RunSimulation via oracle/_ref", "langs": out})
+
+
+def gen_2d():
+    cases = {}
+    for (w, h, lang, seed) in [(32, 16, "bff_noheads", 1), (16, 16, "forth", 2)]:
+        row_off, nbr = grid_pattern(w, h)
+        n = w * h
+        rows = run_epochs(lang, n, seed, 12, callback_interval=1,
+                          row_off=row_off, nbr=nbr)
+        cases[f"{lang}_{w}x{h}"] = {"w": w, "h": h, "lang": lang, "seed": seed,
+                                   "radius": 2, "rows": rows}
+    dump("pairing_2d.json", {"source": "reference RunSimulation via oracle/_ref",
+                             "cases": cases})
+
+
+def gen_modes():
+    cases = {}
+    spec = {
+        "fixed_shuffle": dict(fixed_shuffle=True),
+        "cyclic": dict(permute_programs=False),
+        "zero_init": dict(zero_init=True),
+        "reset_interval_8": dict(reset_interval=8),
+        "initial_program": dict(initial_program="[[{.>]-]0000]-]>.{[[<<+-0"),
+        "mutation_1e-3": dict(mutation_prob=round(1e-3 * (1 << 30))),
+    }
+    for name, kw in spec.items():
+        lang = "bff_noheads" if name != "initial_program" else "bff"
+        rows = run_epochs(lang, 512, 21, 20, callback_interval=1, **kw)
+        cases[name] = {"lang": lang, "num_programs": 512, "seed": 21,
+                       "kwargs": kw, "rows": rows}
+    dump("modes.json", {"source": "reference RunSimulation via oracle/_ref",
+                        "cases": cases})
+
+
+def gen_full_size():
+    out = {}
+    for seed in [10248, 0]:
+        rows = run_epochs("bff_noheads", 131072, seed, 257, callback_interval=64)
+        out[str(seed)] = [{"epoch": r["epoch"], "soup": None, "total_ops": r["total_ops"]}
+                          for r in rows]
+        # full hashes for these
+        full = []
+
+        def cb(st):
+            full.append(sha(R.soup_of(st)))
+            return st.epoch >= 257
+
+        R.run_simulation("bff_noheads", 131072, seed, cb, callback_interval=64)
+        for r, h in zip(out[str(seed)], full):
+            r["soup"] = h
+    dump("full_size.json", {
+        "params": {"lang": "bff_noheads", "num_programs": 131072,
+                   "mutation_prob": 262144, "callback_interval": 64},
+        "source": "reference RunSimulation via oracle/_ref", "seeds": out})
+
+
+def gen_cli_logs():
+    """CSV logs of the reference CLI (test.sh:30 settings) with this image's
+    libbrotli, plus the reference's own testdata/ files for comparison."""
+    out = {}
+    tmp = tempfile.mkdtemp()
+    for lang in LANGS:
+        log = os.path.join(tmp, lang + ".txt")
+        subprocess.run([R.MAIN_PATH, "--lang", lang, "--max_epochs", "256",
+                        "--disable_output", "--log", log, "--seed", "10248"],
+                       check=True)
+        out[lang] = open(log).read()
+    shutil.rmtree(tmp)
+    ref_dir = os.path.join("/root/reference", "testdata")
+    dst = os.path.join(HERE, "reference_testdata")
+    os.makedirs(dst, exist_ok=True)
+    for lang in LANGS:
+        shutil.copyfile(os.path.join(ref_dir, lang + ".txt"),
+                        os.path.join(dst, lang + ".txt"))
+    dump("cli_logs_brotli109.json", {
+        "command": "bin/main --lang L --max_epochs 256 --disable_output --log F --seed 10248",
+        "brotli": "libbrotlienc 1.0.9 (this image)", "logs": out})
+
+
+GENS = {"eval": gen_eval, "epochs": gen_epochs, "selfrep": gen_selfrep,
+        "2d": gen_2d, "modes": gen_modes, "full": gen_full_size,
+        "cli": gen_cli_logs}
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--only", nargs="*", default=list(GENS))
+    args = ap.parse_args()
+    if not R.available():
+        sys.exit("oracle/_ref is not built: run make -f oracle/ref.mk")
+    for name in args.only:
+        GENS[name]()
