@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Benchmark of the cubff per-epoch hot path on one MI355X.
+
+Workload: bff_noheads, 2^17 programs x 64 bytes, uniform SplitMix64 bytes from
+seed 0, default mutation probability 1/4096 (P = 262144 / 2^30), at most 8192
+steps per interaction, Fisher-Yates pairing every epoch.  One "step" is one
+epoch: device pairing + mutate + evaluate + write back over the whole soup.
+
+Prints one JSON line.  `value` is interpreted instructions per second (the
+reference's ops: executed non-no-op instructions) over the timed epochs, with
+the soup resident in HBM.  The roofline object prices the epoch kernel against
+the integer-VALU peak with the frozen work model
+
+    W_epoch = (N/2) * 128 * C_mut + sum(steps) * C_step,
+    C_mut = 32 lane-ops per mutation draw, C_step = 8 lane-ops per BFF step,
+
+where steps counts every interpreter loop iteration including no-ops, and
+peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 int32 lane-ops/s.
+
+The CPU baseline runs the reference build from oracle/_ref (compiled from the
+reference sources) on a bounded sample of the same workload on the host's
+cores; without it, the oracle's OpenMP port is timed instead.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("interpreted instr/s + epochs/s, 2^17×64B bff_noheads; "
+          "batched+sched vs plain")
+PEAK_LANE_OPS = 256 * 64 * 2.4e9
+C_MUT, C_STEP = 32, {"bff": 8, "forth": 10, "subleq": 10, "rsubleq4": 12}
+
+
+def c_step(lang: str) -> int:
+    if lang.startswith("bff"):
+        return C_STEP["bff"]
+    if lang.startswith("forth"):
+        return C_STEP["forth"]
+    return C_STEP[lang]
+
+
+def cpu_baseline(lang: str, n: int, seed: int, prob: int, budget_s: float):
+    """Times the reference CPU path (or the oracle port) on the host."""
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    from oracle import refimpl
+    if refimpl.available():
+        interval = 32
+        marks = []
+
+        def cb(st):
+            marks.append((time.perf_counter(), int(st.total_ops), int(st.epoch)))
+            return (marks[-1][0] - marks[0][0]) >= budget_s
+
+        refimpl.run_simulation(lang, n, seed, cb, mutation_prob=prob,
+                               callback_interval=interval)
+        (t0, ops0, e0), (t1, ops1, e1) = marks[0], marks[-1]
+        return {"value": (ops1 - ops0) / (t1 - t0), "unit": "instr/s",
+                "cores": threads, "kind": "reference",
+                "epochs_per_s": (e1 - e0) / (t1 - t0),
+                "sample": (f"reference RunSimulation (oracle/_ref, g++ -O3 -fopenmp) on "
+                           f"the same workload, epochs {e0}..{e1 - 1} from seed {seed}, "
+                           f"callbacks every {interval} epochs inside the timed span")}
+    from oracle import oracle as O
+    S = O.Soup(lang, n, seed, prob, nthreads=threads)
+    S.step()
+    t0 = time.perf_counter()
+    ops = 0
+    e = 0
+    while time.perf_counter() - t0 < budget_s:
+        ops += S.step()[0]
+        e += 1
+    dt = time.perf_counter() - t0
+    return {"value": ops / dt, "unit": "instr/s", "cores": threads, "kind": "port",
+            "epochs_per_s": e / dt,
+            "sample": f"oracle OpenMP port, epochs 1..{e} of the same workload"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=256, help="timed epochs")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed epochs")
+    ap.add_argument("--lang", default="bff_noheads")
+    ap.add_argument("--num", type=int, default=131072)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="wall-clock budget of the CPU baseline sample (0 = skip)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the epoch runs on a single MI355X and "
+              "this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    from cubff_amd import capi
+    prob = 1 << 18  # round(1/4096 * 2^30), main.cc:183, :226
+    n, seed, lang = args.num, args.seed, args.lang
+    with capi.Soup(lang, n) as g:
+        g.init_programs(capi.epoch_seed(seed, 0))
+        g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, args.warmup, prob)
+        g.sync()
+        g.reset_counters()
+        t0 = time.perf_counter()
+        total_ms, kern_ms = g.run_epochs_timed(capi.PAIR_SHUFFLE, seed, args.warmup,
+                                               args.steps, prob)
+        g.sync()
+        wall = time.perf_counter() - t0
+        ops, runs, steps = g.read_counters(with_steps=True)
+
+    K = args.steps
+    dt = total_ms / 1e3
+    work = (n // 2) * 128 * C_MUT * K + steps * c_step(lang)
+    achieved = work / (kern_ms / 1e3)
+    line = {
+        "metric": METRIC,
+        "value": ops / dt,
+        "unit": "instr/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{lang}, {n} programs x 64 B, seed {seed}, mutation "
+                         f"P={prob}/2^30, 8192 max steps, Fisher-Yates pairing, "
+                         f"epochs {args.warmup}..{args.warmup + K - 1}"),
+            "lang": lang, "num_programs": n, "seed": seed,
+            "parallelism": "single soup, one lane per pair slot",
+        },
+        "epochs_per_s": K / dt,
+        "interp_steps_per_s": steps / dt,
+        "ops_per_pair": ops / max(runs, 1),
+        "wall_s": wall,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_LANE_OPS / 1e12,
+            "unit": "T int32 lane-ops/s",
+            "frac": achieved / PEAK_LANE_OPS,
+            "traffic": None,
+            "kernel": "k_epoch (mutate + evaluate + scatter)",
+            "kernel_ms_avg": kern_ms / K,
+            "work_per_launch": work / K,
+            "model": "W = (N/2)*128*32 + sum(steps)*C_step",
+        },
+        "cpu_baseline": None,
+    }
+    if args.cpu_seconds > 0:
+        line["cpu_baseline"] = cpu_baseline(lang, n, seed, prob, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -22,7 +22,7 @@ EXPORTS = [
     "cubff_ctx_create", "cubff_ctx_destroy", "cubff_init_programs",
     "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
     "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
-    "cubff_run_epochs", "cubff_read_counters", "cubff_reset_counters",
+    "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
     "cubff_splitmix64", "cubff_epoch_seed",
@@ -67,7 +67,10 @@ def load():
         "cubff_pair": (_i, [_vp, _i, _u64, _u64]),
         "cubff_run_epoch": (_i, [_vp, _u64, ctypes.c_uint32]),
         "cubff_run_epochs": (_i, [_vp, _i, _u64, _u64, _u64, ctypes.c_uint32]),
-        "cubff_read_counters": (_i, [_vp, _u64p, _u64p]),
+        "cubff_run_epochs_timed": (_i, [_vp, _i, _u64, _u64, _u64, ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_float),
+                                        ctypes.POINTER(ctypes.c_float)]),
+        "cubff_read_counters": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_reset_counters": (_i, [_vp]),
         "cubff_read_pair_ops": (_i, [_vp, _u32p, _u32p, _u64]),
         "cubff_read_shuffle": (_i, [_vp, _u32p, _u64p]),
@@ -195,9 +198,24 @@ class Soup:
         _check(load().cubff_run_epochs(self._h, mode, params_seed, first_epoch,
                                        count, mutation_prob))
 
-    def read_counters(self):
-        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
-        _check(load().cubff_read_counters(self._h, ctypes.byref(a), ctypes.byref(b)))
+    def run_epochs_timed(self, mode: int, params_seed: int, first_epoch: int,
+                         count: int, mutation_prob: int):
+        """Returns (total_ms, epoch_kernel_ms) from HIP events on the
+        context stream."""
+        a, b = ctypes.c_float(0), ctypes.c_float(0)
+        _check(load().cubff_run_epochs_timed(self._h, mode, params_seed, first_epoch,
+                                             count, mutation_prob, ctypes.byref(a),
+                                             ctypes.byref(b)))
+        return float(a.value), float(b.value)
+
+    def read_counters(self, with_steps: bool = False):
+        """(ops, num_runs) since the last reset; (ops, num_runs, steps) with
+        with_steps."""
+        a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load().cubff_read_counters(self._h, ctypes.byref(a), ctypes.byref(b),
+                                          ctypes.byref(c)))
+        if with_steps:
+            return int(a.value), int(b.value), int(c.value)
         return int(a.value), int(b.value)
 
     def reset_counters(self):

@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/cubff_hip.h"
 #include "device_common.h"
@@ -19,7 +20,7 @@ struct cubff_ctx {
   uint8_t *soup = nullptr;
   uint32_t *shuf = nullptr;
   uint64_t *num_indices = nullptr;         // device scalar
-  unsigned long long *counters = nullptr;  // device [ops, num_runs]
+  unsigned long long *counters = nullptr;  // device [ops, num_runs, steps, -]
   uint32_t *pair_ops = nullptr;
   uint32_t *pair_steps = nullptr;
   uint64_t *selfrep = nullptr;
@@ -130,7 +131,7 @@ int cubff_ctx_create(cubff_ctx **out, int lang, uint64_t n, int device) {
     return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
   }
   if ((r = dev_alloc(&c->soup, n * kTape)) || (r = dev_alloc(&c->shuf, n)) ||
-      (r = dev_alloc(&c->num_indices, 1)) || (r = dev_alloc(&c->counters, 2)) ||
+      (r = dev_alloc(&c->num_indices, 1)) || (r = dev_alloc(&c->counters, kNumCounters)) ||
       (r = dev_alloc(&c->pair_ops, n / 2)) ||
       (r = dev_alloc(&c->pair_steps, n / 2)) ||
       (r = dev_alloc(&c->fy.j, n)) || (r = dev_alloc(&c->fy.cnt, n + 1)) ||
@@ -154,7 +155,7 @@ int cubff_ctx_create(cubff_ctx **out, int lang, uint64_t n, int device) {
   }
   if (e2 == hipSuccess) e2 = launch_set_u64(c->stream, c->num_indices, n / 2);
   if (e2 == hipSuccess)
-    e2 = launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 4);
+    e2 = launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 2 * kNumCounters);
   if (e2 == hipSuccess) e2 = launch_fill_u32(c->stream, c->pair_ops, 0, n / 2);
   if (e2 == hipSuccess) e2 = launch_fill_u32(c->stream, c->pair_steps, 0, n / 2);
   if (e2 == hipSuccess) e2 = hipStreamSynchronize(c->stream);
@@ -321,21 +322,63 @@ int cubff_run_epochs(cubff_ctx *c, int mode, uint64_t params_seed,
   return CUBFF_OK;
 }
 
-int cubff_read_counters(cubff_ctx *c, uint64_t *ops, uint64_t *num_runs) {
+int cubff_run_epochs_timed(cubff_ctx *c, int mode, uint64_t params_seed,
+                           uint64_t first_epoch, uint64_t count, uint32_t prob,
+                           float *total_ms, float *epoch_kernel_ms) {
   int r;
   if ((r = check_ctx(c))) return r;
-  unsigned long long h[2];
+  if (mode == CUBFF_PAIR_HOST) return fail(CUBFF_EINVAL, "run_epochs needs a device pairing mode");
+  if (mode == CUBFF_PAIR_2D) return fail(CUBFF_EINVAL, "timed runs support the closed-form and shuffle pairings");
+  const uint64_t sm = splitmix64(params_seed);
+  hipEvent_t ev0, ev1;
+  HIP_TRY(hipEventCreate(&ev0));
+  HIP_TRY(hipEventCreate(&ev1));
+  std::vector<hipEvent_t> ks(2 * count);
+  for (auto &e : ks) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(ev0, c->stream));
+  for (uint64_t i = 0; i < count; i++) {
+    const uint64_t e = first_epoch + i;
+    if ((r = cubff_pair(c, mode, params_seed, e))) return r;
+    HIP_TRY(hipEventRecord(ks[2 * i], c->stream));
+    HIP_TRY(launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
+                         epoch_seed_pre(sm, e), prob, c->pair_ops, c->pair_steps,
+                         c->counters));
+    HIP_TRY(hipEventRecord(ks[2 * i + 1], c->stream));
+  }
+  HIP_TRY(hipEventRecord(ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(ev1));
+  float tot = 0, kern = 0;
+  HIP_TRY(hipEventElapsedTime(&tot, ev0, ev1));
+  for (uint64_t i = 0; i < count; i++) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ks[2 * i], ks[2 * i + 1]));
+    kern += ms;
+  }
+  for (auto &e : ks) (void)hipEventDestroy(e);
+  (void)hipEventDestroy(ev0);
+  (void)hipEventDestroy(ev1);
+  if (total_ms) *total_ms = tot;
+  if (epoch_kernel_ms) *epoch_kernel_ms = kern;
+  return CUBFF_OK;
+}
+
+int cubff_read_counters(cubff_ctx *c, uint64_t *ops, uint64_t *num_runs,
+                        uint64_t *steps) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  unsigned long long h[kNumCounters];
   HIP_TRY(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (ops) *ops = h[0];
   if (num_runs) *num_runs = h[1];
+  if (steps) *steps = h[2];
   return CUBFF_OK;
 }
 
 int cubff_reset_counters(cubff_ctx *c) {
   int r;
   if ((r = check_ctx(c))) return r;
-  HIP_TRY(launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 4));
+  HIP_TRY(launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->counters), 0, 2 * kNumCounters));
   return CUBFF_OK;
 }
 

@@ -9,6 +9,7 @@ namespace cubff {
 constexpr int kEpochLanes = 256;      // lanes (= pair slots) per epoch block
 constexpr int kSelfrepProgs = 16;     // programs per self-rep block
 constexpr int kSelfrepLanes = 256;    // >= kSelfrepProgs * 13, multiple of 64
+constexpr int kNumCounters = 4;       // ops, num_runs, steps, unused
 
 // Device scratch of the Fisher-Yates pairing; every array has n entries
 // except cnt, which has n + 1 (the last entry is the bucket allocator).

@@ -91,21 +91,23 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch(
   using L = typename LangOf<LANG>::T;
   constexpr int kAux = lang_needs_aux(LANG) ? 1 : 0;
   __shared__ __align__(16) uint8_t lds[kEpochLanes * kTape2 * (1 + kAux)];
-  __shared__ unsigned long long block_ops;
+  __shared__ unsigned long long block_ops, block_steps;
   const int lane = threadIdx.x;
-  if (lane == 0) block_ops = 0;
+  if (lane == 0) {
+    block_ops = 0;
+    block_steps = 0;
+  }
   const uint64_t k = (uint64_t)blockIdx.x * kEpochLanes + lane;
   const uint64_t slots = n / 2;
   const auto t = LdsTape<kEpochLanes>::at(lds, lane);
   const auto aux = LdsTape<kEpochLanes>::at(lds + kEpochLanes * kTape2 * kAux, lane);
-  uint32_t ops = 0;
+  uint32_t ops = 0, steps = 0;
   __syncthreads();
   if (k < slots) {
     const uint32_t p1 = shuf[2 * k], p2 = shuf[2 * k + 1];
     load_row(t, 0, soup + (uint64_t)p1 * kTape);
     load_row(t, 1, soup + (uint64_t)p2 * kTape);
     mutate(t, n, seed_value, k, prob);
-    uint32_t steps = 0;
     if (k < *num_indices_p) ops = L::evaluate(t, aux, kEvalSteps, &steps);
     store_row(t, 0, soup + (uint64_t)p1 * kTape);
     store_row(t, 1, soup + (uint64_t)p2 * kTape);  // p2's half wins if p1 == p2
@@ -113,9 +115,12 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch(
     pair_steps[k] = steps;
   }
   const uint64_t wsum = wave_sum_u64(ops);
+  const uint64_t wsteps = wave_sum_u64(steps);
   if ((lane & 63) == 0 && wsum) atomicAdd(&block_ops, (unsigned long long)wsum);
+  if ((lane & 63) == 0 && wsteps) atomicAdd(&block_steps, (unsigned long long)wsteps);
   __syncthreads();
   if (lane == 0 && block_ops) atomicAdd(&counters[0], block_ops);
+  if (lane == 0 && block_steps) atomicAdd(&counters[2], block_steps);
   if (blockIdx.x == 0 && lane == 0) atomicAdd(&counters[1], (unsigned long long)*num_indices_p);
 }
 

@@ -106,102 +106,185 @@ __device__ __forceinline__ int bff_kind(uint32_t c) {
   }
 }
 
+// 128-bit per-lane position masks (two 64-bit halves held in VGPRs).
+struct Mask128 {
+  uint64_t lo, hi;
+  __device__ __forceinline__ bool test(int p) const {
+    return ((p < 64 ? lo : hi) >> (p & 63)) & 1;
+  }
+  __device__ __forceinline__ void flip(int p, bool f) {
+    const uint64_t b = (uint64_t)f << (p & 63);
+    if (p < 64) lo ^= b; else hi ^= b;
+  }
+  // Smallest set position >= p (0 <= p <= 128), or 128.
+  __device__ __forceinline__ int next_from(int p) const {
+    const uint64_t l = p >= 64 ? 0 : lo & (~0ULL << p);
+    const uint64_t h = p <= 64 ? hi : (p >= 128 ? 0 : hi & (~0ULL << (p - 64)));
+    return l ? __builtin_ctzll(l) : (h ? 64 + __builtin_ctzll(h) : 128);
+  }
+  // Largest set position <= p (-1 <= p <= 127), or -1.
+  __device__ __forceinline__ int prev_upto(int p) const {
+    const uint64_t h = p < 64 ? 0 : (p >= 127 ? hi : hi & (~0ULL >> (127 - p)));
+    const uint64_t l = p < 0 ? 0 : (p >= 63 ? lo : lo & (~0ULL >> (63 - p)));
+    return h ? 127 - __builtin_clzll(h) : (l ? 63 - __builtin_clzll(l) : -1);
+  }
+};
+
+// Byte classes of a BFF-family map: command (not a comment and not the
+// null op), loop start, loop end.
+template <int MAP>
+struct BffClass {
+  __device__ __forceinline__ static bool cmd(uint32_t v) {
+    if constexpr (MAP == kMapAscii) {
+      // + , - . < > (0x2b-0x2e, 0x3c, 0x3e) and [ ] { } (0x5b, 0x5d, 0x7b, 0x7d).
+      constexpr uint64_t kLo = (0xfULL << 0x2b) | (1ULL << 0x3c) | (1ULL << 0x3e);
+      constexpr uint64_t kHi = (1ULL << (0x5b - 64)) | (1ULL << (0x5d - 64)) |
+                               (1ULL << (0x7b - 64)) | (1ULL << (0x7d - 64));
+      return v < 128 && (((v < 64 ? kLo : kHi) >> (v & 63)) & 1);
+    } else if constexpr (MAP == kMap4bit) {
+      return (v & 15) >= 6;
+    } else if constexpr (MAP == kMapPerm) {
+      return v - 1 < 10u;
+    } else {
+      return v < 7;
+    }
+  }
+  __device__ __forceinline__ static bool ls(uint32_t v) {
+    if constexpr (MAP == kMapAscii) return v == '[';
+    else if constexpr (MAP == kMap4bit) return (v & 15) == 6;
+    else if constexpr (MAP == kMapPerm) return v == 9;
+    else return v == 5;
+  }
+  __device__ __forceinline__ static bool le(uint32_t v) {
+    if constexpr (MAP == kMapAscii) return v == ']';
+    else if constexpr (MAP == kMap4bit) return (v & 15) == 7;
+    else if constexpr (MAP == kMapPerm) return v == 10;
+    else return v == 6;
+  }
+  // Loop-test "byte counts as zero": kind == kNull for the standard logic
+  // (bff.inc.h:318, :332), the byte itself for bff8 / selfmove.
+  __device__ __forceinline__ static bool null(uint32_t v) {
+    if constexpr (MAP == kMap4bit) return (v & 15) == 0;
+    else return v == 0;
+  }
+};
+
 template <int MAP, bool HEADS, int LOGIC>
 struct BffLang {
   static constexpr int kMap = MAP;
   static constexpr bool kHeads = HEADS;
   static constexpr int kLogic = LOGIC;
-
-  // Forward scan to the matching loop end (bff.inc.h:319-328).
-  template <class T>
-  __device__ __forceinline__ static int scan_forward(const T &t, int pc) {
-    int depth = 1;
-    int p = pc + 1;
-    for (; p < kTape2 && depth > 0; p++) {
-      int k = bff_kind<MAP>(t.get(p));
-      depth += (k == kLoopStart) - (k == kLoopEnd);
-    }
-    return depth != 0 ? kTape2 : p - 1;
-  }
-
-  // Backward scan to the matching loop start (bff.inc.h:333-342).
-  template <class T>
-  __device__ __forceinline__ static int scan_backward(const T &t, int pc) {
-    int depth = 1;
-    int p = pc - 1;
-    for (; p >= 0 && depth > 0; p--) {
-      int k = bff_kind<MAP>(t.get(p));
-      depth += (k == kLoopEnd) - (k == kLoopStart);
-    }
-    return depth != 0 ? -1 : p + 1;
-  }
+  using C = BffClass<MAP>;
 
   // Evaluate (bff.inc.h:356-389) with EvaluateOne (bff.inc.h:289-349,
   // bff8.cu:23-65, bff_selfmove.cu:44-98).
+  //
+  // Runs of comment / null bytes change nothing but the step counters, so
+  // the lane advances over a whole run at once, found with a 128-bit mask
+  // of command positions; the step cap still binds inside a run.  Bracket
+  // scans (bff.inc.h:317-344) walk 128-bit masks of `[` and `]` positions
+  // instead of every byte.  The masks are kept equal to the tape: every
+  // write that changes a byte's class flips the affected bits.
   template <class T>
   __device__ static uint32_t evaluate(const T &t, const T & /*aux*/,
                                       uint32_t stepcount, uint32_t *steps) {
+    Mask128 mc{0, 0}, ml{0, 0}, me{0, 0};
+#pragma unroll
+    for (int w = 0; w < kTape2 / 4; w++) {
+      const uint32_t word = t.word(w);
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const uint32_t v = (word >> (8 * b)) & 0xff;
+        const int p = w * 4 + b;
+        const uint64_t bit = 1ULL << (p & 63);
+        if (p < 64) {
+          mc.lo |= C::cmd(v) ? bit : 0;
+          ml.lo |= C::ls(v) ? bit : 0;
+          me.lo |= C::le(v) ? bit : 0;
+        } else {
+          mc.hi |= C::cmd(v) ? bit : 0;
+          ml.hi |= C::ls(v) ? bit : 0;
+          me.hi |= C::le(v) ? bit : 0;
+        }
+      }
+    }
     int h0, h1, pc;
     if constexpr (HEADS) {  // InitialState, bff.inc.h:269-275
       h0 = t.get(0) & (kTape2 - 1);
       h1 = t.get(1) & (kTape2 - 1);
       pc = 2;
-    } else {  // bff.inc.h:277-282 (128 masks to 0)
+    } else {  // bff.inc.h:277-282; 128 masks to 0
       h0 = 0;
       h1 = 0;
       pc = 0;
     }
     uint32_t i = 0, nskip = 0;
-    while (i < stepcount) {
-      h0 &= kTape2 - 1;
-      h1 &= kTape2 - 1;
+    for (;;) {
+      // Comment run [pc, nxt): each byte is one step that only counts.
+      const int nxt = mc.next_from(pc);
+      const uint32_t adv = min((uint32_t)(nxt - pc), stepcount - i);
+      i += adv;
+      nskip += adv;
+      pc += adv;
+      if (i >= stepcount || pc >= kTape2) break;
+      // One command at pc.
+      const uint32_t c = t.get(pc);
+      const uint32_t v0 = t.get(h0);
+      const uint32_t v1 = t.get(h1);
       i++;
-      int k = bff_kind<MAP>(t.get(pc));
-      switch (k) {
-        case kDec0: h0--; break;
-        case kInc0: h0++; break;
-        case kDec1:
-          if (LOGIC == kLogicSelfmove) nskip++; else h1--;
-          break;
-        case kInc1:
-          if (LOGIC == kLogicSelfmove) nskip++; else h1++;
-          break;
-        case kPlus: t.set(h0, t.get(h0) + 1); break;
-        case kMinus: t.set(h0, t.get(h0) - 1); break;
-        case kCopy01:
-          if (LOGIC == kLogicSelfmove) nskip++; else t.set(h1, t.get(h0));
-          break;
-        case kCopy10:
-          t.set(h0, t.get(h1));
-          if (LOGIC == kLogicSelfmove) h1++;
-          break;
-        case kLoopStart: {
-          uint32_t v = t.get(h0);
-          if (LOGIC == kLogicBff8) {
-            if (v == 0) pc += 8;
-          } else if (LOGIC == kLogicSelfmove) {
-            if (v == 0) pc = scan_forward(t, pc);
-          } else if (bff_kind<MAP>(v) == kNull) {
-            pc = scan_forward(t, pc);
-          }
-          break;
+      const int k = bff_kind<MAP>(c);
+      if (k >= kDec0) {  // head moves (kNull/kNoop cannot occur here)
+        h0 = (h0 + (k == kInc0) - (k == kDec0)) & (kTape2 - 1);
+        h1 = (h1 + (k == kInc1) - (k == kDec1)) & (kTape2 - 1);
+      } else if (k >= kPlus) {  // + - . ,
+        int addr = h0;
+        uint32_t old = v0, val;
+        if (k == kPlus) val = v0 + 1;
+        else if (k == kMinus) val = v0 - 1;
+        else if (k == kCopy01) { addr = h1; old = v1; val = v0; }
+        else val = v1;  // kCopy10
+        val &= 0xff;
+        if (LOGIC == kLogicSelfmove && k == kCopy10) h1 = (h1 + 1) & (kTape2 - 1);
+        if (val != old) {
+          t.set(addr, (uint8_t)val);
+          mc.flip(addr, C::cmd(val) != C::cmd(old));
+          ml.flip(addr, C::ls(val) != C::ls(old));
+          me.flip(addr, C::le(val) != C::le(old));
         }
-        case kLoopEnd: {
-          uint32_t v = t.get(h0);
-          if (LOGIC == kLogicBff8) {
-            if (v != 0) pc -= 8;
-          } else if (LOGIC == kLogicSelfmove) {
-            if (v != 0) pc = scan_backward(t, pc);
-          } else if (bff_kind<MAP>(v) != kNull) {
-            pc = scan_backward(t, pc);
+      } else if (k == kLoopStart) {
+        if (LOGIC == kLogicBff8) {
+          if (v0 == 0) pc += 8;
+        } else if (C::null(v0)) {
+          // Forward scan to the matching loop end; unmatched -> 128.
+          int depth = 1, p = pc + 1;
+          for (;;) {
+            const int ql = ml.next_from(p), qe = me.next_from(p);
+            const int q = min(ql, qe);
+            if (q >= kTape2) { pc = kTape2; break; }
+            depth += (q == ql) ? 1 : -1;
+            if (depth == 0) { pc = q; break; }
+            p = q + 1;
           }
-          break;
         }
-        default: nskip++; break;
+      } else {  // kLoopEnd
+        if (LOGIC == kLogicBff8) {
+          if (v0 != 0) pc -= 8;
+        } else if (!C::null(v0)) {
+          // Backward scan to the matching loop start; unmatched -> -1.
+          int depth = 1, p = pc - 1;
+          for (;;) {
+            const int ql = ml.prev_upto(p), qe = me.prev_upto(p);
+            const int q = max(ql, qe);
+            if (q < 0) { pc = -1; break; }
+            depth += (q == qe) ? 1 : -1;
+            if (depth == 0) { pc = q; break; }
+            p = q - 1;
+          }
+        }
       }
       if (pc < 0) break;
       pc++;
-      if (pc >= kTape2) break;
+      if (pc >= kTape2 || i >= stepcount) break;
     }
     *steps = i;
     return i - nskip;

@@ -98,10 +98,22 @@ int cubff_run_epochs(cubff_ctx *ctx, int mode, uint64_t params_seed,
                      uint64_t first_epoch, uint64_t count,
                      uint32_t mutation_prob);
 
+/* cubff_run_epochs with HIP events recorded on the context stream: *total_ms
+ * receives the device time of the whole sequence, *epoch_kernel_ms the summed
+ * duration of the epoch (mutate + evaluate) launches alone.  Blocks until
+ * done.  No reference counterpart: the reference times wall-clock intervals
+ * between callbacks (common_language.h:497, :508-517). */
+int cubff_run_epochs_timed(cubff_ctx *ctx, int mode, uint64_t params_seed,
+                           uint64_t first_epoch, uint64_t count,
+                           uint32_t mutation_prob, float *total_ms,
+                           float *epoch_kernel_ms);
+
 /* Interval counters: ops summed since the last reset (insn_count,
- * common_language.h:499-501) and the number of evaluated slots (num_runs,
- * :494). */
-int cubff_read_counters(cubff_ctx *ctx, uint64_t *ops, uint64_t *num_runs);
+ * common_language.h:499-501), the number of evaluated slots (num_runs,
+ * :494) and the loop iterations including no-ops (no reference counterpart;
+ * the work measure of the roofline).  Each pointer may be null. */
+int cubff_read_counters(cubff_ctx *ctx, uint64_t *ops, uint64_t *num_runs,
+                        uint64_t *steps);
 int cubff_reset_counters(cubff_ctx *ctx);  /* :585-587 */
 
 /* Per-slot results of the last epoch: ops (the Evaluate return value) and
