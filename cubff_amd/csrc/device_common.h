@@ -32,6 +32,7 @@ __host__ __device__ __forceinline__ uint64_t epoch_seed_pre(uint64_t sm_seed,
 // 64 lanes never conflict.  Consecutive tape bytes share a dword.
 template <int LANES>
 struct LdsTape {
+  static constexpr int kRegionBytes = kTape2 * LANES;
   uint8_t *p;  // LDS address of this lane's dword 0
 
   __device__ __forceinline__ static LdsTape at(uint8_t *region, int lane) {

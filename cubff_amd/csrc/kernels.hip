@@ -43,8 +43,8 @@ __global__ void k_init(uint8_t *__restrict__ soup, uint64_t n,
 // draws, evaluate if k < num_indices, scatter with 16-byte stores, record
 // per-slot ops/steps and add the block's ops to the interval counter.
 
-template <int LANES>
-__device__ __forceinline__ void load_row(const LdsTape<LANES> &t, int half,
+template <class Tape>
+__device__ __forceinline__ void load_row(const Tape &t, int half,
                                          const uint8_t *__restrict__ src) {
   const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
 #pragma unroll
@@ -58,8 +58,8 @@ __device__ __forceinline__ void load_row(const LdsTape<LANES> &t, int half,
   }
 }
 
-template <int LANES>
-__device__ __forceinline__ void store_row(const LdsTape<LANES> &t, int half,
+template <class Tape>
+__device__ __forceinline__ void store_row(const Tape &t, int half,
                                           uint8_t *__restrict__ dst) {
   uint4 *d4 = reinterpret_cast<uint4 *>(dst);
 #pragma unroll
@@ -71,8 +71,8 @@ __device__ __forceinline__ void store_row(const LdsTape<LANES> &t, int half,
 
 // Mutation of common_language.h:172-180: byte b of slot k is replaced by
 // r & 0xFF when ((r >> 8) & (2^30 - 1)) < P, r = SM((N*S(e) + k)*128 + b).
-template <int LANES>
-__device__ __forceinline__ void mutate(const LdsTape<LANES> &t, uint64_t n,
+template <class Tape>
+__device__ __forceinline__ void mutate(const Tape &t, uint64_t n,
                                        uint64_t seed_value, uint64_t k,
                                        uint32_t prob) {
   const uint64_t base = (n * seed_value + k) * (uint64_t)kTape2;
@@ -89,8 +89,10 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch(
     uint64_t seed_value, uint32_t prob, uint32_t *__restrict__ pair_ops,
     uint32_t *__restrict__ pair_steps, unsigned long long *__restrict__ counters) {
   using L = typename LangOf<LANG>::T;
-  constexpr int kAux = lang_needs_aux(LANG) ? 1 : 0;
-  __shared__ __align__(16) uint8_t lds[kEpochLanes * kTape2 * (1 + kAux)];
+  using Tape = typename L::template Tape<kEpochLanes>;
+  using Aux = LdsTape<kEpochLanes>;
+  constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
+  __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   __shared__ unsigned long long block_ops, block_steps;
   const int lane = threadIdx.x;
   if (lane == 0) {
@@ -99,8 +101,8 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch(
   }
   const uint64_t k = (uint64_t)blockIdx.x * kEpochLanes + lane;
   const uint64_t slots = n / 2;
-  const auto t = LdsTape<kEpochLanes>::at(lds, lane);
-  const auto aux = LdsTape<kEpochLanes>::at(lds + kEpochLanes * kTape2 * kAux, lane);
+  const auto t = Tape::at(lds, lane);
+  const auto aux = Aux::at(lds + Tape::kRegionBytes, lane);
   uint32_t ops = 0, steps = 0;
   __syncthreads();
   if (k < slots) {
@@ -137,15 +139,17 @@ __global__ __launch_bounds__(kSelfrepLanes) void k_selfrep(
     const uint8_t *__restrict__ soup, uint64_t n, uint64_t count,
     uint64_t seed_value, uint64_t *__restrict__ result) {
   using L = typename LangOf<LANG>::T;
-  constexpr int kAux = lang_needs_aux(LANG) ? 1 : 0;
   constexpr int kLanes = kSelfrepLanes;  // kSelfrepProgs * 13 (+ padding)
-  __shared__ __align__(16) uint8_t lds[kLanes * kTape2 * (1 + kAux)];
+  using Tape = typename L::template Tape<kLanes>;
+  using Aux = LdsTape<kLanes>;
+  constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
+  __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   __shared__ uint32_t res[kSelfrepProgs][2];
   const int lane = threadIdx.x;
   const int q = lane / kIters, it = lane % kIters;
   const uint64_t idx = (uint64_t)blockIdx.x * kSelfrepProgs + q;
-  const auto t = LdsTape<kLanes>::at(lds, lane);
-  const auto aux = LdsTape<kLanes>::at(lds + kLanes * kTape2 * kAux, lane);
+  const auto t = Tape::at(lds, lane);
+  const auto aux = Aux::at(lds + Tape::kRegionBytes, lane);
   if (lane < kSelfrepProgs * 2) res[lane / 2][lane % 2] = 0;
   const bool active = q < kSelfrepProgs && idx < count;
   if (active) {
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(kSelfrepLanes) void k_selfrep(
     uint8_t v[kIters];
 #pragma unroll
     for (int a = 0; a < kIters; a++)
-      v[a] = LdsTape<kLanes>::at(lds, qq * kIters + a).get(i);
+      v[a] = Tape::at(lds, qq * kIters + a).get(i);
     for (int a = 0; a < kIters; a++) {
       if (i < kTape && v[a] != orig) continue;
       int c = 1;
@@ -215,13 +219,15 @@ __global__ __launch_bounds__(kEpochLanes) void k_eval_tapes(
     uint8_t *__restrict__ tapes, uint64_t count, uint32_t stepcount,
     uint32_t *__restrict__ ops_out, uint32_t *__restrict__ steps_out) {
   using L = typename LangOf<LANG>::T;
-  constexpr int kAux = lang_needs_aux(LANG) ? 1 : 0;
-  __shared__ __align__(16) uint8_t lds[kEpochLanes * kTape2 * (1 + kAux)];
+  using Tape = typename L::template Tape<kEpochLanes>;
+  using Aux = LdsTape<kEpochLanes>;
+  constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
+  __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   const int lane = threadIdx.x;
   const uint64_t k = (uint64_t)blockIdx.x * kEpochLanes + lane;
   if (k >= count) return;
-  const auto t = LdsTape<kEpochLanes>::at(lds, lane);
-  const auto aux = LdsTape<kEpochLanes>::at(lds + kEpochLanes * kTape2 * kAux, lane);
+  const auto t = Tape::at(lds, lane);
+  const auto aux = Aux::at(lds + Tape::kRegionBytes, lane);
   load_row(t, 0, tapes + k * kTape2);
   load_row(t, 1, tapes + k * kTape2 + kTape);
   uint32_t steps = 0;

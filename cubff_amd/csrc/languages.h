@@ -106,12 +106,40 @@ __device__ __forceinline__ int bff_kind(uint32_t c) {
   }
 }
 
+// Byte-linear per-lane tape in LDS: byte i of lane l sits at i * kStride + l.
+// One multiply-add forms any byte address.  The stride of LANES + 4 bytes
+// moves each row by one bank, so lanes reading different positions rarely
+// collide.  Row kTape2 is a per-lane dummy byte that absorbs writes which
+// must not land on the tape.
+template <int LANES>
+struct LinTape {
+  static constexpr int kStride = LANES + 4;
+  static constexpr int kRegionBytes = (kTape2 + 1) * kStride;
+  static constexpr int kDummy = kTape2;
+  uint8_t *p;
+
+  __device__ __forceinline__ static LinTape at(uint8_t *region, int lane) {
+    return LinTape{region + lane};
+  }
+  __device__ __forceinline__ uint8_t get(uint32_t i) const { return p[i * kStride]; }
+  __device__ __forceinline__ void set(uint32_t i, uint8_t v) const {
+    p[i * kStride] = v;
+  }
+  __device__ __forceinline__ uint32_t word(uint32_t w) const {
+    return (uint32_t)get(4 * w) | ((uint32_t)get(4 * w + 1) << 8) |
+           ((uint32_t)get(4 * w + 2) << 16) | ((uint32_t)get(4 * w + 3) << 24);
+  }
+  __device__ __forceinline__ void set_word(uint32_t w, uint32_t v) const {
+    set(4 * w, (uint8_t)v);
+    set(4 * w + 1, (uint8_t)(v >> 8));
+    set(4 * w + 2, (uint8_t)(v >> 16));
+    set(4 * w + 3, (uint8_t)(v >> 24));
+  }
+};
+
 // 128-bit per-lane position masks (two 64-bit halves held in VGPRs).
 struct Mask128 {
   uint64_t lo, hi;
-  __device__ __forceinline__ bool test(int p) const {
-    return ((p < 64 ? lo : hi) >> (p & 63)) & 1;
-  }
   __device__ __forceinline__ void flip(int p, bool f) {
     const uint64_t b = (uint64_t)f << (p & 63);
     if (p < 64) lo ^= b; else hi ^= b;
@@ -130,8 +158,12 @@ struct Mask128 {
   }
 };
 
-// Byte classes of a BFF-family map: command (not a comment and not the
-// null op), loop start, loop end.
+// Byte classes and opcode decode of a BFF-family map.
+//
+// At a command position the opcode is decoded without a compare tree: a
+// 4-bit hash of the byte indexes a 64-bit table of 4-bit kinds.  For the
+// ASCII map the hash ((c * 7) >> 2) & 15 is collision-free on [ ] + - . , <
+// > { }; the other maps index by the byte (or its low nibble) directly.
 template <int MAP>
 struct BffClass {
   __device__ __forceinline__ static bool cmd(uint32_t v) {
@@ -167,45 +199,69 @@ struct BffClass {
     if constexpr (MAP == kMap4bit) return (v & 15) == 0;
     else return v == 0;
   }
+  // BffKind of a command byte (undefined for comments / null).
+  __device__ __forceinline__ static uint32_t kind(uint32_t c) {
+    uint32_t h;
+    uint64_t table;
+    if constexpr (MAP == kMapAscii) {
+      h = ((c * 7) >> 2) & 15;
+      table = 0x0357296080000104ULL;
+    } else if constexpr (MAP == kMap4bit) {
+      h = c & 15;
+      table = 0x9876543210000000ULL;
+    } else if constexpr (MAP == kMapPerm) {
+      h = c & 15;
+      table = 0x0000010543298760ULL;
+    } else {
+      h = c & 15;
+      table = 0x0000000001053267ULL;
+    }
+    return (uint32_t)(table >> (4 * h)) & 15;
+  }
 };
 
 template <int MAP, bool HEADS, int LOGIC>
 struct BffLang {
-  static constexpr int kMap = MAP;
-  static constexpr bool kHeads = HEADS;
-  static constexpr int kLogic = LOGIC;
+  template <int LANES> using Tape = LinTape<LANES>;
   using C = BffClass<MAP>;
+  // Per-kind fields, 2 bits each, indexed by BffKind: head-0 delta + 1,
+  // head-1 delta + 1 (bff_selfmove's `,` also advances head 1), byte delta + 1.
+  static constexpr uint32_t kTH0 = 0x58555;
+  static constexpr uint32_t kTH1 = LOGIC == kLogicSelfmove ? 0x55955 : 0x85555;
+  static constexpr uint32_t kTDelta = 0x55525;
 
   // Evaluate (bff.inc.h:356-389) with EvaluateOne (bff.inc.h:289-349,
   // bff8.cu:23-65, bff_selfmove.cu:44-98).
   //
-  // Runs of comment / null bytes change nothing but the step counters, so
-  // the lane advances over a whole run at once, found with a 128-bit mask
-  // of command positions; the step cap still binds inside a run.  Bracket
-  // scans (bff.inc.h:317-344) walk 128-bit masks of `[` and `]` positions
-  // instead of every byte.  The masks are kept equal to the tape: every
-  // write that changes a byte's class flips the affected bits.
-  template <class T>
-  __device__ static uint32_t evaluate(const T &t, const T & /*aux*/,
+  // The loop keeps pc on a command byte.  Runs of comment / null bytes
+  // change nothing but the step counters, so after each command the lane
+  // moves straight to the next command, found in a 128-bit mask of command
+  // positions, and counts the skipped bytes as no-op steps (the step cap
+  // still binds inside a run).  Each command is straight-line code: table
+  // decode, head deltas, one (possibly dummy) byte store.  Bracket scans
+  // (bff.inc.h:317-344) walk 128-bit masks of `[` and `]` positions; the
+  // last forward and last backward jump are cached as (source, position
+  // after the target, next command from there) until a write changes any
+  // byte's class.  Masks always equal the tape: every write that changes a
+  // byte's class flips the affected bits.
+  template <class T, class A>
+  __device__ static uint32_t evaluate(const T &t, const A & /*aux*/,
                                       uint32_t stepcount, uint32_t *steps) {
     Mask128 mc{0, 0}, ml{0, 0}, me{0, 0};
+    for (int w = 0; w < kTape2 / 8; w++) {
+      uint64_t bc = 0, bl = 0, be = 0;
 #pragma unroll
-    for (int w = 0; w < kTape2 / 4; w++) {
-      const uint32_t word = t.word(w);
-#pragma unroll
-      for (int b = 0; b < 4; b++) {
-        const uint32_t v = (word >> (8 * b)) & 0xff;
-        const int p = w * 4 + b;
-        const uint64_t bit = 1ULL << (p & 63);
-        if (p < 64) {
-          mc.lo |= C::cmd(v) ? bit : 0;
-          ml.lo |= C::ls(v) ? bit : 0;
-          me.lo |= C::le(v) ? bit : 0;
-        } else {
-          mc.hi |= C::cmd(v) ? bit : 0;
-          ml.hi |= C::ls(v) ? bit : 0;
-          me.hi |= C::le(v) ? bit : 0;
-        }
+      for (int b = 0; b < 8; b++) {
+        const uint32_t v = t.get(w * 8 + b);
+        bc |= (uint64_t)C::cmd(v) << b;
+        bl |= (uint64_t)C::ls(v) << b;
+        be |= (uint64_t)C::le(v) << b;
+      }
+      const int sh = (w * 8) & 63;
+      if (w < 8) {
+        mc.lo |= bc << sh; ml.lo |= bl << sh; me.lo |= be << sh;
+      } else {
+        mc.hi |= bc << sh; ml.hi |= bl << sh; me.hi |= be << sh;
       }
     }
     int h0, h1, pc;
@@ -218,73 +274,99 @@ struct BffLang {
       h1 = 0;
       pc = 0;
     }
-    uint32_t i = 0, nskip = 0;
-    for (;;) {
-      // Comment run [pc, nxt): each byte is one step that only counts.
+    uint32_t i, nskip;
+    {
       const int nxt = mc.next_from(pc);
-      const uint32_t adv = min((uint32_t)(nxt - pc), stepcount - i);
-      i += adv;
-      nskip += adv;
-      pc += adv;
-      if (i >= stepcount || pc >= kTape2) break;
-      // One command at pc.
-      const uint32_t c = t.get(pc);
-      const uint32_t v0 = t.get(h0);
-      const uint32_t v1 = t.get(h1);
-      i++;
-      const int k = bff_kind<MAP>(c);
-      if (k >= kDec0) {  // head moves (kNull/kNoop cannot occur here)
-        h0 = (h0 + (k == kInc0) - (k == kDec0)) & (kTape2 - 1);
-        h1 = (h1 + (k == kInc1) - (k == kDec1)) & (kTape2 - 1);
-      } else if (k >= kPlus) {  // + - . ,
-        int addr = h0;
-        uint32_t old = v0, val;
-        if (k == kPlus) val = v0 + 1;
-        else if (k == kMinus) val = v0 - 1;
-        else if (k == kCopy01) { addr = h1; old = v1; val = v0; }
-        else val = v1;  // kCopy10
-        val &= 0xff;
-        if (LOGIC == kLogicSelfmove && k == kCopy10) h1 = (h1 + 1) & (kTape2 - 1);
-        if (val != old) {
-          t.set(addr, (uint8_t)val);
-          mc.flip(addr, C::cmd(val) != C::cmd(old));
-          ml.flip(addr, C::ls(val) != C::ls(old));
-          me.flip(addr, C::le(val) != C::le(old));
-        }
-      } else if (k == kLoopStart) {
-        if (LOGIC == kLogicBff8) {
-          if (v0 == 0) pc += 8;
-        } else if (C::null(v0)) {
-          // Forward scan to the matching loop end; unmatched -> 128.
-          int depth = 1, p = pc + 1;
-          for (;;) {
-            const int ql = ml.next_from(p), qe = me.next_from(p);
-            const int q = min(ql, qe);
-            if (q >= kTape2) { pc = kTape2; break; }
-            depth += (q == ql) ? 1 : -1;
-            if (depth == 0) { pc = q; break; }
-            p = q + 1;
+      i = nskip = min((uint32_t)(nxt - pc), stepcount);
+      pc = nxt;
+    }
+    if (i < stepcount && pc < kTape2) {
+      // Jump caches: source pc, position after the jump, next command.
+      int bsrc = -1, bafter = 0, bnext = 0;
+      int fsrc = -1, fafter = 0, fnext = 0;
+      for (;;) {
+        const uint32_t c = t.get(pc);
+        const uint32_t v0 = t.get(h0);
+        const uint32_t v1 = t.get(h1);
+        int fall = mc.next_from(pc + 1);
+        const uint32_t k = C::kind(c);
+        const bool c01 = k == kCopy01;
+        const int waddr = c01 ? h1 : h0;
+        const uint32_t wold = c01 ? v1 : v0;
+        const uint32_t wval =
+            ((k == kCopy10 ? v1 : v0) + ((kTDelta >> (2 * k)) & 3) - 1) & 0xff;
+        const bool changed = (k - kPlus) < 4u && wval != wold;
+        t.set(changed ? waddr : T::kDummy, (uint8_t)wval);
+        h0 = (h0 + (int)((kTH0 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
+        h1 = (h1 + (int)((kTH1 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
+        if (changed) {
+          const bool fc = C::cmd(wval) != C::cmd(wold);
+          const bool fl = C::ls(wval) != C::ls(wold);
+          const bool fe = C::le(wval) != C::le(wold);
+          if (fc | fl | fe) {
+            mc.flip(waddr, fc);
+            ml.flip(waddr, fl);
+            me.flip(waddr, fe);
+            bsrc = fsrc = -1;
+            fall = mc.next_from(pc + 1);
           }
         }
-      } else {  // kLoopEnd
-        if (LOGIC == kLogicBff8) {
-          if (v0 != 0) pc -= 8;
-        } else if (!C::null(v0)) {
-          // Backward scan to the matching loop start; unmatched -> -1.
-          int depth = 1, p = pc - 1;
-          for (;;) {
-            const int ql = ml.prev_upto(p), qe = me.prev_upto(p);
-            const int q = max(ql, qe);
-            if (q < 0) { pc = -1; break; }
-            depth += (q == qe) ? 1 : -1;
-            if (depth == 0) { pc = q; break; }
-            p = q - 1;
+        int after, nextc;
+        if constexpr (LOGIC == kLogicBff8) {
+          // bff8.cu:51-60: [ skips 8 when the byte is zero, ] goes back 8.
+          const bool tf = k == kLoopStart && v0 == 0;
+          const bool tb = k == kLoopEnd && v0 != 0;
+          after = pc + 1 + (tf ? 8 : 0) - (tb ? 8 : 0);
+          if (tb && pc < 8) after = -1;  // pc - 8 < 0 stops the run
+          after = min(after, kTape2);
+          nextc = after < 0 ? -1 : (tf | tb ? mc.next_from(after) : fall);
+        } else {
+          const bool z = LOGIC == kLogicSelfmove ? v0 == 0 : C::null(v0);
+          const bool tf = k == kLoopStart && z;
+          const bool tb = k == kLoopEnd && !z;
+          if (tb && pc != bsrc) {
+            // Backward scan to the matching loop start; unmatched -> stop.
+            int depth = 1, p = pc - 1, q;
+            for (;;) {
+              const int ql = ml.prev_upto(p), qe = me.prev_upto(p);
+              q = max(ql, qe);
+              if (q < 0) break;
+              depth += (q == qe) ? 1 : -1;
+              if (depth == 0) break;
+              p = q - 1;
+            }
+            bsrc = pc;
+            bafter = q + 1;
+            bnext = q < 0 ? -1 : mc.next_from(q + 1);
           }
+          if (tf && pc != fsrc) {
+            // Forward scan to the matching loop end; unmatched -> 128.
+            int depth = 1, p = pc + 1, q;
+            for (;;) {
+              const int ql = ml.next_from(p), qe = me.next_from(p);
+              q = min(ql, qe);
+              if (q >= kTape2) break;
+              depth += (q == ql) ? 1 : -1;
+              if (depth == 0) break;
+              p = q + 1;
+            }
+            fsrc = pc;
+            fafter = min(q + 1, kTape2);
+            fnext = mc.next_from(fafter);
+          }
+          after = tb ? bafter : (tf ? fafter : pc + 1);
+          nextc = tb ? bnext : (tf ? fnext : fall);
         }
+        i++;
+        if (nextc < 0) break;
+        // Comment run [after, nextc): each byte is one counted no-op step.
+        const uint32_t rem = stepcount - i;
+        const uint32_t adv = min((uint32_t)(nextc - after), rem);
+        i += adv;
+        nskip += adv;
+        if (adv == rem || nextc >= kTape2) break;
+        pc = nextc;
       }
-      if (pc < 0) break;
-      pc++;
-      if (pc >= kTape2 || i >= stepcount) break;
     }
     *steps = i;
     return i - nskip;
@@ -341,6 +423,7 @@ __device__ __forceinline__ int forth_kind(uint32_t c) {
 
 template <int LANG>
 struct ForthLang {
+  template <int LANES> using Tape = LdsTape<LANES>;
   // Reset of forthtrivial_reset.cu:58-68.
   template <class T>
   __device__ static void reset(const T &t, int pos) {
@@ -355,8 +438,8 @@ struct ForthLang {
   // Evaluate (forth.inc.h:379-397) with EvaluateOne (forth.inc.h:251-373,
   // forthtrivial_reset.cu:123-252).  The stack (forth.inc.h:236-249) lives in
   // the lane's aux LDS region.
-  template <class T>
-  __device__ static uint32_t evaluate(const T &t, const T &stk,
+  template <class T, class A>
+  __device__ static uint32_t evaluate(const T &t, const A &stk,
                                       uint32_t stepcount, uint32_t *steps) {
     int sp = 0;
     bool overflow = false;
@@ -449,8 +532,9 @@ struct ForthLang {
 // iteration counts as an op.
 
 struct SubleqLang {
-  template <class T>
-  __device__ static uint32_t evaluate(const T &t, const T &, uint32_t stepcount,
+  template <int LANES> using Tape = LdsTape<LANES>;
+  template <class T, class A>
+  __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
                                       uint32_t *steps) {
     int pos = 0;
     uint32_t i = 0;
@@ -470,8 +554,9 @@ struct SubleqLang {
 };
 
 struct Rsubleq4Lang {
-  template <class T>
-  __device__ static uint32_t evaluate(const T &t, const T &, uint32_t stepcount,
+  template <int LANES> using Tape = LdsTape<LANES>;
+  template <class T, class A>
+  __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
                                       uint32_t *steps) {
     int pos = 0;
     uint32_t i = 0;
