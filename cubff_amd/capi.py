@@ -26,7 +26,16 @@ EXPORTS = [
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
     "cubff_splitmix64", "cubff_epoch_seed",
+    "cubff_batch_last_error", "cubff_batch_create", "cubff_batch_destroy",
+    "cubff_batch_init", "cubff_batch_set_launch", "cubff_batch_get_launch",
+    "cubff_batch_run_epochs", "cubff_batch_run_epochs_timed",
+    "cubff_batch_autotune", "cubff_batch_sync", "cubff_batch_read_soup",
+    "cubff_batch_write_soup", "cubff_batch_read_counters",
+    "cubff_batch_reset_counters", "cubff_batch_read_pair_stats",
+    "cubff_batch_read_shuffle",
 ]
+
+SCHED_PLAIN, SCHED_LEARNED = 0, 1
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -80,6 +89,26 @@ def load():
         "cubff_sync": (_i, [_vp]),
         "cubff_splitmix64": (_u64, [_u64]),
         "cubff_epoch_seed": (_u64, [_u64, _u64]),
+        "cubff_batch_last_error": (ctypes.c_char_p, []),
+        "cubff_batch_create": (_i, [ctypes.POINTER(_vp), ctypes.c_uint32,
+                                    ctypes.POINTER(_i), _u64p, _i]),
+        "cubff_batch_destroy": (_i, [_vp]),
+        "cubff_batch_init": (_i, [_vp, _u64p, _u32p, _i]),
+        "cubff_batch_set_launch": (_i, [_vp, _i, ctypes.c_uint32]),
+        "cubff_batch_get_launch": (_i, [_vp, ctypes.POINTER(_i), _u32p]),
+        "cubff_batch_run_epochs": (_i, [_vp, _u64, _u64, _i, _i]),
+        "cubff_batch_run_epochs_timed": (_i, [_vp, _u64, _u64, _i,
+                                              ctypes.POINTER(ctypes.c_float),
+                                              ctypes.POINTER(ctypes.c_float)]),
+        "cubff_batch_autotune": (_i, [_vp, _u64, _u64, _i, ctypes.POINTER(_i), _u32p,
+                                      ctypes.POINTER(ctypes.c_float)]),
+        "cubff_batch_sync": (_i, [_vp]),
+        "cubff_batch_read_soup": (_i, [_vp, ctypes.c_uint32, _u8p]),
+        "cubff_batch_write_soup": (_i, [_vp, ctypes.c_uint32, _u8p]),
+        "cubff_batch_read_counters": (_i, [_vp, ctypes.c_uint32, _u64p, _u64p, _u64p]),
+        "cubff_batch_reset_counters": (_i, [_vp]),
+        "cubff_batch_read_pair_stats": (_i, [_vp, ctypes.c_uint32, _u32p, _u32p]),
+        "cubff_batch_read_shuffle": (_i, [_vp, ctypes.c_uint32, _u32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -242,3 +271,108 @@ class Soup:
 
     def sync(self):
         _check(load().cubff_sync(self._h))
+
+
+def _bcheck(rc: int):
+    if rc != 0:
+        raise CubffError(load().cubff_batch_last_error().decode())
+
+
+class Batch:
+    """Many independent soups advanced together (cubff_batch)."""
+
+    def __init__(self, langs, num_programs, device: int = 0):
+        L = load()
+        self.langs = list(langs)
+        self.n = [int(x) for x in num_programs]
+        if len(self.langs) != len(self.n):
+            raise ValueError("langs and num_programs differ in length")
+        ids = (ctypes.c_int * len(self.langs))(*[lang_id(x) for x in self.langs])
+        ns = np.array(self.n, dtype=np.uint64)
+        h = _vp()
+        _bcheck(L.cubff_batch_create(ctypes.byref(h), len(self.n), ids, _ptr(ns, _u64p),
+                                     device))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().cubff_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def init(self, params_seeds, mutation_probs, zero_init: bool = False):
+        sd = np.array([x & (2**64 - 1) for x in params_seeds], dtype=np.uint64)
+        pr = np.array(mutation_probs, dtype=np.uint32)
+        _bcheck(load().cubff_batch_init(self._h, _ptr(sd, _u64p), _ptr(pr, _u32p),
+                                        int(zero_init)))
+
+    def set_launch(self, wg: int, tapes_per_lane: int):
+        _bcheck(load().cubff_batch_set_launch(self._h, wg, tapes_per_lane))
+
+    def get_launch(self):
+        w, t = ctypes.c_int(0), ctypes.c_uint32(0)
+        _bcheck(load().cubff_batch_get_launch(self._h, ctypes.byref(w), ctypes.byref(t)))
+        return int(w.value), int(t.value)
+
+    def run_epochs(self, first_epoch: int, count: int, sched: int = SCHED_PLAIN,
+                   use_graph: bool = False):
+        _bcheck(load().cubff_batch_run_epochs(self._h, first_epoch, count, sched,
+                                              int(use_graph)))
+
+    def run_epochs_timed(self, first_epoch: int, count: int, sched: int = SCHED_PLAIN):
+        a, b = ctypes.c_float(0), ctypes.c_float(0)
+        _bcheck(load().cubff_batch_run_epochs_timed(self._h, first_epoch, count, sched,
+                                                    ctypes.byref(a), ctypes.byref(b)))
+        return float(a.value), float(b.value)
+
+    def autotune(self, first_epoch: int, epochs_per_config: int, sched: int = SCHED_PLAIN):
+        w, t = ctypes.c_int(0), ctypes.c_uint32(0)
+        tab = (ctypes.c_float * 12)()
+        _bcheck(load().cubff_batch_autotune(self._h, first_epoch, epochs_per_config, sched,
+                                            ctypes.byref(w), ctypes.byref(t), tab))
+        return int(w.value), int(t.value), [float(x) for x in tab]
+
+    def sync(self):
+        _bcheck(load().cubff_batch_sync(self._h))
+
+    def read_soup(self, i: int) -> np.ndarray:
+        out = np.zeros(self.n[i] * 64, dtype=np.uint8)
+        _bcheck(load().cubff_batch_read_soup(self._h, i, _ptr(out, _u8p)))
+        return out
+
+    def write_soup(self, i: int, data: np.ndarray):
+        d = np.ascontiguousarray(data, dtype=np.uint8)
+        assert d.size == self.n[i] * 64
+        _bcheck(load().cubff_batch_write_soup(self._h, i, _ptr(d, _u8p)))
+
+    def read_counters(self, i: int):
+        a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _bcheck(load().cubff_batch_read_counters(self._h, i, ctypes.byref(a),
+                                                 ctypes.byref(b), ctypes.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def reset_counters(self):
+        _bcheck(load().cubff_batch_reset_counters(self._h))
+
+    def read_pair_stats(self, i: int):
+        ops = np.zeros(self.n[i] // 2, dtype=np.uint32)
+        st = np.zeros(self.n[i] // 2, dtype=np.uint32)
+        _bcheck(load().cubff_batch_read_pair_stats(self._h, i, _ptr(ops, _u32p),
+                                                   _ptr(st, _u32p)))
+        return ops, st
+
+    def read_shuffle(self, i: int) -> np.ndarray:
+        s = np.zeros(self.n[i], dtype=np.uint32)
+        _bcheck(load().cubff_batch_read_shuffle(self._h, i, _ptr(s, _u32p)))
+        return s

@@ -1,0 +1,450 @@
+// C-ABI of the batched-soup path (declared in include/cubff_hip.h).
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cubff_hip.h"
+#include "batch.h"
+#include "device_common.h"
+#include "kernels.h"
+#include "languages.h"
+#include "sched.h"
+
+using namespace cubff;
+
+struct cubff_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t num_soups = 0;
+  std::vector<int> langs;
+  std::vector<uint64_t> n, prog_base, slot_base;
+  BatchDev d{};
+  // Device allocations owned by the batch (d's const pointers alias these).
+  uint64_t *prog_base_d = nullptr, *slot_base_d = nullptr, *sm_seed_d = nullptr;
+  uint64_t *num_indices_d = nullptr;
+  uint32_t *prob_d = nullptr;
+  // Per language present: the identity slot list and its scheduled order.
+  std::vector<SchedList> lists;  // offsets into plain_list / order_list
+  uint32_t *plain_list = nullptr;   // total_slots
+  uint32_t *order_list = nullptr;   // total_slots (scheduled order)
+  SchedState sched{};
+  int wg = 256;
+  uint32_t per_lane = 1;
+  bool have_seeds = false;
+  hipGraphExec_t graph = nullptr;
+  int graph_sched = -1, graph_wg = 0;
+  uint32_t graph_per_lane = 0;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess)                                                    \
+      return fail(CUBFF_EHIP, std::string(#expr " failed: ") +              \
+                                  hipGetErrorString(_e));                    \
+  } while (0)
+
+template <typename T>
+int alloc(T **p, uint64_t count) {
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(p), (count ? count : 1) * sizeof(T));
+  if (e != hipSuccess)
+    return fail(CUBFF_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  return CUBFF_OK;
+}
+
+template <typename T>
+void release(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+int check(cubff_batch *b) {
+  if (!b) return fail(CUBFF_EINVAL, "null batch");
+  HIP_TRY(hipSetDevice(b->device));
+  return CUBFF_OK;
+}
+
+void drop_graph(cubff_batch *b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_sched = -1;
+}
+
+// One epoch on the batch stream: pairing, stage, [schedule], interpret,
+// end-of-epoch bookkeeping.  interp_ev (nullable) brackets the interpreter
+// launches with events.
+int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
+  const hipStream_t st = b->stream;
+  HIP_TRY(batch_pair_shuffle(st, b->d));
+  HIP_TRY(batch_stage(st, b->d));
+  const uint32_t *list = b->plain_list;
+  if (sched == CUBFF_SCHED_LEARNED) {
+    HIP_TRY(sched_order(st, b->sched, b->d, b->plain_list, b->order_list, b->lists.data(),
+                        b->lists.size()));
+    list = b->order_list;
+  } else if (sched != CUBFF_SCHED_PLAIN) {
+    return fail(CUBFF_EINVAL, "unknown scheduling mode");
+  }
+  if (ev0) HIP_TRY(hipEventRecord(ev0, st));
+  for (const auto &l : b->lists)
+    HIP_TRY(batch_interp(st, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
+  if (ev1) HIP_TRY(hipEventRecord(ev1, st));
+  if (sched == CUBFF_SCHED_LEARNED) HIP_TRY(sched_record(st, b->sched, b->d));
+  HIP_TRY(batch_end_epoch(st, b->d));
+  return CUBFF_OK;
+}
+
+int set_epoch(cubff_batch *b, uint64_t epoch) {
+  HIP_TRY(hipMemcpyAsync(b->d.epoch, &epoch, 8, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cubff_batch_last_error(void) { return g_err.c_str(); }
+
+int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
+                       const uint64_t *num_programs, int device) {
+  if (!out) return fail(CUBFF_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (num_soups == 0 || !langs || !num_programs)
+    return fail(CUBFF_EINVAL, "need at least one soup");
+  uint64_t total = 0;
+  for (uint32_t s = 0; s < num_soups; s++) {
+    if (langs[s] < 0 || langs[s] >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
+    if (num_programs[s] == 0 || num_programs[s] % 2)
+      return fail(CUBFF_EINVAL, "num_programs must be even and > 0");
+    total += num_programs[s];
+  }
+  if (total >= (1ULL << 32)) return fail(CUBFF_EINVAL, "total programs must be < 2^32");
+  HIP_TRY(hipSetDevice(device));
+  cubff_batch *b = new cubff_batch();
+  b->device = device;
+  b->num_soups = num_soups;
+  b->langs.assign(langs, langs + num_soups);
+  b->n.assign(num_programs, num_programs + num_soups);
+  b->prog_base.resize(num_soups + 1);
+  b->slot_base.resize(num_soups + 1);
+  b->prog_base[0] = b->slot_base[0] = 0;
+  for (uint32_t s = 0; s < num_soups; s++) {
+    b->prog_base[s + 1] = b->prog_base[s] + b->n[s];
+    b->slot_base[s + 1] = b->slot_base[s] + b->n[s] / 2;
+  }
+  const uint64_t P = b->prog_base[num_soups], S = b->slot_base[num_soups];
+  BatchDev &d = b->d;
+  d.num_soups = num_soups;
+  d.total_programs = P;
+  d.total_slots = S;
+  int r;
+  uint32_t *cnt = nullptr;
+  unsigned long long *ctr = nullptr;
+  if ((r = alloc(&b->prog_base_d, num_soups + 1)) || (r = alloc(&b->slot_base_d, num_soups + 1)) ||
+      (r = alloc(&b->sm_seed_d, num_soups)) || (r = alloc(&b->num_indices_d, num_soups)) ||
+      (r = alloc(&b->prob_d, num_soups)) || (r = alloc(&d.soup, P * kTape)) ||
+      (r = alloc(&d.shuf, P)) || (r = alloc(&d.stage, S * kTape2)) ||
+      (r = alloc(&d.tgt, 2 * S)) || (r = alloc(&d.pair_ops, S)) ||
+      (r = alloc(&d.pair_steps, S)) || (r = alloc(&ctr, (uint64_t)num_soups * kNumBatchCounters)) ||
+      (r = alloc(&d.epoch, 1)) || (r = alloc(&d.fy_j, P)) || (r = alloc(&cnt, P + 1)) ||
+      (r = alloc(&d.fy_base, P)) || (r = alloc(&d.fy_cursor, P)) ||
+      (r = alloc(&d.fy_items, P)) || (r = alloc(&d.fy_first, P)) ||
+      (r = alloc(&d.fy_succ, P)) || (r = alloc(&b->plain_list, S)) ||
+      (r = alloc(&b->order_list, S))) {
+    d.fy_cnt = cnt;
+    d.counters = ctr;
+    std::string keep = g_err;
+    cubff_batch_destroy(b);
+    g_err = keep;
+    return r;
+  }
+  d.fy_cnt = cnt;
+  d.counters = ctr;
+  d.prog_base = b->prog_base_d;
+  d.slot_base = b->slot_base_d;
+  d.sm_seed = b->sm_seed_d;
+  d.prob = b->prob_d;
+  d.num_indices = b->num_indices_d;
+  hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    cubff_batch_destroy(b);
+    return fail(CUBFF_EHIP, "hipStreamCreate failed");
+  }
+  // Slot lists grouped by language, in soup order within each language.
+  std::vector<uint32_t> plain;
+  plain.reserve(S);
+  for (int lang = 0; lang < kNumLangs; lang++) {
+    const uint64_t begin = plain.size();
+    for (uint32_t s = 0; s < num_soups; s++)
+      if (b->langs[s] == lang)
+        for (uint64_t g = b->slot_base[s]; g < b->slot_base[s + 1]; g++) plain.push_back((uint32_t)g);
+    if (plain.size() > begin) b->lists.push_back({lang, begin, plain.size() - begin});
+  }
+  std::vector<uint64_t> ni(num_soups);
+  for (uint32_t s = 0; s < num_soups; s++) ni[s] = b->n[s] / 2;
+  const uint64_t zero = 0;
+  e = hipMemcpy(b->prog_base_d, b->prog_base.data(), (num_soups + 1) * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->slot_base_d, b->slot_base.data(), (num_soups + 1) * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->num_indices_d, ni.data(), num_soups * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(b->plain_list, plain.data(), S * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d.epoch, &zero, 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = batch_fill_u32(b->stream, reinterpret_cast<uint32_t *>(d.counters), 0,
+                       2ull * num_soups * kNumBatchCounters);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    cubff_batch_destroy(b);
+    return fail(CUBFF_EHIP, std::string("batch init failed: ") + hipGetErrorString(e));
+  }
+  *out = b;
+  return CUBFF_OK;
+}
+
+int cubff_batch_destroy(cubff_batch *b) {
+  if (!b) return CUBFF_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  sched_free(b->sched);
+  release(b->prog_base_d);
+  release(b->slot_base_d);
+  release(b->sm_seed_d);
+  release(b->num_indices_d);
+  release(b->prob_d);
+  release(b->d.soup);
+  release(b->d.shuf);
+  release(b->d.stage);
+  release(b->d.tgt);
+  release(b->d.pair_ops);
+  release(b->d.pair_steps);
+  release(b->d.counters);
+  release(b->d.epoch);
+  release(b->d.fy_j);
+  release(b->d.fy_cnt);
+  release(b->d.fy_base);
+  release(b->d.fy_cursor);
+  release(b->d.fy_items);
+  release(b->d.fy_first);
+  release(b->d.fy_succ);
+  release(b->plain_list);
+  release(b->order_list);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CUBFF_OK;
+}
+
+int cubff_batch_init(cubff_batch *b, const uint64_t *params_seeds,
+                     const uint32_t *mutation_probs, int zero_init) {
+  int r;
+  if ((r = check(b))) return r;
+  if (!params_seeds || !mutation_probs) return fail(CUBFF_EINVAL, "null seeds / probabilities");
+  std::vector<uint64_t> sm(b->num_soups);
+  for (uint32_t s = 0; s < b->num_soups; s++) sm[s] = splitmix64(params_seeds[s]);
+  HIP_TRY(hipMemcpyAsync(b->sm_seed_d, sm.data(), b->num_soups * 8, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->prob_d, mutation_probs, b->num_soups * 4, hipMemcpyHostToDevice, b->stream));
+  for (uint32_t s = 0; s < b->num_soups; s++)
+    HIP_TRY(launch_init(b->stream, b->d.soup + b->prog_base[s] * kTape, b->n[s],
+                        epoch_seed_pre(sm[s], 0), b->n[s], zero_init));
+  HIP_TRY(batch_fill_u32(b->stream, reinterpret_cast<uint32_t *>(b->d.counters), 0,
+                         2ull * b->num_soups * kNumBatchCounters));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->have_seeds = true;
+  return CUBFF_OK;
+}
+
+int cubff_batch_set_launch(cubff_batch *b, int wg, uint32_t tapes_per_lane) {
+  int r;
+  if ((r = check(b))) return r;
+  if (wg != 64 && wg != 128 && wg != 256) return fail(CUBFF_EINVAL, "workgroup size must be 64, 128 or 256");
+  if (tapes_per_lane < 1 || tapes_per_lane > 64) return fail(CUBFF_EINVAL, "tapes_per_lane must be in [1, 64]");
+  b->wg = wg;
+  b->per_lane = tapes_per_lane;
+  return CUBFF_OK;
+}
+
+int cubff_batch_get_launch(cubff_batch *b, int *wg, uint32_t *tapes_per_lane) {
+  int r;
+  if ((r = check(b))) return r;
+  if (wg) *wg = b->wg;
+  if (tapes_per_lane) *tapes_per_lane = b->per_lane;
+  return CUBFF_OK;
+}
+
+int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
+                           int sched, int use_graph) {
+  int r;
+  if ((r = check(b))) return r;
+  if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
+  if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
+    return fail(CUBFF_ENOMEM, "scheduler allocation failed");
+  if ((r = set_epoch(b, first_epoch))) return r;
+  if (!use_graph) {
+    for (uint64_t i = 0; i < count; i++)
+      if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
+    return CUBFF_OK;
+  }
+  if (!b->graph || b->graph_sched != sched || b->graph_wg != b->wg ||
+      b->graph_per_lane != b->per_lane) {
+    drop_graph(b);
+    hipGraph_t g;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    r = enqueue_epoch(b, sched, nullptr, nullptr);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r) return r;
+    if (e != hipSuccess) return fail(CUBFF_EHIP, "graph capture failed");
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(CUBFF_EHIP, "graph instantiate failed");
+    b->graph_sched = sched;
+    b->graph_wg = b->wg;
+    b->graph_per_lane = b->per_lane;
+  }
+  for (uint64_t i = 0; i < count; i++) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+  return CUBFF_OK;
+}
+
+int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t count,
+                                 int sched, float *total_ms, float *interp_ms) {
+  int r;
+  if ((r = check(b))) return r;
+  if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
+  if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
+    return fail(CUBFF_ENOMEM, "scheduler allocation failed");
+  if ((r = set_epoch(b, first_epoch))) return r;
+  std::vector<hipEvent_t> ev(2 * count + 2);
+  for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+  HIP_TRY(hipEventRecord(ev[0], b->stream));
+  for (uint64_t i = 0; i < count; i++)
+    if ((r = enqueue_epoch(b, sched, ev[2 + 2 * i], ev[3 + 2 * i]))) return r;
+  HIP_TRY(hipEventRecord(ev[1], b->stream));
+  HIP_TRY(hipEventSynchronize(ev[1]));
+  float tot = 0, it = 0;
+  HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev[1]));
+  for (uint64_t i = 0; i < count; i++) {
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ev[2 + 2 * i], ev[3 + 2 * i]));
+    it += ms;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (total_ms) *total_ms = tot;
+  if (interp_ms) *interp_ms = it;
+  return CUBFF_OK;
+}
+
+int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch, uint64_t epochs_per_config,
+                         int sched, int *best_wg, uint32_t *best_tapes_per_lane,
+                         float *ms_table) {
+  int r;
+  if ((r = check(b))) return r;
+  static const int kWg[3] = {64, 128, 256};
+  float best = 1e30f;
+  int bw = b->wg;
+  uint32_t bt = b->per_lane;
+  uint64_t e = first_epoch;
+  for (int wi = 0; wi < 3; wi++) {
+    for (uint32_t tpl = 1; tpl <= 4; tpl++) {
+      b->wg = kWg[wi];
+      b->per_lane = tpl;
+      float tot = 0, it = 0;
+      if ((r = cubff_batch_run_epochs_timed(b, e, epochs_per_config, sched, &tot, &it))) return r;
+      e += epochs_per_config;
+      if (ms_table) ms_table[wi * 4 + (tpl - 1)] = tot / epochs_per_config;
+      if (tot < best) {
+        best = tot;
+        bw = kWg[wi];
+        bt = tpl;
+      }
+    }
+  }
+  b->wg = bw;
+  b->per_lane = bt;
+  if (best_wg) *best_wg = bw;
+  if (best_tapes_per_lane) *best_tapes_per_lane = bt;
+  return CUBFF_OK;
+}
+
+int cubff_batch_sync(cubff_batch *b) {
+  int r;
+  if ((r = check(b))) return r;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+int cubff_batch_read_soup(cubff_batch *b, uint32_t soup, uint8_t *dst) {
+  int r;
+  if ((r = check(b))) return r;
+  if (soup >= b->num_soups || !dst) return fail(CUBFF_EINVAL, "bad soup index or null destination");
+  HIP_TRY(hipMemcpyAsync(dst, b->d.soup + b->prog_base[soup] * kTape, b->n[soup] * kTape,
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+int cubff_batch_write_soup(cubff_batch *b, uint32_t soup, const uint8_t *src) {
+  int r;
+  if ((r = check(b))) return r;
+  if (soup >= b->num_soups || !src) return fail(CUBFF_EINVAL, "bad soup index or null source");
+  HIP_TRY(hipMemcpyAsync(b->d.soup + b->prog_base[soup] * kTape, src, b->n[soup] * kTape,
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+int cubff_batch_read_counters(cubff_batch *b, uint32_t soup, uint64_t *ops,
+                              uint64_t *num_runs, uint64_t *steps) {
+  int r;
+  if ((r = check(b))) return r;
+  if (soup >= b->num_soups) return fail(CUBFF_EINVAL, "bad soup index");
+  unsigned long long h[kNumBatchCounters];
+  HIP_TRY(hipMemcpyAsync(h, b->d.counters + (uint64_t)soup * kNumBatchCounters, sizeof(h),
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (ops) *ops = h[0];
+  if (num_runs) *num_runs = h[1];
+  if (steps) *steps = h[2];
+  return CUBFF_OK;
+}
+
+int cubff_batch_reset_counters(cubff_batch *b) {
+  int r;
+  if ((r = check(b))) return r;
+  HIP_TRY(batch_fill_u32(b->stream, reinterpret_cast<uint32_t *>(b->d.counters), 0,
+                         2ull * b->num_soups * kNumBatchCounters));
+  return CUBFF_OK;
+}
+
+int cubff_batch_read_pair_stats(cubff_batch *b, uint32_t soup, uint32_t *ops,
+                                uint32_t *steps) {
+  int r;
+  if ((r = check(b))) return r;
+  if (soup >= b->num_soups) return fail(CUBFF_EINVAL, "bad soup index");
+  const uint64_t s0 = b->slot_base[soup], ns = b->slot_base[soup + 1] - s0;
+  if (ops) HIP_TRY(hipMemcpyAsync(ops, b->d.pair_ops + s0, ns * 4, hipMemcpyDeviceToHost, b->stream));
+  if (steps) HIP_TRY(hipMemcpyAsync(steps, b->d.pair_steps + s0, ns * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s) {
+  int r;
+  if ((r = check(b))) return r;
+  if (soup >= b->num_soups || !s) return fail(CUBFF_EINVAL, "bad soup index or null destination");
+  HIP_TRY(hipMemcpyAsync(s, b->d.shuf + b->prog_base[soup], b->n[soup] * 4,
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return CUBFF_OK;
+}
+
+}  // extern "C"

@@ -155,6 +155,56 @@ int cubff_sync(cubff_ctx *ctx);
 uint64_t cubff_splitmix64(uint64_t x);
 uint64_t cubff_epoch_seed(uint64_t params_seed, uint64_t x);
 
+
+/* ------------------------------------------------------------------------ */
+/* Batched soups.  No reference counterpart: the reference runs one soup per
+ * process.  A batch advances many independent soups (each with its own
+ * language, size, params seed and mutation threshold) with one set of
+ * launches per epoch; every soup's bytes, pairings and counters equal those
+ * of a separate single-soup run (RunSimulation, common_language.h:344-597,
+ * with Fisher-Yates pairing). */
+typedef struct cubff_batch cubff_batch;
+
+#define CUBFF_SCHED_PLAIN 0    /* slots in soup order */
+#define CUBFF_SCHED_LEARNED 1  /* MLP-predicted length, longest bucket first */
+
+const char *cubff_batch_last_error(void);
+int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
+                       const uint64_t *num_programs, int device);
+int cubff_batch_destroy(cubff_batch *b);
+/* Sets each soup's params seed and mutation threshold (of 2^30) and fills it
+ * with InitPrograms(S(0)) (common_language.h:371-373); clears counters. */
+int cubff_batch_init(cubff_batch *b, const uint64_t *params_seeds,
+                     const uint32_t *mutation_probs, int zero_init);
+/* Interpreter launch shape: workgroup size (64, 128 or 256 lanes) and tapes
+ * evaluated one after another by each lane. */
+int cubff_batch_set_launch(cubff_batch *b, int wg, uint32_t tapes_per_lane);
+int cubff_batch_get_launch(cubff_batch *b, int *wg, uint32_t *tapes_per_lane);
+/* `count` epochs from first_epoch; use_graph replays one captured epoch. */
+int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
+                           int sched, int use_graph);
+/* As above with HIP events: device time of the whole run and of the
+ * interpreter launches alone.  Blocks until done. */
+int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch,
+                                 uint64_t count, int sched, float *total_ms,
+                                 float *interp_ms);
+/* Runs epochs_per_config live epochs under each of the 12 launch shapes
+ * (workgroup 64/128/256 x 1-4 tapes per lane), keeps the fastest and reports
+ * it; ms_table (nullable, 12 floats) receives ms/epoch per shape.  Results
+ * do not depend on the launch shape, so the tuning epochs are real epochs. */
+int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch,
+                         uint64_t epochs_per_config, int sched, int *best_wg,
+                         uint32_t *best_tapes_per_lane, float *ms_table);
+int cubff_batch_sync(cubff_batch *b);
+int cubff_batch_read_soup(cubff_batch *b, uint32_t soup, uint8_t *dst);
+int cubff_batch_write_soup(cubff_batch *b, uint32_t soup, const uint8_t *src);
+int cubff_batch_read_counters(cubff_batch *b, uint32_t soup, uint64_t *ops,
+                              uint64_t *num_runs, uint64_t *steps);
+int cubff_batch_reset_counters(cubff_batch *b);
+int cubff_batch_read_pair_stats(cubff_batch *b, uint32_t soup, uint32_t *ops,
+                                uint32_t *steps);
+int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,85 @@
+"""GPU parity of the batched path: every soup of a batch equals a separate
+oracle run with its own language, size, seed and mutation threshold, under
+every launch shape, with and without graph replay."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from cubff_amd import capi
+    capi.load()
+    return capi
+
+
+def _oracle_run(lang, n, seed, prob, epochs):
+    ref = O.Soup(lang, n, seed, prob)
+    hist = []
+    for e in range(epochs):
+        tot, po, ps = ref.step(O.pairing_shuffle(n, seed, e))
+        hist.append((ref.soup.copy(), po.copy(), ps.copy(), tot))
+    return hist
+
+
+SPEC = [  # (lang, n, seed, prob)
+    ("bff_noheads", 4096, 0, 10737),
+    ("bff_noheads", 1024, 1, 262144),
+    ("forth", 2048, 2, 1073742),
+    ("bff", 512, 3, 262144),
+    ("subleq", 256, 4, 262144),
+    ("bff_noheads", 2048, 5, 0),
+    ("rsubleq4", 1024, 6, 131072),
+    ("forthtrivial_reset", 512, 7, 262144),
+]
+
+
+@pytest.mark.parametrize("wg,tpl,graph", [(256, 1, False), (64, 3, False), (128, 2, True),
+                                          (256, 4, True)])
+def test_batch_matches_oracle(capi, wg, tpl, graph):
+    E = 6
+    refs = [_oracle_run(l, n, s, p, E) for (l, n, s, p) in SPEC]
+    with capi.Batch([x[0] for x in SPEC], [x[1] for x in SPEC]) as b:
+        b.init([x[2] for x in SPEC], [x[3] for x in SPEC])
+        b.set_launch(wg, tpl)
+        cum = [0] * len(SPEC)
+        for e in range(E):
+            b.run_epochs(e, 1, capi.SCHED_PLAIN, use_graph=graph)
+            for i, (l, n, s, p) in enumerate(SPEC):
+                soup, po, ps, tot = refs[i][e]
+                cum[i] += tot
+                assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, e)), (i, e)
+                assert np.array_equal(b.read_soup(i), soup), (i, e)
+                ops, steps = b.read_pair_stats(i)
+                assert np.array_equal(ops, po) and np.array_equal(steps, ps), (i, e)
+                assert b.read_counters(i)[:2] == (cum[i], (e + 1) * (n // 2)), (i, e)
+
+
+def test_batch_graph_multi_epoch(capi):
+    # A captured epoch replayed 10 times equals 10 direct epochs.
+    spec = SPEC[:3]
+    with capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as a, \
+            capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as b:
+        for x in (a, b):
+            x.init([s[2] for s in spec], [s[3] for s in spec])
+        a.run_epochs(0, 10, capi.SCHED_PLAIN, use_graph=True)
+        b.run_epochs(0, 10, capi.SCHED_PLAIN, use_graph=False)
+        for i in range(len(spec)):
+            assert np.array_equal(a.read_soup(i), b.read_soup(i))
+            assert a.read_counters(i) == b.read_counters(i)
+
+
+def test_batch_autotune_keeps_results(capi):
+    spec = SPEC[:2]
+    with capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as a, \
+            capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as b:
+        for x in (a, b):
+            x.init([s[2] for s in spec], [s[3] for s in spec])
+        wg, tpl, table = a.autotune(0, 1)
+        assert wg in (64, 128, 256) and 1 <= tpl <= 4 and all(t > 0 for t in table)
+        b.run_epochs(0, 12)
+        for i in range(len(spec)):
+            assert np.array_equal(a.read_soup(i), b.read_soup(i))
