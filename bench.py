@@ -83,6 +83,49 @@ def cpu_baseline(lang: str, n: int, seed: int, prob: int, budget_s: float):
             "sample": f"oracle OpenMP port, epochs 1..{e} of the same workload"}
 
 
+BATCH_SIZES = [4096, 16384, 65536, 131072]
+
+
+def batched_config():
+    """64 bff_noheads soups: seeds 0..63, sizes cycling 2^12, 2^14, 2^16,
+    2^17, mutation probability log-spaced 1e-5 .. 1e-3."""
+    S = 64
+    ns = [BATCH_SIZES[k % 4] for k in range(S)]
+    probs = [int(round(10 ** (-5 + 2 * k / 63) * (1 << 30))) for k in range(S)]
+    return ["bff_noheads"] * S, ns, list(range(S)), probs
+
+
+def run_batched(capi, sched, epochs, warmup, autotune):
+    langs, ns, seeds, probs = batched_config()
+    with capi.Batch(langs, ns) as b:
+        b.init(seeds, probs)
+        tuned = None
+        e0 = 0
+        if autotune:
+            wg, tpl, table = b.autotune(0, 1, sched)
+            tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
+            e0 = 12
+        b.run_epochs(e0, warmup, sched)
+        b.sync()
+        b.reset_counters()
+        tot, it = b.run_epochs_timed(e0 + warmup, epochs, sched)
+        ops = steps = 0
+        for i in range(len(ns)):
+            o, _, st = b.read_counters(i)
+            ops += o
+            steps += st
+        wg, tpl = b.get_launch()
+        out = {"instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / epochs,
+               "interp_ms_per_epoch": it / epochs, "epochs_timed": epochs,
+               "total_programs": sum(ns), "soups": len(ns), "wg": wg,
+               "tapes_per_lane": tpl, "steps": steps, "ops": ops, "autotune": tuned}
+        if sched == capi.SCHED_LEARNED:
+            tr, loss, buckets = b.sched_stats()
+            out["sched"] = {"trainings": tr, "last_loss": loss,
+                            "bucket_slots": [int(x) for x in buckets]}
+        return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +136,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="wall-clock budget of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--batched-epochs", type=int, default=32,
+                    help="timed epochs of the 64-soup batched runs (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the epoch runs on a single MI355X and "
@@ -156,6 +201,17 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.batched_epochs > 0:
+        plain = run_batched(capi, capi.SCHED_PLAIN, args.batched_epochs, 8, autotune=True)
+        learned = run_batched(capi, capi.SCHED_LEARNED, args.batched_epochs, 8, autotune=False)
+        line["batched"] = {
+            "workload": ("64 bff_noheads soups, seeds 0..63, sizes cycling 2^12/2^14/2^16/"
+                         "2^17 programs, mutation log-spaced 1e-5..1e-3, Fisher-Yates"),
+            "plain": plain, "learned_sched": learned,
+            "learned_vs_plain_single_soup": learned["instr_per_s"] / line["value"],
+            "plain_batched_vs_plain_single_soup": plain["instr_per_s"] / line["value"],
+            "learned_vs_plain_batched": learned["instr_per_s"] / plain["instr_per_s"],
+        }
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(lang, n, seed, prob, args.cpu_seconds)
     print(json.dumps(line))

@@ -32,7 +32,10 @@ EXPORTS = [
     "cubff_batch_autotune", "cubff_batch_sync", "cubff_batch_read_soup",
     "cubff_batch_write_soup", "cubff_batch_read_counters",
     "cubff_batch_reset_counters", "cubff_batch_read_pair_stats",
-    "cubff_batch_read_shuffle",
+    "cubff_batch_read_shuffle", "cubff_batch_sched_stats",
+    "cubff_mlp_last_error", "cubff_mlp_num_params", "cubff_mlp_create",
+    "cubff_mlp_destroy", "cubff_mlp_set_params", "cubff_mlp_get_params",
+    "cubff_mlp_forward", "cubff_mlp_gradients", "cubff_mlp_train",
 ]
 
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
@@ -40,6 +43,7 @@ SCHED_PLAIN, SCHED_LEARNED = 0, 1
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
+_f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 _u64 = ctypes.c_uint64
 _i = ctypes.c_int
@@ -109,6 +113,17 @@ def load():
         "cubff_batch_reset_counters": (_i, [_vp]),
         "cubff_batch_read_pair_stats": (_i, [_vp, ctypes.c_uint32, _u32p, _u32p]),
         "cubff_batch_read_shuffle": (_i, [_vp, ctypes.c_uint32, _u32p]),
+        "cubff_batch_sched_stats": (_i, [_vp, _u64p, ctypes.POINTER(ctypes.c_float), _u32p]),
+        "cubff_mlp_last_error": (ctypes.c_char_p, []),
+        "cubff_mlp_num_params": (_u64, []),
+        "cubff_mlp_create": (_i, [ctypes.POINTER(_vp), _u64, _u64, _i]),
+        "cubff_mlp_destroy": (_i, [_vp]),
+        "cubff_mlp_set_params": (_i, [_vp, _f32p]),
+        "cubff_mlp_get_params": (_i, [_vp, _f32p]),
+        "cubff_mlp_forward": (_i, [_vp, _u8p, _u64, _f32p]),
+        "cubff_mlp_gradients": (_i, [_vp, _u8p, _f32p, _u64, _f32p, _f32p]),
+        "cubff_mlp_train": (_i, [_vp, _u8p, _f32p, _u64, ctypes.c_uint32, ctypes.c_float,
+                                 _f32p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -376,3 +391,75 @@ class Batch:
         s = np.zeros(self.n[i], dtype=np.uint32)
         _bcheck(load().cubff_batch_read_shuffle(self._h, i, _ptr(s, _u32p)))
         return s
+
+    def sched_stats(self):
+        """(trainings, last training loss, slots per length bucket)."""
+        t, l = ctypes.c_uint64(0), ctypes.c_float(0)
+        c = np.zeros(8, dtype=np.uint32)
+        _bcheck(load().cubff_batch_sched_stats(self._h, ctypes.byref(t), ctypes.byref(l),
+                                               _ptr(c, _u32p)))
+        return int(t.value), float(l.value), c
+
+
+def _mcheck(rc: int):
+    if rc != 0:
+        raise CubffError(load().cubff_mlp_last_error().decode())
+
+
+class Mlp:
+    """The scheduler's MLP cost model on its own (cubff_mlp)."""
+
+    def __init__(self, capacity: int, init_seed: int = 0x5eed, device: int = 0):
+        h = _vp()
+        _mcheck(load().cubff_mlp_create(ctypes.byref(h), capacity, init_seed, device))
+        self._h = h
+        self.num_params = int(load().cubff_mlp_num_params())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load().cubff_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, dtype=np.float32)
+        _mcheck(load().cubff_mlp_get_params(self._h, _ptr(out, _f32p)))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _mcheck(load().cubff_mlp_set_params(self._h, _ptr(a, _f32p)))
+
+    def forward(self, tapes: np.ndarray) -> np.ndarray:
+        t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
+        out = np.zeros(t.shape[0], dtype=np.float32)
+        _mcheck(load().cubff_mlp_forward(self._h, _ptr(t, _u8p), t.shape[0], _ptr(out, _f32p)))
+        return out
+
+    def gradients(self, tapes: np.ndarray, targets: np.ndarray):
+        t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
+        y = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = ctypes.c_float(0)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        _mcheck(load().cubff_mlp_gradients(self._h, _ptr(t, _u8p), _ptr(y, _f32p), t.shape[0],
+                                           ctypes.byref(loss), _ptr(g, _f32p)))
+        return float(loss.value), g
+
+    def train(self, tapes: np.ndarray, targets: np.ndarray, steps: int, lr: float = 1e-3):
+        t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
+        y = np.ascontiguousarray(targets, dtype=np.float32)
+        losses = np.zeros(steps, dtype=np.float32)
+        _mcheck(load().cubff_mlp_train(self._h, _ptr(t, _u8p), _ptr(y, _f32p), t.shape[0],
+                                       steps, lr, _ptr(losses, _f32p)))
+        return losses

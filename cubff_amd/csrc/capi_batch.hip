@@ -100,8 +100,22 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   for (const auto &l : b->lists)
     HIP_TRY(batch_interp(st, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
   if (ev1) HIP_TRY(hipEventRecord(ev1, st));
-  if (sched == CUBFF_SCHED_LEARNED) HIP_TRY(sched_record(st, b->sched, b->d));
   HIP_TRY(batch_end_epoch(st, b->d));
+  return CUBFF_OK;
+}
+
+// After an epoch under the learned scheduler: train on its recorded ops on
+// the first epoch and every kTrainInterval epochs after that.
+constexpr uint64_t kTrainInterval = 16;
+constexpr int kTrainSteps = 16;
+constexpr float kTrainLr = 2e-3f;
+
+int after_epoch(cubff_batch *b, int sched) {
+  if (sched != CUBFF_SCHED_LEARNED) return CUBFF_OK;
+  if (b->sched.epochs_seen++ % kTrainInterval == 0) {
+    float loss = 0;
+    HIP_TRY(sched_train(b->stream, b->sched, b->d, kTrainSteps, kTrainLr, &loss));
+  }
   return CUBFF_OK;
 }
 
@@ -291,8 +305,10 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
     return fail(CUBFF_ENOMEM, "scheduler allocation failed");
   if ((r = set_epoch(b, first_epoch))) return r;
   if (!use_graph) {
-    for (uint64_t i = 0; i < count; i++)
+    for (uint64_t i = 0; i < count; i++) {
       if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
+      if ((r = after_epoch(b, sched))) return r;
+    }
     return CUBFF_OK;
   }
   if (!b->graph || b->graph_sched != sched || b->graph_wg != b->wg ||
@@ -311,7 +327,10 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
     b->graph_wg = b->wg;
     b->graph_per_lane = b->per_lane;
   }
-  for (uint64_t i = 0; i < count; i++) HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+  for (uint64_t i = 0; i < count; i++) {
+    HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    if ((r = after_epoch(b, sched))) return r;
+  }
   return CUBFF_OK;
 }
 
@@ -326,8 +345,10 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t 
   std::vector<hipEvent_t> ev(2 * count + 2);
   for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], b->stream));
-  for (uint64_t i = 0; i < count; i++)
+  for (uint64_t i = 0; i < count; i++) {
     if ((r = enqueue_epoch(b, sched, ev[2 + 2 * i], ev[3 + 2 * i]))) return r;
+    if ((r = after_epoch(b, sched))) return r;
+  }
   HIP_TRY(hipEventRecord(ev[1], b->stream));
   HIP_TRY(hipEventSynchronize(ev[1]));
   float tot = 0, it = 0;
@@ -372,6 +393,26 @@ int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch, uint64_t epochs_p
   b->per_lane = bt;
   if (best_wg) *best_wg = bw;
   if (best_tapes_per_lane) *best_tapes_per_lane = bt;
+  return CUBFF_OK;
+}
+
+int cubff_batch_sched_stats(cubff_batch *b, uint64_t *trainings, float *last_loss,
+                            uint32_t *bucket_counts) {
+  int r;
+  if ((r = check(b))) return r;
+  if (trainings) *trainings = b->sched.trainings;
+  if (last_loss) *last_loss = b->sched.last_loss;
+  if (bucket_counts) {
+    for (int i = 0; i < kBuckets; i++) bucket_counts[i] = 0;
+    if (b->sched.ready) {
+      std::vector<uint32_t> c(b->lists.size() * 2 * kBuckets);
+      HIP_TRY(hipMemcpyAsync(c.data(), b->sched.cnt, c.size() * 4, hipMemcpyDeviceToHost,
+                             b->stream));
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      for (size_t li = 0; li < b->lists.size(); li++)
+        for (int i = 0; i < kBuckets; i++) bucket_counts[i] += c[li * 2 * kBuckets + i];
+    }
+  }
   return CUBFF_OK;
 }
 

@@ -1,0 +1,71 @@
+// MLP cost model of the learned scheduler.
+//
+//   x   = [byte histogram | hashed byte-bigram histogram] / 128   (512, bf16)
+//   p   = x E                      (E: 512 x 256, pooled byte embedding)
+//   a1  = relu(p W1 + b1)          (256)
+//   a2  = relu(a1 W2 + b2)         (256)
+//   y   = a2 . w3 + b3             predicted log2(ops + 1)
+//
+// bf16 weights and activations, fp32 accumulation on MFMA; fp32 master
+// weights with Adam.  All GEMMs are hand-written (mlp.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cubff {
+
+constexpr int kMlpIn = 512, kMlpHid = 256;
+// Offsets of the fp32 parameter vector.
+constexpr uint64_t kOffE = 0;
+constexpr uint64_t kOffW1 = kOffE + (uint64_t)kMlpIn * kMlpHid;
+constexpr uint64_t kOffB1 = kOffW1 + (uint64_t)kMlpHid * kMlpHid;
+constexpr uint64_t kOffW2 = kOffB1 + kMlpHid;
+constexpr uint64_t kOffB2 = kOffW2 + (uint64_t)kMlpHid * kMlpHid;
+constexpr uint64_t kOffW3 = kOffB2 + kMlpHid;
+constexpr uint64_t kOffB3 = kOffW3 + kMlpHid;
+constexpr uint64_t kNumParams = kOffB3 + 1;
+
+struct Mlp {
+  uint64_t capacity = 0;  // rows the activation buffers hold (multiple of 128)
+  float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+  // bf16 weights: W (row-major [in][out]) and WT ([out][in]) per layer.
+  uint16_t *e_w = nullptr, *e_wt = nullptr, *w1 = nullptr, *w1t = nullptr;
+  uint16_t *w2 = nullptr, *w2t = nullptr;
+  uint16_t *ew1t = nullptr;            // folded E W1 as [256][512], inference only
+  bool folded_valid = false;
+  uint16_t *x = nullptr;               // [capacity, 512] features
+  uint16_t *p = nullptr, *a1 = nullptr, *a2 = nullptr;  // [capacity, 256]
+  uint16_t *d2 = nullptr, *d1 = nullptr, *dp = nullptr; // [capacity, 256]
+  float *y = nullptr, *target = nullptr;                 // [capacity]
+  float *loss = nullptr;               // device scalar
+  uint64_t adam_t = 0;
+};
+
+hipError_t mlp_alloc(Mlp &m, uint64_t capacity);
+void mlp_free(Mlp &m);
+// Deterministic initialisation from a seed (SplitMix64 uniform, scaled per
+// layer); uploads fp32 and bf16 copies.
+hipError_t mlp_init(Mlp &m, uint64_t seed, hipStream_t st);
+hipError_t mlp_set_params(Mlp &m, const float *host, hipStream_t st);
+hipError_t mlp_get_params(const Mlp &m, float *host, hipStream_t st);
+// Features of `count` 128-byte tapes (device, 128 bytes apart) into m.x.
+hipError_t mlp_features(Mlp &m, const uint8_t *tapes, uint64_t count, hipStream_t st);
+// Same for selected slots of a stage buffer: row i <- tape rows[i].
+hipError_t mlp_features_rows(Mlp &m, const uint8_t *stage, const uint32_t *rows,
+                             uint64_t count, hipStream_t st);
+// Forward over m.x[0..count) into m.y (keeps p, a1, a2 for backward).
+hipError_t mlp_forward(Mlp &m, uint64_t count, hipStream_t st);
+// Inference over m.x[0..count) into m.y with the first two linear maps
+// folded (x (E W1)): one GEMM less; the fold is recomputed after updates.
+hipError_t mlp_infer(Mlp &m, uint64_t count, hipStream_t st);
+hipError_t mlp_fold(Mlp &m, hipStream_t st);
+// Loss 0.5 * mean((y - target)^2) and all gradients into m.grads for
+// m.x / m.target rows [0..count); *m.loss receives the loss.
+hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st);
+hipError_t mlp_adam(Mlp &m, float lr, hipStream_t st);
+// target[i] = log2(ops[rows[i]] + 1).
+hipError_t mlp_targets_from_ops(Mlp &m, const uint32_t *ops, const uint32_t *rows,
+                                uint64_t count, hipStream_t st);
+
+}  // namespace cubff

@@ -1,7 +1,8 @@
-// Learned scheduler of the batched path: an MLP cost model predicts each
-// staged pair's interaction length; pairs are binned into 8 length buckets,
-// longest first, before the interpreter launch.  Scheduling only reorders
-// work: results are bit-identical with and without it.
+// Learned scheduler of the batched path: the MLP cost model (mlp.h) predicts
+// each staged pair's interaction length; pairs are binned into 8 length
+// buckets (quantiles of the epoch's predictions), longest bucket first,
+// before the interpreter launch.  Scheduling
+// only reorders work: results are bit-identical with and without it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,23 +10,26 @@
 #include <stdint.h>
 
 #include "batch.h"
+#include "mlp.h"
 
 namespace cubff {
 
+constexpr int kBuckets = 8;
+
 struct SchedState {
   bool ready = false;
-  uint64_t capacity = 0;  // slots the buffers hold
-  // MLP parameters (fp32 master copies + bf16 copies used by the GEMMs).
-  float *params = nullptr;        // all fp32 parameters, one allocation
-  float *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-  uint16_t *params_bf16 = nullptr;
-  // Per-slot activations / outputs.
-  uint16_t *feat = nullptr;       // [capacity, 256] bf16 byte histograms
-  uint16_t *act1 = nullptr, *act2 = nullptr;  // [capacity, 256] bf16
-  float *pred = nullptr;          // [capacity] predicted log2(steps + 1)
-  uint32_t *bucket_cnt = nullptr; // per list x 8 buckets (+ cursors)
-  uint64_t trained_epochs = 0;
+  Mlp mlp;
+  uint32_t *bucket = nullptr;      // [total_slots] bucket of each slot (0 = longest)
+  uint32_t *cnt = nullptr;         // [num_lists * 2 * kBuckets] counts + cursors
+  uint32_t *hist = nullptr;        // prediction histogram
+  uint32_t *thr = nullptr;         // bucket thresholds (prediction bins)
+  uint32_t *train_rows = nullptr;  // [kTrainRows]
+  uint64_t train_rows_n = 0;
+  uint64_t epochs_seen = 0, trainings = 0;
+  float last_loss = 0.f;
 };
+
+constexpr uint64_t kTrainRows = 8192;
 
 // Identifies a list of slots of one language inside plain/order lists.
 struct SchedList {
@@ -35,20 +39,14 @@ struct SchedList {
 
 int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists);
 void sched_free(SchedState &s);
-hipError_t sched_order_list(hipStream_t st, SchedState &s, const BatchDev &d,
-                            const uint32_t *plain, uint32_t *order, uint64_t count,
-                            size_t list_index);
-hipError_t sched_record(hipStream_t st, SchedState &s, const BatchDev &d);
-
-template <class LL>
+// Predicts every staged slot, then writes each list's slots to `order`
+// grouped by bucket, longest bucket first.
 hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d,
-                       const uint32_t *plain, uint32_t *order, const LL *lists,
-                       size_t num_lists) {
-  hipError_t e = hipSuccess;
-  for (size_t i = 0; i < num_lists && e == hipSuccess; i++)
-    e = sched_order_list(st, s, d, plain + lists[i].begin, order + lists[i].begin,
-                         lists[i].count, i);
-  return e;
-}
+                       const uint32_t *plain, uint32_t *order, const SchedList *lists,
+                       size_t num_lists);
+// Trains on the last epoch's staged tapes and recorded ops (host-driven,
+// outside captured graphs).  Returns the loss of the first step.
+hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int steps,
+                       float lr, float *first_loss);
 
 }  // namespace cubff

@@ -1,19 +1,199 @@
-// Learned scheduler (MLP cost model + length buckets).  Placeholder until the
-// model kernels land: every entry point reports that it is unavailable.
+// Learned scheduler: MLP inference over the staged tapes, 8 length buckets
+// (longest first), online training on recorded ops.
 #include <hip/hip_runtime.h>
+
+#include <vector>
 
 #include "sched.h"
 
 namespace cubff {
 
-int sched_ensure(SchedState &, const BatchDev &, size_t) { return -1; }
-void sched_free(SchedState &) {}
-hipError_t sched_order_list(hipStream_t, SchedState &, const BatchDev &, const uint32_t *,
-                            uint32_t *, uint64_t, size_t) {
-  return hipErrorNotSupported;
+// Buckets are quantiles of the current epoch's predictions, longest first:
+// bucket b holds the slots ranked in (F[b-1], F[b]] of the batch, F = 0.5%,
+// 2%, 5%, 10%, 20%, 40%, 70%, 100%.  Ranking is what the interpreter needs;
+// quantiles keep the top buckets small whatever the model's calibration.
+constexpr int kPredBins = 1024;
+constexpr float kPredScale = kPredBins / 16.0f;  // predictions in [0, 16)
+
+__device__ __forceinline__ uint32_t pred_bin(float y) {
+  const float b = y * kPredScale;
+  return b <= 0.f ? 0u : (b >= kPredBins - 1 ? kPredBins - 1 : (uint32_t)b);
 }
-hipError_t sched_record(hipStream_t, SchedState &, const BatchDev &) {
-  return hipErrorNotSupported;
+
+__global__ void k_pred_hist(const float *__restrict__ y, uint64_t count,
+                            uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[kPredBins];
+  for (int i = threadIdx.x; i < kPredBins; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[pred_bin(y[i])], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < kPredBins; i += blockDim.x)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// thr[b] = lowest prediction bin of bucket b (walking down from the top).
+__global__ void k_pred_thresholds(const uint32_t *__restrict__ hist, uint64_t count,
+                                  uint32_t *__restrict__ thr) {
+  if (threadIdx.x != 0) return;
+  const float F[kBuckets - 1] = {0.005f, 0.02f, 0.05f, 0.1f, 0.2f, 0.4f, 0.7f};
+  uint64_t run = 0;
+  int b = 0;
+  for (int bin = kPredBins - 1; bin >= 0 && b < kBuckets - 1; bin--) {
+    run += hist[bin];
+    while (b < kBuckets - 1 && (double)run >= F[b] * (double)count) thr[b++] = (uint32_t)bin;
+  }
+  for (; b < kBuckets - 1; b++) thr[b] = 0;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(float y, const uint32_t *__restrict__ thr) {
+  const uint32_t bin = pred_bin(y);
+  uint32_t k = 0;
+#pragma unroll
+  for (int i = 0; i < kBuckets - 1; i++) k += bin < thr[i];
+  return k;
+}
+
+__global__ void k_sched_count(const uint32_t *__restrict__ list, uint64_t count,
+                              const float *__restrict__ y, const uint32_t *__restrict__ thr,
+                              uint32_t *__restrict__ bucket, uint32_t *__restrict__ cnt) {
+  __shared__ uint32_t local[kBuckets];
+  if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    const uint32_t g = list[i];
+    const uint32_t b = bucket_of(y[g], thr);
+    bucket[g] = b;
+    atomicAdd(&local[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets && local[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], local[threadIdx.x]);
+}
+
+// cursor[b] = exclusive prefix of counts (bucket 0 first).
+__global__ void k_sched_prefix(uint32_t *__restrict__ cnt) {
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < kBuckets; b++) {
+      cnt[kBuckets + b] = run;
+      run += cnt[b];
+    }
+  }
+}
+
+// Block-aggregated scatter: ranks within the block come from LDS counters,
+// and each bucket reserves one range per block with a single global atomic.
+__global__ void k_sched_scatter(const uint32_t *__restrict__ list, uint64_t count,
+                                const uint32_t *__restrict__ bucket,
+                                uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
+  __shared__ uint32_t local[kBuckets], base[kBuckets];
+  if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t g = 0, b = 0, rank = 0;
+  if (i < count) {
+    g = list[i];
+    b = bucket[g];
+    rank = atomicAdd(&local[b], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x < kBuckets && local[threadIdx.x])
+    base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], local[threadIdx.x]);
+  __syncthreads();
+  if (i < count) order[base[b] + rank] = g;
+}
+
+__global__ void k_sched_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
+  if (s.ready) return 0;
+  if (mlp_alloc(s.mlp, d.total_slots) != hipSuccess) return -1;
+  if (hipMalloc(&s.bucket, d.total_slots * 4) != hipSuccess) return -1;
+  if (hipMalloc(&s.cnt, (num_lists ? num_lists : 1) * 2 * kBuckets * 4) != hipSuccess) return -1;
+  if (hipMalloc(&s.hist, kPredBins * 4) != hipSuccess) return -1;
+  if (hipMalloc(&s.thr, kBuckets * 4) != hipSuccess) return -1;
+  // Training sample: kTrainRows slots spread evenly over the batch.
+  s.train_rows_n = d.total_slots < kTrainRows ? d.total_slots / 64 * 64 : kTrainRows;
+  if (s.train_rows_n == 0) s.train_rows_n = d.total_slots;
+  std::vector<uint32_t> rows(s.train_rows_n);
+  for (uint64_t i = 0; i < s.train_rows_n; i++)
+    rows[i] = (uint32_t)((i * d.total_slots) / s.train_rows_n);
+  if (hipMalloc(&s.train_rows, s.train_rows_n * 4) != hipSuccess) return -1;
+  if (hipMemcpy(s.train_rows, rows.data(), s.train_rows_n * 4, hipMemcpyHostToDevice) !=
+      hipSuccess)
+    return -1;
+  hipStream_t st = nullptr;
+  if (mlp_init(s.mlp, 0x5eed, st) != hipSuccess) return -1;
+  if (mlp_fold(s.mlp, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
+  s.ready = true;
+  return 0;
+}
+
+void sched_free(SchedState &s) {
+  mlp_free(s.mlp);
+  if (s.bucket) (void)hipFree(s.bucket);
+  if (s.cnt) (void)hipFree(s.cnt);
+  if (s.hist) (void)hipFree(s.hist);
+  if (s.thr) (void)hipFree(s.thr);
+  if (s.train_rows) (void)hipFree(s.train_rows);
+  s = SchedState{};
+}
+
+hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const uint32_t *plain,
+                       uint32_t *order, const SchedList *lists, size_t num_lists) {
+  if (!s.ready) return hipErrorNotReady;
+  hipError_t e;
+  if ((e = mlp_features(s.mlp, d.stage, d.total_slots, st))) return e;
+  if ((e = mlp_infer(s.mlp, d.total_slots, st))) return e;
+  hipLaunchKernelGGL(k_sched_fill, dim3(1), dim3(256), 0, st, s.cnt, 0u,
+                     (uint64_t)num_lists * 2 * kBuckets);
+  hipLaunchKernelGGL(k_sched_fill, dim3(kPredBins / 256), dim3(256), 0, st, s.hist, 0u,
+                     (uint64_t)kPredBins);
+  hipLaunchKernelGGL(k_pred_hist, dim3(256), dim3(256), 0, st, s.mlp.y, d.total_slots, s.hist);
+  hipLaunchKernelGGL(k_pred_thresholds, dim3(1), dim3(64), 0, st, s.hist, d.total_slots, s.thr);
+  for (size_t li = 0; li < num_lists; li++) {
+    const uint64_t n = lists[li].count;
+    if (n == 0) continue;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    uint32_t *cnt = s.cnt + li * 2 * kBuckets;
+    hipLaunchKernelGGL(k_sched_count, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
+                       s.mlp.y, s.thr, s.bucket, cnt);
+    hipLaunchKernelGGL(k_sched_prefix, dim3(1), dim3(64), 0, st, cnt);
+    hipLaunchKernelGGL(k_sched_scatter, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
+                       s.bucket, cnt + kBuckets, order + lists[li].begin);
+  }
+  return hipGetLastError();
+}
+
+hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int steps, float lr,
+                       float *first_loss) {
+  if (!s.ready) return hipErrorNotReady;
+  hipError_t e;
+  const uint64_t n = s.train_rows_n;
+  if ((e = mlp_features_rows(s.mlp, d.stage, s.train_rows, n, st))) return e;
+  if ((e = mlp_targets_from_ops(s.mlp, d.pair_ops, s.train_rows, n, st))) return e;
+  // The first round starts from random weights: train it longer.
+  if (s.trainings == 0) steps *= 8;
+  for (int i = 0; i < steps; i++) {
+    if ((e = mlp_forward(s.mlp, n, st))) return e;
+    if ((e = mlp_backward(s.mlp, n, st))) return e;
+    if (i == 0 && first_loss) {
+      if ((e = hipMemcpyAsync(first_loss, s.mlp.loss, 4, hipMemcpyDeviceToHost, st))) return e;
+    }
+    if ((e = mlp_adam(s.mlp, lr, st))) return e;
+  }
+  // Inference (possibly replayed from a captured graph) reads the folded
+  // first layer: refresh it now, outside any capture.
+  if ((e = mlp_fold(s.mlp, st))) return e;
+  if ((e = hipStreamSynchronize(st))) return e;
+  s.trainings++;
+  if (first_loss) s.last_loss = *first_loss;
+  return hipSuccess;
 }
 
 }  // namespace cubff

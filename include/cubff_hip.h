@@ -204,6 +204,36 @@ int cubff_batch_reset_counters(cubff_batch *b);
 int cubff_batch_read_pair_stats(cubff_batch *b, uint32_t soup, uint32_t *ops,
                                 uint32_t *steps);
 int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s);
+/* Learned-scheduler statistics: trainings so far, the loss of the last
+ * training round, and the last epoch's slot count per length bucket
+ * (8 counts, bucket 0 = longest predicted). */
+int cubff_batch_sched_stats(cubff_batch *b, uint64_t *trainings,
+                            float *last_loss, uint32_t *bucket_counts);
+
+/* ------------------------------------------------------------------------ */
+/* The MLP cost model of the learned scheduler on its own.  No reference
+ * counterpart.  Input: 128-byte tapes; output: predicted log2(ops + 1).
+ * Parameters are one fp32 vector of cubff_mlp_num_params() values:
+ * E[512][256], W1[256][256], b1[256], W2[256][256], b2[256], w3[256], b3. */
+typedef struct cubff_mlp cubff_mlp;
+const char *cubff_mlp_last_error(void);
+uint64_t cubff_mlp_num_params(void);
+int cubff_mlp_create(cubff_mlp **out, uint64_t capacity, uint64_t init_seed,
+                     int device);
+int cubff_mlp_destroy(cubff_mlp *h);
+int cubff_mlp_set_params(cubff_mlp *h, const float *params);
+int cubff_mlp_get_params(cubff_mlp *h, float *params);
+int cubff_mlp_forward(cubff_mlp *h, const uint8_t *tapes, uint64_t count,
+                      float *pred);
+/* Loss 0.5 * mean((pred - target)^2) and its gradient (same layout as the
+ * parameters). */
+int cubff_mlp_gradients(cubff_mlp *h, const uint8_t *tapes,
+                        const float *targets, uint64_t count, float *loss,
+                        float *grads);
+/* `steps` full-batch Adam steps; losses (nullable) receives each step's
+ * loss before its update. */
+int cubff_mlp_train(cubff_mlp *h, const uint8_t *tapes, const float *targets,
+                    uint64_t count, uint32_t steps, float lr, float *losses);
 
 #ifdef __cplusplus
 }

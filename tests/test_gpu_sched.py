@@ -1,0 +1,132 @@
+"""GPU tests of the learned scheduler: the MLP's MFMA forward/backward against
+the CPU reference (tests/mlp_ref.py), training reduces the loss and beats the
+untrained model, and learned scheduling leaves every soup bit-exact."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import mlp_ref as R
+from tests.tapegen import biased_tapes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from cubff_amd import capi
+    capi.load()
+    return capi
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_mlp_forward_matches_cpu(capi):
+    tapes = biased_tapes("bff_noheads", 300, seed=11)
+    with capi.Mlp(512, init_seed=3) as m:
+        params = m.get_params()
+        y = m.forward(tapes)
+    y_bf, _, _ = R.forward_backward(params, tapes, np.zeros(300), bf16_points=True)
+    y_64, _, _ = R.forward_backward(params, tapes, np.zeros(300), bf16_points=False)
+    # Same rounding points: fp32 accumulation order is the only difference.
+    assert np.max(np.abs(y - y_bf)) < 2e-2 * max(1.0, np.max(np.abs(y_bf)))
+    assert _rel(y, y_bf) < 5e-3
+    # Against float64 without intermediate rounding: bf16 activations cost
+    # about three significant bits.
+    assert _rel(y, y_64) < 3e-2
+
+
+def test_mlp_gradients_match_cpu(capi):
+    tapes = biased_tapes("bff_noheads", 256, seed=12)
+    targets = np.random.default_rng(0).uniform(0, 13, 256).astype(np.float32)
+    with capi.Mlp(256, init_seed=4) as m:
+        params = m.get_params()
+        loss, g = m.gradients(tapes, targets)
+    _, loss_bf, g_bf = R.forward_backward(params, tapes, targets, bf16_points=True)
+    _, loss_64, g_64 = R.forward_backward(params, tapes, targets, bf16_points=False)
+    assert abs(loss - loss_bf) < 1e-3 * max(1.0, loss_bf)
+    for name, (a, b) in R.BLOCKS.items():
+        assert _rel(g[a:b], g_bf[a:b]) < 2e-2, name
+        assert _rel(g[a:b], g_64[a:b]) < 6e-2, name
+
+
+def test_mlp_gradient_finite_difference(capi):
+    # Directional derivative of the smooth (unrounded, float64) model along
+    # the GPU gradient equals the GPU gradient's norm, to bf16 accuracy.
+    tapes = biased_tapes("bff", 128, seed=13)
+    targets = np.random.default_rng(1).uniform(0, 10, 128).astype(np.float32)
+    with capi.Mlp(128, init_seed=5) as m:
+        params = m.get_params().astype(np.float64)
+        _, g = m.gradients(tapes, targets)
+    d = g.astype(np.float64) / np.linalg.norm(g)
+    h = 1e-4
+    f = lambda p: R.forward_backward(p, tapes, targets, False, False)[1]
+    fd = (f(params + h * d) - f(params - h * d)) / (2 * h)
+    assert abs(fd - np.linalg.norm(g)) < 0.05 * np.linalg.norm(g)
+
+
+def _soup_data(n=8192, epochs=24, seed=9):
+    """(staged tape, ops) pairs of a bff_noheads soup, from the oracle."""
+    from tests.tapegen import splitmix64_np
+    S = O.Soup("bff_noheads", n, seed)
+    X, Y = [], []
+    for e in range(epochs):
+        s = O.pairing_shuffle(n, seed, e)
+        sp = S.soup.reshape(n, 64)
+        t = np.concatenate([sp[s[0::2]], sp[s[1::2]]], axis=1).copy()
+        sv = O.epoch_seed(seed, e)
+        k = np.arange(n // 2, dtype=np.uint64)
+        with np.errstate(over="ignore"):
+            idx = ((np.uint64(n) * np.uint64(sv) + k)[:, None] * np.uint64(128)
+                   + np.arange(128, dtype=np.uint64)[None, :])
+        r = splitmix64_np(idx)
+        mut = ((r >> np.uint64(8)) & np.uint64((1 << 30) - 1)) < np.uint64(1 << 18)
+        t[mut] = (r[mut] & np.uint64(255)).astype(np.uint8)
+        _, po, _ = S.step(s)
+        X.append(t)
+        Y.append(po)
+    return np.concatenate(X), np.log2(np.concatenate(Y).astype(np.float32) + 1)
+
+
+def test_training_reduces_loss_and_beats_untrained(capi):
+    X, y = _soup_data()
+    ntr = 65536
+    with capi.Mlp(ntr, init_seed=6) as m:
+        p0 = m.forward(X[ntr:ntr + 16384])
+        losses = m.train(X[:ntr], y[:ntr], steps=60, lr=2e-3)
+        p1 = m.forward(X[ntr:ntr + 16384])
+    yt = y[ntr:ntr + 16384]
+    assert losses[-1] < 0.7 * losses[0]
+    mse0, mse1 = np.mean((p0 - yt) ** 2), np.mean((p1 - yt) ** 2)
+    base = np.mean((yt - yt.mean()) ** 2)
+    assert mse1 < mse0 and mse1 < base
+    # The longest interactions concentrate in the top predicted decile.
+    top = np.argsort(-p1)[: len(p1) // 10]
+    long_ = yt >= np.log2(1001)
+    assert long_[top].sum() >= 2 * 0.1 * long_.sum()
+
+
+def test_learned_schedule_is_bit_exact(capi):
+    spec = [("bff_noheads", 4096, 0, 262144), ("forth", 2048, 1, 262144),
+            ("bff", 2048, 2, 1 << 20)]
+    with capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as a, \
+            capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as b:
+        for x in (a, b):
+            x.init([s[2] for s in spec], [s[3] for s in spec])
+        a.run_epochs(0, 20, capi.SCHED_LEARNED, use_graph=False)
+        b.run_epochs(0, 20, capi.SCHED_PLAIN)
+        for i in range(len(spec)):
+            assert np.array_equal(a.read_soup(i), b.read_soup(i))
+            assert a.read_counters(i) == b.read_counters(i)
+        trainings, loss, buckets = a.sched_stats()
+        assert trainings == 2 and np.isfinite(loss)
+        assert buckets.sum() == sum(n // 2 for _, n, _, _ in spec)
+    with capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as c:
+        c.init([s[2] for s in spec], [s[3] for s in spec])
+        c.run_epochs(0, 20, capi.SCHED_LEARNED, use_graph=True)
+        for i in range(len(spec)):
+            ref = O.Soup(spec[i][0], spec[i][1], spec[i][2], spec[i][3])
+            for e in range(20):
+                ref.step()
+            assert np.array_equal(c.read_soup(i), ref.soup)
