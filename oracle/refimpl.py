@@ -143,3 +143,18 @@ def shuffle_of(state: RefState, n: int) -> np.ndarray:
 
 def replication_of(state: RefState, n: int) -> np.ndarray:
     return np.ctypeslib.as_array(state.replication_per_prog, shape=(n,)).copy()
+
+
+def parse(lang: str, program: str) -> np.ndarray:
+    """LanguageInterface::Parse of the reference build."""
+    buf = np.zeros(4096, dtype=np.uint8)
+    n = lib().ref_parse(lang.encode(), program.encode(), buf.ctypes.data_as(_u8p), buf.size)
+    assert n <= buf.size
+    return buf[:n].copy()
+
+
+def eval_parsed_selfrep(lang: str, parsed: np.ndarray, epoch: int, seed: int) -> int:
+    """EvalParsedSelfrep (common_language.h:326-342) of the reference build."""
+    p = np.ascontiguousarray(parsed, dtype=np.uint8)
+    return int(lib().ref_eval_parsed_selfrep(lang.encode(), p.ctypes.data_as(_u8p), p.size,
+                                             epoch, seed))

@@ -12,6 +12,7 @@ SHA-256 of the raw little-endian bytes (first 16 hex digits where noted).
 from __future__ import annotations
 
 import argparse
+import base64
 import hashlib
 import json
 import os
@@ -212,9 +213,165 @@ def gen_cli_logs():
         "brotli": "libbrotlienc 1.0.9 (this image)", "logs": out})
 
 
+def run_programs():
+    """Program texts for --run cases: a fixed BFF program, random command
+    strings for the BFF maps, random hex for Forth and SUBLEQ."""
+    rng = np.random.default_rng(2024)
+    out = {}
+    for lang in LANGS:
+        if lang.startswith("bff"):
+            texts = ["[[{.>]-]0000]-]>.{[[<<+-0"]
+            for k in range(3):
+                texts.append("".join(rng.choice(list("[]+-.,<>{}0"), size=20 + 30 * k)))
+        else:
+            texts = ["".join(f"{b:02x}" for b in rng.integers(0, 256, size=m))
+                     for m in (8, 40, 100)]
+        out[lang] = texts
+    return out
+
+
+def gen_cli_cases():
+    """Reference CLI outputs that need no timing: --run transcripts, --help,
+    argument errors, and deterministic logs / checkpoints / frames of small
+    runs with every mode flag."""
+    main = R.MAIN_PATH
+    tmp = tempfile.mkdtemp()
+
+    def run(args, cwd=tmp):
+        p = subprocess.run([main] + args, cwd=cwd, capture_output=True)
+        return p.returncode, p.stdout, p.stderr
+
+    runs = {}
+    for lang, texts in run_programs().items():
+        rows = []
+        for t in texts:
+            for steps in (0, 1000):
+                rc, out, _ = run(["--lang", lang, "--run", t, "--run_steps", str(steps)])
+                assert rc == 0
+                parsed = R.parse(lang, t)
+                rows.append({"text": t, "steps": steps, "parsed": parsed.tobytes().hex(),
+                             "stdout_zlib_b64": base64.b64encode(zlib.compress(out, 9)).decode()})
+        runs[lang] = rows
+
+    selfrep = {}
+    for lang, texts in run_programs().items():
+        # Programs longer than one 64-byte tape overflow the reference's
+        # buffer (common_language.h:330-331), so only short ones are used.
+        selfrep[lang] = [{"text": t, "epoch": e, "seed": s,
+                          "result": R.eval_parsed_selfrep(lang, R.parse(lang, t), e, s)}
+                         for t in texts if R.parse(lang, t).size <= 64
+                         for (e, s) in ((0, 0), (5, 7))]
+
+    # Self-replicators harvested from 257-epoch soups (131072 programs, seed
+    # 10248) of the languages where they appear that early.
+    for lang in ("forthtrivial", "bff_noheads_4bit", "forthtrivial_reset"):
+        best = []
+
+        def cb(st, lang=lang, best=best):
+            if st.epoch < 257:
+                return False
+            rep = R.replication_of(st, 131072)
+            soup = R.soup_of(st).reshape(-1, 64)
+            seen = set()
+            for i in np.argsort(-rep.astype(np.int64), kind="stable"):
+                prog = soup[i].tobytes()
+                if prog not in seen:
+                    seen.add(prog)
+                    best.append((int(rep[i]), soup[i].copy()))
+                if len(best) == 3:
+                    break
+            return True
+
+        R.run_simulation(lang, 131072, 10248, cb, callback_interval=256, eval_selfrep=True)
+        for score, prog in best:
+            for e, s in ((0, 0), (256, 10248), (5, 7)):
+                selfrep[lang].append({"parsed": prog.tobytes().hex(), "epoch": e, "seed": s,
+                                      "soup_score": score,
+                                      "result": R.eval_parsed_selfrep(lang, prog, e, s)})
+
+    rc, _, help_err = run(["--help"])
+    errors = []
+    for args in (["--nope"], ["--num"], ["--fixed_shuffle", "--num", "1000", "--lang", "bff"],
+                 ["--save_interval", "100", "--lang", "bff"],
+                 ["--clear_interval", "100", "--lang", "bff"],
+                 ["--stopping_selfrep_count", "3", "--lang", "bff"],
+                 ["--draw_to_2d", ".", "--num", "1000", "--lang", "bff"],
+                 ["--draw_to_2d", ".", "--grid_width_2d", "7", "--num", "1000", "--lang", "bff"],
+                 ["--interaction_pattern", "missing.txt", "--lang", "bff"],
+                 ["--lang", "nope"]):
+        rc, out, err = run(args)
+        errors.append({"args": args, "rc": rc, "stdout": out.decode(), "stderr": err.decode()})
+
+    pattern = "\n".join(f"{a} {b}" for a, nb in enumerate(_grid_lists(32, 32)) for b in nb)
+    with open(os.path.join(tmp, "pattern.txt"), "w") as f:
+        f.write(pattern + "\n")
+    base = ["--disable_output", "--log", "log.csv", "--num", "1024", "--max_epochs", "96",
+            "--print_interval", "16", "--save_interval", "32"]
+    specs = {
+        "shuffle_bff_noheads": ["--lang", "bff_noheads", "--seed", "3"],
+        "selfrep_bff_noheads": ["--lang", "bff_noheads", "--seed", "4", "--eval_selfrep"],
+        "selfrep_forth": ["--lang", "forth", "--seed", "5", "--eval_selfrep"],
+        "fixed_shuffle": ["--lang", "bff", "--fixed_shuffle"],
+        "cyclic": ["--lang", "bff_noheads", "--nopermute_programs"],
+        "zero_init_initial": ["--lang", "bff", "--zero_init", "--initial_program",
+                              "[[{.>]-]0000]-]>.{[[<<+-0"],
+        "reset": ["--lang", "subleq", "--reset_interval", "40"],
+        "mutation": ["--lang", "rsubleq4", "--mutation_prob", "0.001"],
+        "pattern_2d": ["--lang", "bff_noheads", "--interaction_pattern", "pattern.txt",
+                       "--seed", "8"],
+        "checkpoints": ["--lang", "forthtrivial", "--checkpoint_dir", "ck"],
+        "frames_1d": ["--lang", "bff", "--draw_to", "f1", "--max_epochs", "32"],
+        "frames_2d": ["--lang", "forthcopy", "--draw_to_2d", "f2", "--max_epochs", "32"],
+        "frames_2d_w": ["--lang", "subleq", "--draw_to_2d", "f3", "--grid_width_2d", "64",
+                        "--max_epochs", "32"],
+        "stop_bpb": ["--lang", "bff_noheads_4bit", "--stopping_bpb", "7.5",
+                     "--max_epochs", "100000"],
+    }
+    cases = {}
+    for name, extra in specs.items():
+        d = tempfile.mkdtemp(dir=tmp)
+        for sub in ("f1", "f2", "f3"):
+            os.makedirs(os.path.join(d, sub))
+        shutil.copyfile(os.path.join(tmp, "pattern.txt"), os.path.join(d, "pattern.txt"))
+        args = base + extra
+        rc, out, err = run(args, cwd=d)
+        assert rc == 0, (name, err)
+        files = {}
+        for root, _, fs in os.walk(d):
+            for fn in fs:
+                rel = os.path.relpath(os.path.join(root, fn), d)
+                if rel in ("log.csv", "pattern.txt"):
+                    continue
+                with open(os.path.join(root, fn), "rb") as f:
+                    files[rel] = hashlib.sha256(f.read()).hexdigest()
+        cases[name] = {"args": args, "log": open(os.path.join(d, "log.csv")).read(),
+                       "files": files}
+    # Resume from the epoch-64 checkpoint of the "checkpoints" case.
+    d = tempfile.mkdtemp(dir=tmp)
+    run(base + specs["checkpoints"], cwd=d)
+    args = ["--disable_output", "--log", "log.csv", "--lang", "forthtrivial", "--load",
+            "ck/0000000064.dat", "--max_epochs", "96", "--print_interval", "16",
+            "--save_interval", "32"]
+    rc, _, err = run(args, cwd=d)
+    assert rc == 0, err
+    cases["resume"] = {"args": args, "setup": "checkpoints",
+                       "log": open(os.path.join(d, "log.csv")).read(), "files": {}}
+    shutil.rmtree(tmp)
+    dump("cli_cases.json", {
+        "source": "reference bin/main built in oracle/_ref (libbrotlienc 1.0.9)",
+        "runs": runs, "eval_selfrep": selfrep, "help": help_err.decode(), "errors": errors,
+        "pattern_2d": "make_2d_pattern.py 32 32 (radius 2)", "cases": cases})
+
+
+def _grid_lists(w, h, r=2):
+    return [[y * w + x for y in range(i - r, i + r + 1) for x in range(j - r, j + r + 1)
+             if 0 <= y < h and 0 <= x < w and (y, x) != (i, j)]
+            for i in range(h) for j in range(w)]
+
+
 GENS = {"eval": gen_eval, "epochs": gen_epochs, "selfrep": gen_selfrep,
         "2d": gen_2d, "modes": gen_modes, "full": gen_full_size,
-        "cli": gen_cli_logs}
+        "cli": gen_cli_logs, "cli_cases": gen_cli_cases}
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
