@@ -229,6 +229,8 @@ struct BffLang {
   static constexpr uint32_t kTH0 = 0x58555;
   static constexpr uint32_t kTH1 = LOGIC == kLogicSelfmove ? 0x55955 : 0x85555;
   static constexpr uint32_t kTDelta = 0x55525;
+  // Loop acceleration needs the standard `[` / `]` semantics.
+  static constexpr bool kAccel = LOGIC == kLogicStd;
 
   // Evaluate (bff.inc.h:356-389) with EvaluateOne (bff.inc.h:289-349,
   // bff8.cu:23-65, bff_selfmove.cu:44-98).
@@ -284,12 +286,24 @@ struct BffLang {
       // Jump caches: source pc, position after the jump, next command.
       int bsrc = -1, bafter = 0, bnext = 0;
       int fsrc = -1, fafter = 0, fnext = 0;
+      // Loop acceleration (standard jump logic only).  The anchor is the
+      // state right after the last taken backward jump: its source aq, the
+      // heads, step and no-op counts, and the bytes under the heads.  Flags
+      // record what the commands since then did.
+      int aq = -1, ah0 = 0, ah1 = 0;
+      uint32_t ai = 0, askip = 0, aw0 = 0, aw1 = 0;
+      bool wrote = false, moved = false, branched = false;
       for (;;) {
         const uint32_t c = t.get(pc);
         const uint32_t v0 = t.get(h0);
         const uint32_t v1 = t.get(h1);
         int fall = mc.next_from(pc + 1);
         const uint32_t k = C::kind(c);
+        if constexpr (kAccel) {
+          wrote |= (k - kPlus) < 4u;
+          moved |= (k - kDec0) < 4u;
+          branched |= k == kLoopStart || (k == kLoopEnd && pc != aq);
+        }
         const bool c01 = k == kCopy01;
         const int waddr = c01 ? h1 : h0;
         const uint32_t wold = c01 ? v1 : v0;
@@ -319,7 +333,7 @@ struct BffLang {
           after = pc + 1 + (tf ? 8 : 0) - (tb ? 8 : 0);
           if (tb && pc < 8) after = -1;  // pc - 8 < 0 stops the run
           after = min(after, kTape2);
-          nextc = after < 0 ? -1 : (tf | tb ? mc.next_from(after) : fall);
+          nextc = after < 0 ? -1 : ((tf | tb) ? mc.next_from(after) : fall);
         } else {
           const bool z = LOGIC == kLogicSelfmove ? v0 == 0 : C::null(v0);
           const bool tf = k == kLoopStart && z;
@@ -356,6 +370,53 @@ struct BffLang {
           }
           after = tb ? bafter : (tf ? fafter : pc + 1);
           nextc = tb ? bnext : (tf ? fnext : fall);
+          if constexpr (kAccel) {
+            if (tb && nextc >= 0) {
+              // A second taken jump from aq with no other bracket command in
+              // between closes one iteration of a straight-line loop body of
+              // L steps.  Its next iterations repeat it exactly when
+              //   (a) it wrote nothing: the tape is constant, the heads
+              //       advance by fixed deltas, and the loop runs until the
+              //       byte under head 0 at the `]` is null; or
+              //   (b) it moved no head and left the two bytes under the
+              //       heads (the only ones it can write) as they were: the
+              //       whole state repeats.
+              // Whole iterations are then counted without being run, up to
+              // the step cap and the exiting iteration, which runs normally.
+              const uint32_t i1 = i + 1;
+              if (pc == aq && !branched &&
+                  (!wrote || (!moved && v0 == aw0 && v1 == aw1))) {
+                const uint32_t len = i1 - ai;
+                const uint32_t skips = nskip - askip;
+                uint32_t reps = (stepcount - i1) / len;
+                const int d0 = (h0 - ah0) & (kTape2 - 1);
+                const int d1 = (h1 - ah1) & (kTape2 - 1);
+                if (d0 != 0) {
+                  // First later iteration whose `]` sees a null byte.
+                  for (uint32_t j = 1; j <= (uint32_t)kTape2 && j <= reps; j++) {
+                    if (C::null(t.get((h0 + j * d0) & (kTape2 - 1)))) {
+                      reps = j - 1;
+                      break;
+                    }
+                  }
+                }
+                i += reps * len;
+                nskip += reps * skips;
+                h0 = (h0 + (int)(reps * d0)) & (kTape2 - 1);
+                h1 = (h1 + (int)(reps * d1)) & (kTape2 - 1);
+                aq = -1;
+              } else {
+                aq = pc;
+                ah0 = h0;
+                ah1 = h1;
+                ai = i1;
+                askip = nskip;
+                aw0 = v0;
+                aw1 = v1;
+                wrote = moved = branched = false;
+              }
+            }
+          }
         }
         i++;
         if (nextc < 0) break;
