@@ -35,7 +35,7 @@ EXPORTS = [
     "cubff_batch_read_shuffle", "cubff_batch_sched_stats",
     "cubff_mlp_last_error", "cubff_mlp_num_params", "cubff_mlp_create",
     "cubff_mlp_destroy", "cubff_mlp_set_params", "cubff_mlp_get_params",
-    "cubff_mlp_forward", "cubff_mlp_gradients", "cubff_mlp_train",
+    "cubff_mlp_forward", "cubff_mlp_predict", "cubff_mlp_gradients", "cubff_mlp_train",
 ]
 
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
@@ -121,6 +121,7 @@ def load():
         "cubff_mlp_set_params": (_i, [_vp, _f32p]),
         "cubff_mlp_get_params": (_i, [_vp, _f32p]),
         "cubff_mlp_forward": (_i, [_vp, _u8p, _u64, _f32p]),
+        "cubff_mlp_predict": (_i, [_vp, _u8p, _u64, _f32p]),
         "cubff_mlp_gradients": (_i, [_vp, _u8p, _f32p, _u64, _f32p, _f32p]),
         "cubff_mlp_train": (_i, [_vp, _u8p, _f32p, _u64, ctypes.c_uint32, ctypes.c_float,
                                  _f32p]),
@@ -445,6 +446,13 @@ class Mlp:
         t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
         out = np.zeros(t.shape[0], dtype=np.float32)
         _mcheck(load().cubff_mlp_forward(self._h, _ptr(t, _u8p), t.shape[0], _ptr(out, _f32p)))
+        return out
+
+    def predict(self, tapes: np.ndarray) -> np.ndarray:
+        """The scheduler's fused inference kernel."""
+        t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
+        out = np.zeros(t.shape[0], dtype=np.float32)
+        _mcheck(load().cubff_mlp_predict(self._h, _ptr(t, _u8p), t.shape[0], _ptr(out, _f32p)))
         return out
 
     def gradients(self, tapes: np.ndarray, targets: np.ndarray):

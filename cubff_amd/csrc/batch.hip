@@ -9,8 +9,8 @@
 //                 128-byte tape per pair slot
 //   k_b_interp<L> evaluate staged tapes in the order of `order` (identity or
 //                 longest-predicted first), scatter the halves back
-//                 (common_language.h:184-192), record per-slot ops/steps and
-//                 per-soup counters
+//                 (common_language.h:184-192), record per-slot ops/steps
+//   k_b_end_epoch per-soup counter sums, num_runs, epoch counter
 #include <hip/hip_runtime.h>
 
 #include "batch.h"
@@ -47,11 +47,15 @@ __global__ void k_b_fy_draw(const BatchDev d) {
   atomicAdd(&d.fy_cnt[j], 1u);
 }
 
+// Contiguous item ranges per bucket; every lane of the block takes part in
+// the wavefront allocation (blockDim is a multiple of 64).
 __global__ void k_b_fy_alloc(const BatchDev d) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= d.total_programs) return;
-  const uint32_t c = d.fy_cnt[p];
-  d.fy_base[p] = c ? atomicAdd(&d.fy_cnt[d.total_programs], c) : 0u;
+  const bool in = p < d.total_programs;
+  const uint32_t c = in ? d.fy_cnt[p] : 0u;
+  const uint32_t base = wave_alloc_u32(&d.fy_cnt[d.total_programs], c);
+  if (!in) return;
+  d.fy_base[p] = base;
   d.fy_cursor[p] = 0;
 }
 
@@ -189,17 +193,36 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
     }
     d.pair_ops[g] = ops;
     d.pair_steps[g] = steps;
-    if (ops) atomicAdd(&d.counters[kNumBatchCounters * s + 0], (unsigned long long)ops);
-    if (steps) atomicAdd(&d.counters[kNumBatchCounters * s + 2], (unsigned long long)steps);
   }
 }
 
-// num_runs += num_indices per soup (common_language.h:494), then the next
+// Per-soup interval counters (common_language.h:192, :494): one block per
+// soup sums its slots' ops and steps; num_runs += num_indices; then the next
 // epoch.
-__global__ void k_b_end_epoch(const BatchDev d) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < d.num_soups) d.counters[kNumBatchCounters * s + 1] += d.num_indices[s];
-  if (s == 0) *d.epoch += 1;
+__global__ __launch_bounds__(256) void k_b_end_epoch(const BatchDev d) {
+  __shared__ unsigned long long part[2][4];
+  const uint32_t s = blockIdx.x;
+  const uint64_t lo = d.slot_base[s], hi = d.slot_base[s + 1];
+  unsigned long long ops = 0, steps = 0;
+  for (uint64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) {
+    ops += d.pair_ops[g];
+    steps += d.pair_steps[g];
+  }
+  ops = wave_sum_u64(ops);
+  steps = wave_sum_u64(steps);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][w] = ops;
+    part[1][w] = steps;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long *c = d.counters + kNumBatchCounters * (uint64_t)s;
+    c[0] += part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    c[1] += d.num_indices[s];
+    c[2] += part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    if (s == 0) *d.epoch += 1;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -233,7 +256,7 @@ hipError_t batch_stage(hipStream_t st, const BatchDev &d) {
 }
 
 hipError_t batch_end_epoch(hipStream_t st, const BatchDev &d) {
-  hipLaunchKernelGGL(k_b_end_epoch, dim3(nblk(d.num_soups, 64)), dim3(64), 0, st, d);
+  hipLaunchKernelGGL(k_b_end_epoch, dim3(d.num_soups), dim3(256), 0, st, d);
   return hipGetLastError();
 }
 

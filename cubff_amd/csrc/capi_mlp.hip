@@ -102,6 +102,17 @@ int cubff_mlp_forward(cubff_mlp *h, const uint8_t *tapes, uint64_t count, float 
   return CUBFF_OK;
 }
 
+int cubff_mlp_predict(cubff_mlp *h, const uint8_t *tapes, uint64_t count, float *pred) {
+  if (!h || !tapes || !pred) return fail(CUBFF_EINVAL, "null argument");
+  if (count == 0 || count > h->cap) return fail(CUBFF_EINVAL, "count must be in [1, capacity]");
+  MLP_TRY(hipSetDevice(h->device));
+  MLP_TRY(hipMemcpyAsync(h->tapes, tapes, count * kTape2, hipMemcpyHostToDevice, h->stream));
+  MLP_TRY(mlp_predict(h->m, h->tapes, count, h->m.y, h->stream));
+  MLP_TRY(hipMemcpyAsync(pred, h->m.y, count * 4, hipMemcpyDeviceToHost, h->stream));
+  MLP_TRY(hipStreamSynchronize(h->stream));
+  return CUBFF_OK;
+}
+
 int cubff_mlp_gradients(cubff_mlp *h, const uint8_t *tapes, const float *targets,
                         uint64_t count, float *loss, float *grads) {
   if (!h || !tapes || !targets) return fail(CUBFF_EINVAL, "null argument");

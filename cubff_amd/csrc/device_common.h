@@ -53,6 +53,23 @@ struct LdsTape {
   }
 };
 
+// Wavefront-aggregated bump allocation: every lane of the (fully active)
+// wavefront asks for c slots of *counter and receives the first of a private
+// contiguous range; one atomic per wavefront instead of one per lane.
+__device__ __forceinline__ uint32_t wave_alloc_u32(uint32_t *counter, uint32_t c) {
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  uint32_t base = 0;
+  if (lane == 63 && incl) base = atomicAdd(counter, incl);
+  base = __shfl(base, 63, 64);
+  return base + incl - c;
+}
+
 // Sum over the 64 lanes of a wavefront.
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 #pragma unroll

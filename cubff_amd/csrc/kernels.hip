@@ -292,10 +292,12 @@ __global__ void k_fy_alloc(const uint32_t *__restrict__ cnt,
                            uint32_t *__restrict__ base,
                            uint32_t *__restrict__ cursor,
                            unsigned int *__restrict__ alloc, uint64_t n) {
-  uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const uint32_t c = cnt[p];
-  base[p] = c ? atomicAdd(alloc, c) : 0u;
+  const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = p < n;
+  const uint32_t c = in ? cnt[p] : 0u;
+  const uint32_t b = wave_alloc_u32(alloc, c);  // all lanes take part
+  if (!in) return;
+  base[p] = b;
   cursor[p] = 0;
 }
 

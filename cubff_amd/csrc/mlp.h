@@ -56,10 +56,13 @@ hipError_t mlp_features_rows(Mlp &m, const uint8_t *stage, const uint32_t *rows,
                              uint64_t count, hipStream_t st);
 // Forward over m.x[0..count) into m.y (keeps p, a1, a2 for backward).
 hipError_t mlp_forward(Mlp &m, uint64_t count, hipStream_t st);
-// Inference over m.x[0..count) into m.y with the first two linear maps
-// folded (x (E W1)): one GEMM less; the fold is recomputed after updates.
-hipError_t mlp_infer(Mlp &m, uint64_t count, hipStream_t st);
+// Folds the first two linear maps (E W1) for inference; recomputed after
+// every update.
 hipError_t mlp_fold(Mlp &m, hipStream_t st);
+// Fused inference of `count` tapes (device, 128 bytes apart) straight into
+// y[0..count): features, both layers and the output in one kernel, with the
+// folded first layer.  Needs no activation buffers.
+hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y, hipStream_t st);
 // Loss 0.5 * mean((y - target)^2) and all gradients into m.grads for
 // m.x / m.target rows [0..count); *m.loss receives the loss.
 hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st);

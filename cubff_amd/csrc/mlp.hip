@@ -32,6 +32,7 @@ enum GemmEpi : int {
   kEpiBiasRelu = 1,   // out (bf16) = relu(C + bias)
   kEpiBf16 = 2,       // out (bf16) = C
   kEpiMaskBf16 = 3,   // out (bf16) = C * (mask != 0)
+  kEpiF32Add = 4,     // out (fp32) += C (atomic; split-K partial sums)
 };
 
 // C[M, N] = A[M, K] B[K, N] with A(i, k) = A[i * a_si + k * a_sk] and
@@ -40,7 +41,8 @@ enum GemmEpi : int {
 // v_mfma_f32_16x16x16_bf16 accumulators.  Operand fragments: lane l holds
 // A(i0 + l % 16, k0 + 4 (l / 16) + 0..3) and B(k0 + 4 (l / 16) + 0..3,
 // j0 + l % 16); the accumulator holds C(i0 + 4 (l / 16) + r, j0 + l % 16).
-// AC / BC: the four k values are contiguous (one 8-byte load).
+// AC / BC: the four k values are contiguous (one 8-byte load).  Block z
+// covers k in [z * K, (z + 1) * K) (split-K with kEpiF32Add).
 template <bool AC, bool BC, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(int K, const uint16_t *__restrict__ A,
                                               int64_t a_si, int64_t a_sk,
@@ -58,7 +60,8 @@ __global__ __launch_bounds__(256) void k_gemm(int K, const uint16_t *__restrict_
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  const int kbeg = (int)blockIdx.z * K;
+  for (int k0 = kbeg; k0 < kbeg + K; k0 += 16) {
     bf16x4 fa[2], fb[2];
 #pragma unroll
     for (int t = 0; t < 2; t++) {
@@ -93,6 +96,8 @@ __global__ __launch_bounds__(256) void k_gemm(int K, const uint16_t *__restrict_
         float v = acc[a][b][r];
         if constexpr (EPI == kEpiF32) {
           reinterpret_cast<float *>(out)[row * ldo + col] = v;
+        } else if constexpr (EPI == kEpiF32Add) {
+          atomicAdd(reinterpret_cast<float *>(out) + row * ldo + col, v);
         } else {
           if constexpr (EPI == kEpiBiasRelu) v = fmaxf(v + bias[col], 0.f);
           if constexpr (EPI == kEpiMaskBf16) v = mask[row * ldo + col] ? v : 0.f;
@@ -104,12 +109,22 @@ __global__ __launch_bounds__(256) void k_gemm(int K, const uint16_t *__restrict_
 template <bool AC, bool BC, int EPI>
 static hipError_t gemm(hipStream_t st, int M, int N, int K, const uint16_t *A, int64_t a_si,
                        int64_t a_sk, const uint16_t *B, int64_t b_sk, int64_t b_sj,
-                       const float *bias, void *out, int64_t ldo, const uint16_t *mask) {
-  if (M % 64 || N % 64 || K % 16) return hipErrorInvalidValue;
+                       const float *bias, void *out, int64_t ldo, const uint16_t *mask,
+                       int splits = 1) {
+  if (M % 64 || N % 64 || K % (16 * splits)) return hipErrorInvalidValue;
+  if (splits > 1 && EPI != kEpiF32Add) return hipErrorInvalidValue;
   if (M == 0) return hipSuccess;
-  hipLaunchKernelGGL((k_gemm<AC, BC, EPI>), dim3(N / 64, M / 64), dim3(256), 0, st, K, A,
-                     a_si, a_sk, B, b_sk, b_sj, bias, out, ldo, mask);
+  hipLaunchKernelGGL((k_gemm<AC, BC, EPI>), dim3(N / 64, M / 64, splits), dim3(256), 0, st,
+                     K / splits, A, a_si, a_sk, B, b_sk, b_sj, bias, out, ldo, mask);
   return hipGetLastError();
+}
+
+// Splits of a weight-gradient GEMM over the batch dimension K (a multiple of
+// 64): chunks of at least 256 rows, at most 32 of them.
+static int wgrad_splits(uint64_t K) {
+  int s = 1;
+  while (s < 32 && K % (uint64_t)(128 * s) == 0 && K / (uint64_t)(2 * s) >= 256) s *= 2;
+  return s;
 }
 
 static inline uint64_t pad64(uint64_t n) { return (n + 63) / 64 * 64; }
@@ -243,31 +258,30 @@ __global__ void k_out(const uint16_t *__restrict__ a2, const float *__restrict__
 }
 
 // dy = (y - t) / n (zero for padding rows); d2 = dy * w3 * relu'(a2);
-// loss += 0.5 * (y - t)^2 / n.  dy is left in y.
-__global__ void k_out_bwd(const uint16_t *__restrict__ a2, const float *__restrict__ params,
-                          float *__restrict__ y, const float *__restrict__ target,
-                          uint16_t *__restrict__ d2, uint64_t count, uint64_t rows_padded,
-                          float *__restrict__ loss) {
-  const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows_padded) return;
+// loss += 0.5 * (y - t)^2 / n.  One block per row, one thread per column;
+// dy is left in y.
+__global__ __launch_bounds__(kMlpHid) void k_out_bwd(
+    const uint16_t *__restrict__ a2, const float *__restrict__ params, float *__restrict__ y,
+    const float *__restrict__ target, uint16_t *__restrict__ d2, uint64_t count,
+    float *__restrict__ loss) {
+  const uint64_t r = blockIdx.x;
+  const int j = threadIdx.x;
   float dy = 0.f;
   if (r < count) {
     const float e = y[r] - target[r];
     dy = e / (float)count;
-    atomicAdd(loss, 0.5f * e * e / (float)count);
+    if (j == 0) atomicAdd(loss, 0.5f * e * e / (float)count);
   }
-  y[r] = dy;
-  const float *w3 = params + kOffW3;
-  for (int j = 0; j < kMlpHid; j++) {
-    const uint16_t a = a2[r * kMlpHid + j];
-    d2[r * kMlpHid + j] = f2bf(a ? dy * w3[j] : 0.f);
-  }
+  __syncthreads();  // every thread has read y[r] before it is overwritten
+  if (j == 0) y[r] = dy;
+  const uint16_t a = a2[r * kMlpHid + j];
+  d2[r * kMlpHid + j] = f2bf(a ? dy * params[kOffW3 + j] : 0.f);
 }
 
 // dw3[j] += sum over a chunk of rows of dy[r] a2[r][j]; db3 likewise.
 // Grid: one block of 256 threads (one per j) per kRedRows rows; results are
 // accumulated with atomics into zeroed gradients.
-constexpr int kRedRows = 256;
+constexpr int kRedRows = 32;
 __global__ void k_out_wgrad(const uint16_t *__restrict__ a2, const float *__restrict__ dy,
                             uint64_t count, float *__restrict__ grads) {
   const int j = threadIdx.x;
@@ -320,6 +334,200 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
                           uint64_t count, float *__restrict__ target) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) target[i] = log2f((float)ops[rows ? rows[i] : i] + 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Fused inference: features, both hidden layers and the output of 128 tapes
+// per block, with nothing but the tapes read from and the predictions written
+// to memory.
+//
+//   1. byte and bigram counts of each tape into an LDS histogram (four u8
+//      counters per dword, LDS atomics; counts never exceed 128);
+//   2. a1 = relu(x (E W1) + b1): v_mfma_f32_16x16x32_bf16 with A fragments
+//      converted from the counts (count / 128 is exact in bf16) and the
+//      folded weights staged through LDS in K-chunks of 64;
+//   3. a2 = relu(a1 W2 + b2) with a1 kept in LDS as bf16 (over the histogram);
+//   4. y = a2 . w3 + b3, reduced across lanes and waves in LDS.
+//
+// Block: 8 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
+// columns 128 (w % 2) .. +128 (eight 16-column blocks).  Fragment layout of
+// the 16x16x32 MFMA: lane l holds A(l % 16, 8 (l / 16) + 0..7),
+// B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kInfM = 128;        // tapes per block
+constexpr int kHistStride = 520;  // bytes per histogram row (512 counters + pad)
+constexpr int kActStride = 264;   // bf16 per activation row (256 + pad)
+constexpr int kWKC = 64;          // weight chunk along K
+constexpr int kWKP = 72;          // padded chunk row (bf16)
+
+__device__ __forceinline__ bf16x8 counts_to_bf16(uint64_t c8) {
+  bf16x8 r;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const float f = (float)((uint32_t)(c8 >> (8 * q)) & 255u) * (1.0f / 128);
+    r[q] = __builtin_bit_cast(__bf16, (uint16_t)(__float_as_uint(f) >> 16));
+  }
+  return r;
+}
+
+// Stages rows [0, 256) x K-columns [kc, kc + 64) of WT ([256][K] bf16) into ws.
+template <int K>
+__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ WT, int kc,
+                                              uint16_t *__restrict__ ws) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int idx = threadIdx.x + 512 * c;  // 2048 chunks of 8 bf16
+    const int n = idx >> 3, part = idx & 7;
+    *reinterpret_cast<uint4 *>(ws + n * kWKP + part * 8) =
+        *reinterpret_cast<const uint4 *>(WT + (int64_t)n * K + kc + part * 8);
+  }
+}
+
+__global__ __launch_bounds__(512) void k_predict(const uint8_t *__restrict__ tapes,
+                                                 uint64_t count,
+                                                 const uint16_t *__restrict__ ew1t,
+                                                 const uint16_t *__restrict__ w2t,
+                                                 const float *__restrict__ params,
+                                                 float *__restrict__ y) {
+  __shared__ __align__(16) uint8_t act_u[kInfM * kActStride * 2];  // histogram, then a1
+  __shared__ __align__(16) uint16_t ws[kMlpHid * kWKP];
+  __shared__ float ypart[kInfM];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t row0 = (uint64_t)blockIdx.x * kInfM;
+  uint32_t *hist = reinterpret_cast<uint32_t *>(act_u);
+  for (int i = tid; i < kInfM * kHistStride / 4; i += 512) hist[i] = 0;
+  if (tid < kInfM) ypart[tid] = 0.f;
+  __syncthreads();
+
+  // 1. Counts: chunk c = 16 bytes of tape c / 8; bigram (t[i], t[i + 1]) for
+  //    i < 127 goes to counter 256 + code.
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int c = tid + 512 * q;
+    const int r = c >> 3, part = c & 7;
+    if (row0 + r < count) {
+      const uint8_t *src = tapes + (row0 + r) * kTape2 + part * 16;
+      const uint4 v = *reinterpret_cast<const uint4 *>(src);
+      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+      const uint32_t after = part < 7 ? src[16] : 0u;
+      uint32_t *hr = hist + r * (kHistStride / 4);
+#pragma unroll
+      for (int b = 0; b < 16; b++) {
+        const uint32_t cur = (wd[b >> 2] >> (8 * (b & 3))) & 255u;
+        atomicAdd(&hr[cur >> 2], 1u << (8 * (cur & 3)));
+        if (b < 15 || part < 7) {
+          const uint32_t nb = b < 15 ? (wd[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 255u : after;
+          const uint32_t code = 256 + bigram_code(cur, nb);
+          atomicAdd(&hr[code >> 2], 1u << (8 * (code & 3)));
+        }
+      }
+    }
+  }
+
+  const int r16 = lane & 15, kg = lane >> 4;
+  const int rb = 32 * (w >> 1), cb = 128 * (w & 1);
+  f32x4 acc[2][8];
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) acc[t][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // 2. a1 = relu(x (E W1) + b1).
+  for (int kc = 0; kc < kMlpIn; kc += kWKC) {
+    __syncthreads();  // counts complete / previous chunk consumed
+    stage_weights<kMlpIn>(ew1t, kc, ws);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kWKC; ks += 32) {
+      bf16x8 fa[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+        fa[t] = counts_to_bf16(*reinterpret_cast<const uint64_t *>(
+            act_u + (rb + 16 * t + r16) * kHistStride + kc + ks + 8 * kg));
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bf16x8 fb =
+            *reinterpret_cast<const bf16x8 *>(ws + (cb + 16 * b + r16) * kWKP + ks + 8 * kg);
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+          acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[t], fb, acc[t][b], 0, 0, 0);
+      }
+    }
+  }
+  __syncthreads();  // every histogram read is done before a1 overwrites it
+  uint16_t *act = reinterpret_cast<uint16_t *>(act_u);
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int col = cb + 16 * b + r16;
+      const float bias = params[kOffB1 + col];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        act[(rb + 16 * t + 4 * kg + r) * kActStride + col] = f2bf(fmaxf(acc[t][b][r] + bias, 0.f));
+        acc[t][b][r] = 0.f;
+      }
+    }
+
+  // 3. a2 = relu(a1 W2 + b2).
+  for (int kc = 0; kc < kMlpHid; kc += kWKC) {
+    __syncthreads();  // a1 complete / previous chunk consumed
+    stage_weights<kMlpHid>(w2t, kc, ws);
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kWKC; ks += 32) {
+      bf16x8 fa[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++)
+        fa[t] = *reinterpret_cast<const bf16x8 *>(act + (rb + 16 * t + r16) * kActStride + kc +
+                                                  ks + 8 * kg);
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        const bf16x8 fb =
+            *reinterpret_cast<const bf16x8 *>(ws + (cb + 16 * b + r16) * kWKP + ks + 8 * kg);
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+          acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[t], fb, acc[t][b], 0, 0, 0);
+      }
+    }
+  }
+
+  // 4. y = relu(.) . w3 + b3; a2 is rounded to bf16 as in the unfused path.
+  float part[2][4] = {};
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const int col = cb + 16 * b + r16;
+      const float bias = params[kOffB2 + col], w3 = params[kOffW3 + col];
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        part[t][r] += bf2f(f2bf(fmaxf(acc[t][b][r] + bias, 0.f))) * w3;
+    }
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      float v = part[t][r];
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+      if (r16 == 0) atomicAdd(&ypart[rb + 16 * t + 4 * kg + r], v);
+    }
+  __syncthreads();
+  if (tid < kInfM && row0 + tid < count) y[row0 + tid] = ypart[tid] + params[kOffB3];
+}
+
+hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
+                       hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  if (!m.folded_valid) {
+    hipError_t e = mlp_fold(m, st);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)((count + kInfM - 1) / kInfM)), dim3(512), 0, st,
+                     tapes, count, m.ew1t, m.w2t, m.params, y);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -436,25 +644,6 @@ hipError_t mlp_features_rows(Mlp &m, const uint8_t *stage, const uint32_t *rows,
   return hipGetLastError();
 }
 
-hipError_t mlp_infer(Mlp &m, uint64_t count, hipStream_t st) {
-  if (count > m.capacity) return hipErrorInvalidValue;
-  const uint64_t M = pad128(count);
-  if (M == 0) return hipSuccess;
-  if (!m.folded_valid) {
-    hipError_t e = mlp_fold(m, st);
-    if (e != hipSuccess) return e;
-  }
-  const dim3 grid((unsigned)(M / 128)), block(512);
-  // a1 = relu(x (E W1) + b1);  a2 = relu(a1 W2 + b2)
-  hipLaunchKernelGGL((k_gemm_fwd<kMlpIn, kEpiBiasRelu>), grid, block, 0, st, m.x, m.ew1t,
-                     m.params + kOffB1, m.a1);
-  hipLaunchKernelGGL((k_gemm_fwd<kMlpHid, kEpiBiasRelu>), grid, block, 0, st, m.a1, m.w2t,
-                     m.params + kOffB2, m.a2);
-  hipLaunchKernelGGL(k_out, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, m.a2,
-                     m.params, m.y, count);
-  return hipGetLastError();
-}
-
 hipError_t mlp_forward(Mlp &m, uint64_t count, hipStream_t st) {
   if (count > m.capacity) return hipErrorInvalidValue;
   const uint64_t M = pad128(count);
@@ -477,15 +666,16 @@ hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st) {
   const uint64_t M = pad64(count);
   hipError_t e;
   if ((e = hipMemsetAsync(m.loss, 0, 4, st))) return e;
-  if ((e = hipMemsetAsync(m.grads + kOffB1, 0, kMlpHid * 4, st))) return e;
-  if ((e = hipMemsetAsync(m.grads + kOffB2, 0, (2 * kMlpHid + 1) * 4, st))) return e;
+  if ((e = hipMemsetAsync(m.grads, 0, kNumParams * 4, st))) return e;  // all accumulated
+  const int sk = wgrad_splits(M);
   const dim3 red((unsigned)((count + kRedRows - 1) / kRedRows));
-  hipLaunchKernelGGL(k_out_bwd, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st, m.a2,
-                     m.params, m.y, m.target, m.d2, count, M, m.loss);
+  hipLaunchKernelGGL(k_out_bwd, dim3((unsigned)M), dim3(kMlpHid), 0, st, m.a2, m.params, m.y,
+                     m.target, m.d2, count, m.loss);
   hipLaunchKernelGGL(k_out_wgrad, red, dim3(kMlpHid), 0, st, m.a2, m.y, count, m.grads);
   // dW2 = a1^T d2, db2
-  if ((e = gemm<false, false, kEpiF32>(st, kMlpHid, kMlpHid, (int)M, m.a1, 1, kMlpHid, m.d2,
-                                       kMlpHid, 1, nullptr, m.grads + kOffW2, kMlpHid, nullptr)))
+  if ((e = gemm<false, false, kEpiF32Add>(st, kMlpHid, kMlpHid, (int)M, m.a1, 1, kMlpHid,
+                                          m.d2, kMlpHid, 1, nullptr, m.grads + kOffW2, kMlpHid,
+                                          nullptr, sk)))
     return e;
   hipLaunchKernelGGL(k_bias_grad, red, dim3(kMlpHid), 0, st, m.d2, count, m.grads + kOffB2);
   // d1 = (d2 W2^T) * relu'(a1)
@@ -493,8 +683,9 @@ hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st) {
                                           1, kMlpHid, nullptr, m.d1, kMlpHid, m.a1)))
     return e;
   // dW1 = p^T d1, db1
-  if ((e = gemm<false, false, kEpiF32>(st, kMlpHid, kMlpHid, (int)M, m.p, 1, kMlpHid, m.d1,
-                                       kMlpHid, 1, nullptr, m.grads + kOffW1, kMlpHid, nullptr)))
+  if ((e = gemm<false, false, kEpiF32Add>(st, kMlpHid, kMlpHid, (int)M, m.p, 1, kMlpHid,
+                                          m.d1, kMlpHid, 1, nullptr, m.grads + kOffW1, kMlpHid,
+                                          nullptr, sk)))
     return e;
   hipLaunchKernelGGL(k_bias_grad, red, dim3(kMlpHid), 0, st, m.d1, count, m.grads + kOffB1);
   // dp = d1 W1^T
@@ -502,8 +693,9 @@ hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st) {
                                       kMlpHid, nullptr, m.dp, kMlpHid, nullptr)))
     return e;
   // dE = x^T dp
-  if ((e = gemm<false, false, kEpiF32>(st, kMlpIn, kMlpHid, (int)M, m.x, 1, kMlpIn, m.dp,
-                                       kMlpHid, 1, nullptr, m.grads + kOffE, kMlpHid, nullptr)))
+  if ((e = gemm<false, false, kEpiF32Add>(st, kMlpIn, kMlpHid, (int)M, m.x, 1, kMlpIn, m.dp,
+                                          kMlpHid, 1, nullptr, m.grads + kOffE, kMlpHid,
+                                          nullptr, sk)))
     return e;
   return hipGetLastError();
 }

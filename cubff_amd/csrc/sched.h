@@ -23,6 +23,7 @@ struct SchedState {
   uint32_t *cnt = nullptr;         // [num_lists * 2 * kBuckets] counts + cursors
   uint32_t *hist = nullptr;        // prediction histogram
   uint32_t *thr = nullptr;         // bucket thresholds (prediction bins)
+  float *pred = nullptr;           // [total_slots] predicted log2 length
   uint32_t *train_rows = nullptr;  // [kTrainRows]
   uint64_t train_rows_n = 0;
   uint64_t epochs_seen = 0, trainings = 0;

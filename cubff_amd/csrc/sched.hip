@@ -34,14 +34,19 @@ __global__ void k_pred_hist(const float *__restrict__ y, uint64_t count,
 }
 
 // thr[b] = lowest prediction bin of bucket b (walking down from the top).
-__global__ void k_pred_thresholds(const uint32_t *__restrict__ hist, uint64_t count,
-                                  uint32_t *__restrict__ thr) {
+// The histogram is first copied to LDS by the whole block.
+__global__ __launch_bounds__(256) void k_pred_thresholds(const uint32_t *__restrict__ hist,
+                                                         uint64_t count,
+                                                         uint32_t *__restrict__ thr) {
+  __shared__ uint32_t h[kPredBins];
+  for (int i = threadIdx.x; i < kPredBins; i += blockDim.x) h[i] = hist[i];
+  __syncthreads();
   if (threadIdx.x != 0) return;
   const float F[kBuckets - 1] = {0.005f, 0.02f, 0.05f, 0.1f, 0.2f, 0.4f, 0.7f};
   uint64_t run = 0;
   int b = 0;
   for (int bin = kPredBins - 1; bin >= 0 && b < kBuckets - 1; bin--) {
-    run += hist[bin];
+    run += h[bin];
     while (b < kBuckets - 1 && (double)run >= F[b] * (double)count) thr[b++] = (uint32_t)bin;
   }
   for (; b < kBuckets - 1; b++) thr[b] = 0;
@@ -112,14 +117,17 @@ __global__ void k_sched_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
 
 int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
   if (s.ready) return 0;
-  if (mlp_alloc(s.mlp, d.total_slots) != hipSuccess) return -1;
+  // Training sample: kTrainRows slots spread evenly over the batch.
+  s.train_rows_n = d.total_slots < kTrainRows ? d.total_slots / 64 * 64 : kTrainRows;
+  if (s.train_rows_n == 0) s.train_rows_n = d.total_slots;
+  // Activation buffers are only needed for the training rows; inference is
+  // fused (mlp_predict).
+  if (mlp_alloc(s.mlp, s.train_rows_n) != hipSuccess) return -1;
+  if (hipMalloc(&s.pred, d.total_slots * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.bucket, d.total_slots * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.cnt, (num_lists ? num_lists : 1) * 2 * kBuckets * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.hist, kPredBins * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.thr, kBuckets * 4) != hipSuccess) return -1;
-  // Training sample: kTrainRows slots spread evenly over the batch.
-  s.train_rows_n = d.total_slots < kTrainRows ? d.total_slots / 64 * 64 : kTrainRows;
-  if (s.train_rows_n == 0) s.train_rows_n = d.total_slots;
   std::vector<uint32_t> rows(s.train_rows_n);
   for (uint64_t i = 0; i < s.train_rows_n; i++)
     rows[i] = (uint32_t)((i * d.total_slots) / s.train_rows_n);
@@ -140,6 +148,7 @@ void sched_free(SchedState &s) {
   if (s.cnt) (void)hipFree(s.cnt);
   if (s.hist) (void)hipFree(s.hist);
   if (s.thr) (void)hipFree(s.thr);
+  if (s.pred) (void)hipFree(s.pred);
   if (s.train_rows) (void)hipFree(s.train_rows);
   s = SchedState{};
 }
@@ -148,21 +157,20 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const u
                        uint32_t *order, const SchedList *lists, size_t num_lists) {
   if (!s.ready) return hipErrorNotReady;
   hipError_t e;
-  if ((e = mlp_features(s.mlp, d.stage, d.total_slots, st))) return e;
-  if ((e = mlp_infer(s.mlp, d.total_slots, st))) return e;
+  if ((e = mlp_predict(s.mlp, d.stage, d.total_slots, s.pred, st))) return e;
   hipLaunchKernelGGL(k_sched_fill, dim3(1), dim3(256), 0, st, s.cnt, 0u,
                      (uint64_t)num_lists * 2 * kBuckets);
   hipLaunchKernelGGL(k_sched_fill, dim3(kPredBins / 256), dim3(256), 0, st, s.hist, 0u,
                      (uint64_t)kPredBins);
-  hipLaunchKernelGGL(k_pred_hist, dim3(256), dim3(256), 0, st, s.mlp.y, d.total_slots, s.hist);
-  hipLaunchKernelGGL(k_pred_thresholds, dim3(1), dim3(64), 0, st, s.hist, d.total_slots, s.thr);
+  hipLaunchKernelGGL(k_pred_hist, dim3(256), dim3(256), 0, st, s.pred, d.total_slots, s.hist);
+  hipLaunchKernelGGL(k_pred_thresholds, dim3(1), dim3(256), 0, st, s.hist, d.total_slots, s.thr);
   for (size_t li = 0; li < num_lists; li++) {
     const uint64_t n = lists[li].count;
     if (n == 0) continue;
     const unsigned g = (unsigned)((n + 255) / 256);
     uint32_t *cnt = s.cnt + li * 2 * kBuckets;
     hipLaunchKernelGGL(k_sched_count, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
-                       s.mlp.y, s.thr, s.bucket, cnt);
+                       s.pred, s.thr, s.bucket, cnt);
     hipLaunchKernelGGL(k_sched_prefix, dim3(1), dim3(64), 0, st, cnt);
     hipLaunchKernelGGL(k_sched_scatter, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
                        s.bucket, cnt + kBuckets, order + lists[li].begin);

@@ -225,6 +225,11 @@ int cubff_mlp_set_params(cubff_mlp *h, const float *params);
 int cubff_mlp_get_params(cubff_mlp *h, float *params);
 int cubff_mlp_forward(cubff_mlp *h, const uint8_t *tapes, uint64_t count,
                       float *pred);
+/* The scheduler's inference path: one fused kernel (features, both hidden
+ * layers with E W1 folded, output).  Agrees with cubff_mlp_forward to bf16
+ * rounding of the folded first layer. */
+int cubff_mlp_predict(cubff_mlp *h, const uint8_t *tapes, uint64_t count,
+                      float *pred);
 /* Loss 0.5 * mean((pred - target)^2) and its gradient (same layout as the
  * parameters). */
 int cubff_mlp_gradients(cubff_mlp *h, const uint8_t *tapes,

@@ -84,3 +84,14 @@ def forward_backward(params, tapes, targets, bf16_points=True, bf16_weights=True
 BLOCKS = {"E": (OFF_E, OFF_W1), "W1": (OFF_W1, OFF_B1), "b1": (OFF_B1, OFF_W2),
           "W2": (OFF_W2, OFF_B2), "b2": (OFF_B2, OFF_W3), "w3": (OFF_W3, OFF_B3),
           "b3": (OFF_B3, NUM_PARAMS)}
+
+
+def predict_folded(params, tapes):
+    """The scheduler's fused inference: first layer folded in fp32 and rounded
+    to bf16 as one matrix, activations a1, a2 rounded to bf16."""
+    P = unpack(params)
+    x = features(tapes).astype(np.float32)
+    EW1 = bf16((P["E"].astype(np.float64) @ P["W1"].astype(np.float64)).astype(np.float32))
+    a1 = bf16(np.maximum(x @ EW1 + P["b1"], 0))
+    a2 = bf16(np.maximum(a1 @ bf16(P["W2"]) + P["b2"], 0))
+    return a2 @ P["w3"] + P["b3"][0]

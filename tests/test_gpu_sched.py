@@ -37,6 +37,23 @@ def test_mlp_forward_matches_cpu(capi):
     assert _rel(y, y_64) < 3e-2
 
 
+def test_mlp_fused_predict_matches_cpu(capi):
+    # 300 tapes: two full 128-tape blocks and a ragged one.
+    tapes = biased_tapes("bff_noheads", 300, seed=21)
+    tapes[:8] = 0                     # single-bin histograms (counts of 128)
+    tapes[8:16] = np.arange(128)      # every byte once
+    with capi.Mlp(512, init_seed=6) as m:
+        params = m.get_params()
+        y = m.predict(tapes)
+        y_fwd = m.forward(tapes)
+    ref = R.predict_folded(params, tapes)
+    # Same rounding points; fp32 accumulation order is the only difference.
+    assert _rel(y, ref) < 5e-3
+    assert np.max(np.abs(y - ref)) < 2e-2 * max(1.0, np.max(np.abs(ref)))
+    # The unfolded forward differs by the bf16 rounding of p = x E.
+    assert _rel(y, y_fwd) < 3e-2
+
+
 def test_mlp_gradients_match_cpu(capi):
     tapes = biased_tapes("bff_noheads", 256, seed=12)
     targets = np.random.default_rng(0).uniform(0, 13, 256).astype(np.float32)
