@@ -95,20 +95,22 @@ def batched_config():
     return ["bff_noheads"] * S, ns, list(range(S)), probs
 
 
-def run_batched(capi, sched, epochs, warmup, autotune):
-    langs, ns, seeds, probs = batched_config()
+def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
+    langs, ns, seeds, probs = spec if spec is not None else batched_config()
     with capi.Batch(langs, ns) as b:
         b.init(seeds, probs)
         tuned = None
-        e0 = 0
-        if autotune:
-            wg, tpl, table = b.autotune(0, 1, sched)
-            tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
-            e0 = 12
-        b.run_epochs(e0, warmup, sched)
+        # Warm-up first (the learned schedule trains on its first epoch), then
+        # the launch-shape sweep on live epochs, then the timed epochs.
+        b.run_epochs(0, warmup, sched)
         b.sync()
+        e0 = warmup
+        if autotune:
+            wg, tpl, table = b.autotune(e0, 1, sched)
+            tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
+            e0 += 12
         b.reset_counters()
-        tot, it = b.run_epochs_timed(e0 + warmup, epochs, sched)
+        tot, it = b.run_epochs_timed(e0, epochs, sched)
         ops = steps = 0
         for i in range(len(ns)):
             o, _, st = b.read_counters(i)
@@ -126,6 +128,82 @@ def run_batched(capi, sched, epochs, warmup, autotune):
         return out
 
 
+def run_soup_timed(capi, lang, n, seed, prob, warmup, epochs, mode=None, pattern=None):
+    """Single-soup epochs timed with HIP events; returns rates."""
+    mode = capi.PAIR_SHUFFLE if mode is None else mode
+    with capi.Soup(lang, n) as g:
+        g.init_programs(capi.epoch_seed(seed, 0))
+        if pattern is not None:
+            g.set_interactions(*pattern)
+        g.run_epochs(mode, seed, 0, warmup, prob)
+        g.sync()
+        g.reset_counters()
+        if mode == capi.PAIR_2D:
+            # The 2-D matching loops on the host until every edge is decided,
+            # so it is timed by the wall clock around synchronised epochs.
+            t0 = time.perf_counter()
+            g.run_epochs(mode, seed, warmup, epochs, prob)
+            g.sync()
+            tot, kern = (time.perf_counter() - t0) * 1e3, float("nan")
+        else:
+            tot, kern = g.run_epochs_timed(mode, seed, warmup, epochs, prob)
+        ops, runs, steps = g.read_counters(with_steps=True)
+    return {"instr_per_s": ops / (tot / 1e3), "epochs_per_s": epochs / (tot / 1e3),
+            "ms_per_epoch": tot / epochs,
+            "kernel_ms_per_epoch": None if kern != kern else kern / epochs,
+            "epochs_timed": epochs}
+
+
+def run_other_configs(capi, epochs):
+    """The other benchmark configurations, each on one GPU."""
+    from cubff_amd.patterns import grid_pattern
+    out = {}
+    P = 1 << 18
+    langs = {}
+    for lang in ("bff", "bff8", "forth", "forthtrivial", "subleq", "rsubleq4"):
+        langs[lang] = run_soup_timed(capi, lang, 131072, 1, P, 4, epochs)
+    langs["bff_noheads_grid_512x256"] = run_soup_timed(
+        capi, "bff_noheads", 512 * 256, 1, P, 4, epochs, mode=capi.PAIR_2D,
+        pattern=grid_pattern(512, 256, 2))
+    out["all_interpreters"] = {
+        "workload": "2^17 programs, seed 1, Fisher-Yates pairing; plus bff_noheads on a "
+                    "512x256 grid with radius-2 neighbour pairing", "langs": langs}
+    # Large soup with the self-rep check on device every epoch.
+    n, seed, P4 = 1 << 22, 7, 257698
+    with capi.Soup("bff_noheads", n) as g:
+        g.init_programs(capi.epoch_seed(seed, 0))
+        g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 2, P4)
+        g.sync()
+        g.reset_counters()
+        k = max(2, epochs // 4)
+        t0 = time.perf_counter()
+        tot, kern = g.run_epochs_timed(capi.PAIR_SHUFFLE, seed, 2, k, P4)
+        ops, _, _ = g.read_counters(with_steps=True)
+        t1 = time.perf_counter()
+        g.selfrep(capi.epoch_seed(seed, 2 + k - 1), n)
+        t_sr = time.perf_counter() - t1
+    out["large_soup"] = {
+        "workload": "bff_noheads, 2^22 programs, seed 7, P=257698 (p=0.00024)",
+        "instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / k, "epochs_timed": k,
+        "selfrep_ms_per_epoch": t_sr * 1e3,
+        "epochs_per_s_with_selfrep": 1e3 / (tot / k + t_sr * 1e3)}
+    # Learned scheduler on mixed languages.
+    langs5 = ["bff" if s % 2 == 0 else "forth" for s in range(100, 132)]
+    ns5 = [65536] * 32
+    seeds5 = list(range(100, 132))
+    probs5 = [P] * 32
+    res = {}
+    for name, sched in (("plain", capi.SCHED_PLAIN), ("learned_sched", capi.SCHED_LEARNED)):
+        res[name] = run_batched(capi, sched, epochs, 8, autotune=True,
+                                spec=(langs5, ns5, seeds5, probs5))
+    res["learned_vs_plain_batched"] = (res["learned_sched"]["instr_per_s"] /
+                                       res["plain"]["instr_per_s"])
+    res["workload"] = ("32 soups x 2^16 programs, seeds 100..131, bff on even seeds and "
+                       "forth on odd seeds, Fisher-Yates")
+    out["mixed_sched"] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +216,8 @@ def main():
                     help="wall-clock budget of the CPU baseline sample (0 = skip)")
     ap.add_argument("--batched-epochs", type=int, default=32,
                     help="timed epochs of the 64-soup batched runs (0 = skip)")
+    ap.add_argument("--config-epochs", type=int, default=16,
+                    help="timed epochs of the other configurations (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the epoch runs on a single MI355X and "
@@ -203,7 +283,7 @@ def main():
     }
     if args.batched_epochs > 0:
         plain = run_batched(capi, capi.SCHED_PLAIN, args.batched_epochs, 8, autotune=True)
-        learned = run_batched(capi, capi.SCHED_LEARNED, args.batched_epochs, 8, autotune=False)
+        learned = run_batched(capi, capi.SCHED_LEARNED, args.batched_epochs, 8, autotune=True)
         line["batched"] = {
             "workload": ("64 bff_noheads soups, seeds 0..63, sizes cycling 2^12/2^14/2^16/"
                          "2^17 programs, mutation log-spaced 1e-5..1e-3, Fisher-Yates"),
@@ -212,6 +292,8 @@ def main():
             "plain_batched_vs_plain_single_soup": plain["instr_per_s"] / line["value"],
             "learned_vs_plain_batched": learned["instr_per_s"] / plain["instr_per_s"],
         }
+    if args.config_epochs > 0:
+        line["configs"] = run_other_configs(capi, args.config_epochs)
     if args.cpu_seconds > 0:
         line["cpu_baseline"] = cpu_baseline(lang, n, seed, prob, args.cpu_seconds)
     print(json.dumps(line))

@@ -24,6 +24,7 @@ struct BatchDev {
   uint8_t *stage;              // total_slots * 128
   uint32_t *tgt;               // total_slots * 2, global program indices
   uint32_t *pair_ops, *pair_steps;     // total_slots
+  uint32_t *pair_work;                 // total_slots: commands actually interpreted
   unsigned long long *counters;        // num_soups * kNumBatchCounters
   uint64_t *epoch;             // device scalar: the epoch being run
   // Fisher-Yates scratch (total_programs entries; fy_cnt has one more).

@@ -47,15 +47,34 @@ __global__ void k_b_fy_draw(const BatchDev d) {
   atomicAdd(&d.fy_cnt[j], 1u);
 }
 
-// Contiguous item ranges per bucket; every lane of the block takes part in
-// the wavefront allocation (blockDim is a multiple of 64).
-__global__ void k_b_fy_alloc(const BatchDev d) {
+// Contiguous item ranges per bucket: a block-wide scan of the counts and
+// one global atomic per block of 1024.
+__global__ __launch_bounds__(1024) void k_b_fy_alloc(const BatchDev d) {
+  __shared__ uint32_t wtot[16], bbase;
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = p < d.total_programs;
   const uint32_t c = in ? d.fy_cnt[p] : 0u;
-  const uint32_t base = wave_alloc_u32(&d.fy_cnt[d.total_programs], c);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  if (lane == 63) wtot[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int i = 0; i < 16; i++) {
+      const uint32_t x = wtot[i];
+      wtot[i] = run;
+      run += x;
+    }
+    bbase = run ? atomicAdd(&d.fy_cnt[d.total_programs], run) : 0u;
+  }
+  __syncthreads();
   if (!in) return;
-  d.fy_base[p] = base;
+  d.fy_base[p] = bbase + wtot[w] + incl - c;
   d.fy_cursor[p] = 0;
 }
 
@@ -179,8 +198,8 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
     }
     const uint32_t s = soup_of(d.slot_base, d.num_soups, g);
     const uint64_t k = g - d.slot_base[s];
-    uint32_t ops = 0, steps = 0;
-    if (k < d.num_indices[s]) ops = L::evaluate(t, aux, kEvalSteps, &steps);
+    uint32_t ops = 0, steps = 0, work = 0;
+    if (k < d.num_indices[s]) ops = L::evaluate(t, aux, kEvalSteps, &steps, &work);
     const uint32_t p1 = d.tgt[2 * (uint64_t)g], p2 = d.tgt[2 * (uint64_t)g + 1];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
@@ -193,6 +212,7 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
     }
     d.pair_ops[g] = ops;
     d.pair_steps[g] = steps;
+    d.pair_work[g] = work;
   }
 }
 
@@ -243,7 +263,7 @@ hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d) {
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_b_fill, dim3(nblk(n + 1, 256)), dim3(256), 0, st, d.fy_cnt, 0u, n + 1);
   hipLaunchKernelGGL(k_b_fy_draw, dim3(g), dim3(256), 0, st, d);
-  hipLaunchKernelGGL(k_b_fy_alloc, dim3(g), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_b_fy_alloc, dim3(nblk(n, 1024)), dim3(1024), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_place, dim3(g), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_sort, dim3(g), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_final, dim3(g), dim3(256), 0, st, d);

@@ -171,7 +171,8 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
       (r = alloc(&b->prob_d, num_soups)) || (r = alloc(&d.soup, P * kTape)) ||
       (r = alloc(&d.shuf, P)) || (r = alloc(&d.stage, S * kTape2)) ||
       (r = alloc(&d.tgt, 2 * S)) || (r = alloc(&d.pair_ops, S)) ||
-      (r = alloc(&d.pair_steps, S)) || (r = alloc(&ctr, (uint64_t)num_soups * kNumBatchCounters)) ||
+      (r = alloc(&d.pair_steps, S)) || (r = alloc(&d.pair_work, S)) ||
+      (r = alloc(&ctr, (uint64_t)num_soups * kNumBatchCounters)) ||
       (r = alloc(&d.epoch, 1)) || (r = alloc(&d.fy_j, P)) || (r = alloc(&cnt, P + 1)) ||
       (r = alloc(&d.fy_base, P)) || (r = alloc(&d.fy_cursor, P)) ||
       (r = alloc(&d.fy_items, P)) || (r = alloc(&d.fy_first, P)) ||
@@ -243,6 +244,7 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->d.tgt);
   release(b->d.pair_ops);
   release(b->d.pair_steps);
+  release(b->d.pair_work);
   release(b->d.counters);
   release(b->d.epoch);
   release(b->d.fy_j);

@@ -248,7 +248,8 @@ struct BffLang {
   // byte's class flips the affected bits.
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A & /*aux*/,
-                                      uint32_t stepcount, uint32_t *steps) {
+                                      uint32_t stepcount, uint32_t *steps,
+                                      uint32_t *work = nullptr) {
     Mask128 mc{0, 0}, ml{0, 0}, me{0, 0};
     for (int w = 0; w < kTape2 / 8; w++) {
       uint64_t bc = 0, bl = 0, be = 0;
@@ -276,7 +277,7 @@ struct BffLang {
       h1 = 0;
       pc = 0;
     }
-    uint32_t i, nskip;
+    uint32_t i, nskip, iters = 0;  // iters: commands actually interpreted
     {
       const int nxt = mc.next_from(pc);
       i = nskip = min((uint32_t)(nxt - pc), stepcount);
@@ -294,6 +295,7 @@ struct BffLang {
       uint32_t ai = 0, askip = 0, aw0 = 0, aw1 = 0;
       bool wrote = false, moved = false, branched = false;
       for (;;) {
+        iters++;
         const uint32_t c = t.get(pc);
         const uint32_t v0 = t.get(h0);
         const uint32_t v1 = t.get(h1);
@@ -430,6 +432,7 @@ struct BffLang {
       }
     }
     *steps = i;
+    if (work) *work = iters;
     return i - nskip;
   }
 };
@@ -501,7 +504,8 @@ struct ForthLang {
   // the lane's aux LDS region.
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &stk,
-                                      uint32_t stepcount, uint32_t *steps) {
+                                      uint32_t stepcount, uint32_t *steps,
+                                      uint32_t *work = nullptr) {
     int sp = 0;
     bool overflow = false;
     int pos = 0;
@@ -584,6 +588,7 @@ struct ForthLang {
       if (pos >= kTape2 || pos < 0 || overflow) break;
     }
     *steps = i;
+    if (work) *work = i;
     return i - nops;
   }
 };
@@ -596,7 +601,7 @@ struct SubleqLang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
-                                      uint32_t *steps) {
+                                      uint32_t *steps, uint32_t *work = nullptr) {
     int pos = 0;
     uint32_t i = 0;
     while (i < stepcount) {
@@ -610,6 +615,7 @@ struct SubleqLang {
       if (pos + 3 > kTape2 || pos < 0) break;
     }
     *steps = i;
+    if (work) *work = i;
     return i;
   }
 };
@@ -618,7 +624,7 @@ struct Rsubleq4Lang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
-                                      uint32_t *steps) {
+                                      uint32_t *steps, uint32_t *work = nullptr) {
     int pos = 0;
     uint32_t i = 0;
     while (i < stepcount) {
@@ -633,6 +639,7 @@ struct Rsubleq4Lang {
       if (pos + 4 > kTape2 || pos < 0) break;
     }
     *steps = i;
+    if (work) *work = i;
     return i;
   }
 };

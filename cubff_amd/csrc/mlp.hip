@@ -257,43 +257,42 @@ __global__ void k_out(const uint16_t *__restrict__ a2, const float *__restrict__
   y[r] = s;
 }
 
-// dy = (y - t) / n (zero for padding rows); d2 = dy * w3 * relu'(a2);
-// loss += 0.5 * (y - t)^2 / n.  One block per row, one thread per column;
-// dy is left in y.
+// dy = (y - t) / n (zero for padding rows); d2 = dy * w3 * relu'(a2).  One
+// block per row, one thread per column; dy is left in y.
 __global__ __launch_bounds__(kMlpHid) void k_out_bwd(
     const uint16_t *__restrict__ a2, const float *__restrict__ params, float *__restrict__ y,
-    const float *__restrict__ target, uint16_t *__restrict__ d2, uint64_t count,
-    float *__restrict__ loss) {
+    const float *__restrict__ target, uint16_t *__restrict__ d2, uint64_t count) {
   const uint64_t r = blockIdx.x;
   const int j = threadIdx.x;
-  float dy = 0.f;
-  if (r < count) {
-    const float e = y[r] - target[r];
-    dy = e / (float)count;
-    if (j == 0) atomicAdd(loss, 0.5f * e * e / (float)count);
-  }
+  const float dy = r < count ? (y[r] - target[r]) / (float)count : 0.f;
   __syncthreads();  // every thread has read y[r] before it is overwritten
   if (j == 0) y[r] = dy;
   const uint16_t a = a2[r * kMlpHid + j];
   d2[r * kMlpHid + j] = f2bf(a ? dy * params[kOffW3 + j] : 0.f);
 }
 
-// dw3[j] += sum over a chunk of rows of dy[r] a2[r][j]; db3 likewise.
-// Grid: one block of 256 threads (one per j) per kRedRows rows; results are
-// accumulated with atomics into zeroed gradients.
+// dw3[j] += sum over a chunk of rows of dy[r] a2[r][j]; db3 likewise; and
+// the loss 0.5 mean((y - t)^2) = sum 0.5 n dy^2.  Grid: one block of 256
+// threads (one per j) per kRedRows rows; results are accumulated with atomics
+// into zeroed gradients.
 constexpr int kRedRows = 32;
 __global__ void k_out_wgrad(const uint16_t *__restrict__ a2, const float *__restrict__ dy,
-                            uint64_t count, float *__restrict__ grads) {
+                            uint64_t count, float *__restrict__ grads,
+                            float *__restrict__ loss) {
   const int j = threadIdx.x;
   const uint64_t r0 = (uint64_t)blockIdx.x * kRedRows;
   const uint64_t r1 = min(count, r0 + kRedRows);
-  float s = 0.f, sb = 0.f;
+  float s = 0.f, sb = 0.f, sl = 0.f;
   for (uint64_t r = r0; r < r1; r++) {
     s += dy[r] * bf2f(a2[r * kMlpHid + j]);
     sb += dy[r];
+    sl += 0.5f * (float)count * dy[r] * dy[r];
   }
   atomicAdd(&grads[kOffW3 + j], s);
-  if (j == 0) atomicAdd(&grads[kOffB3], sb);
+  if (j == 0) {
+    atomicAdd(&grads[kOffB3], sb);
+    atomicAdd(loss, sl);
+  }
 }
 
 // db[j] += sum over a chunk of rows of d[r][j].
@@ -349,13 +348,15 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
 //   3. a2 = relu(a1 W2 + b2) with a1 kept in LDS as bf16 (over the histogram);
 //   4. y = a2 . w3 + b3, reduced across lanes and waves in LDS.
 //
-// Block: 8 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
-// columns 128 (w % 2) .. +128 (eight 16-column blocks).  Fragment layout of
+// Block: 4 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
+// columns 128 (w % 2) .. +128 (eight 16-column blocks).  About 71 KB of LDS,
+// so two blocks share a CU and one's staging overlaps the other's MFMAs.  Fragment layout of
 // the 16x16x32 MFMA: lane l holds A(l % 16, 8 (l / 16) + 0..7),
 // B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kInfM = 128;        // tapes per block
+constexpr int kInfM = 64;         // tapes per block
+constexpr int kInfThreads = 256;
 constexpr int kHistStride = 520;  // bytes per histogram row (512 counters + pad)
 constexpr int kActStride = 264;   // bf16 per activation row (256 + pad)
 constexpr int kWKC = 64;          // weight chunk along K
@@ -376,15 +377,15 @@ template <int K>
 __device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ WT, int kc,
                                               uint16_t *__restrict__ ws) {
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int idx = threadIdx.x + 512 * c;  // 2048 chunks of 8 bf16
+  for (int c = 0; c < 2048 / kInfThreads; c++) {
+    const int idx = threadIdx.x + kInfThreads * c;  // 2048 chunks of 8 bf16
     const int n = idx >> 3, part = idx & 7;
     *reinterpret_cast<uint4 *>(ws + n * kWKP + part * 8) =
         *reinterpret_cast<const uint4 *>(WT + (int64_t)n * K + kc + part * 8);
   }
 }
 
-__global__ __launch_bounds__(512) void k_predict(const uint8_t *__restrict__ tapes,
+__global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restrict__ tapes,
                                                  uint64_t count,
                                                  const uint16_t *__restrict__ ew1t,
                                                  const uint16_t *__restrict__ w2t,
@@ -396,15 +397,15 @@ __global__ __launch_bounds__(512) void k_predict(const uint8_t *__restrict__ tap
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const uint64_t row0 = (uint64_t)blockIdx.x * kInfM;
   uint32_t *hist = reinterpret_cast<uint32_t *>(act_u);
-  for (int i = tid; i < kInfM * kHistStride / 4; i += 512) hist[i] = 0;
+  for (int i = tid; i < kInfM * kHistStride / 4; i += kInfThreads) hist[i] = 0;
   if (tid < kInfM) ypart[tid] = 0.f;
   __syncthreads();
 
   // 1. Counts: chunk c = 16 bytes of tape c / 8; bigram (t[i], t[i + 1]) for
   //    i < 127 goes to counter 256 + code.
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int c = tid + 512 * q;
+  for (int q = 0; q < kInfM * 8 / kInfThreads; q++) {
+    const int c = tid + kInfThreads * q;
     const int r = c >> 3, part = c & 7;
     if (row0 + r < count) {
       const uint8_t *src = tapes + (row0 + r) * kTape2 + part * 16;
@@ -525,7 +526,8 @@ hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
     hipError_t e = mlp_fold(m, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_predict, dim3((unsigned)((count + kInfM - 1) / kInfM)), dim3(512), 0, st,
+  hipLaunchKernelGGL(k_predict, dim3((unsigned)((count + kInfM - 1) / kInfM)), dim3(kInfThreads),
+                     0, st,
                      tapes, count, m.ew1t, m.w2t, m.params, y);
   return hipGetLastError();
 }
@@ -670,8 +672,9 @@ hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st) {
   const int sk = wgrad_splits(M);
   const dim3 red((unsigned)((count + kRedRows - 1) / kRedRows));
   hipLaunchKernelGGL(k_out_bwd, dim3((unsigned)M), dim3(kMlpHid), 0, st, m.a2, m.params, m.y,
-                     m.target, m.d2, count, m.loss);
-  hipLaunchKernelGGL(k_out_wgrad, red, dim3(kMlpHid), 0, st, m.a2, m.y, count, m.grads);
+                     m.target, m.d2, count);
+  hipLaunchKernelGGL(k_out_wgrad, red, dim3(kMlpHid), 0, st, m.a2, m.y, count, m.grads,
+                     m.loss);
   // dW2 = a1^T d2, db2
   if ((e = gemm<false, false, kEpiF32Add>(st, kMlpHid, kMlpHid, (int)M, m.a1, 1, kMlpHid,
                                           m.d2, kMlpHid, 1, nullptr, m.grads + kOffW2, kMlpHid,

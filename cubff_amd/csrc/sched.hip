@@ -184,7 +184,9 @@ hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int ste
   hipError_t e;
   const uint64_t n = s.train_rows_n;
   if ((e = mlp_features_rows(s.mlp, d.stage, s.train_rows, n, st))) return e;
-  if ((e = mlp_targets_from_ops(s.mlp, d.pair_ops, s.train_rows, n, st))) return e;
+  // Targets: the commands each slot's interpreter actually ran (the cost
+  // the schedule balances), not the reference's op count.
+  if ((e = mlp_targets_from_ops(s.mlp, d.pair_work, s.train_rows, n, st))) return e;
   // The first round starts from random weights: train it longer.
   if (s.trainings == 0) steps *= 8;
   for (int i = 0; i < steps; i++) {
