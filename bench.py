@@ -83,6 +83,23 @@ def cpu_baseline(lang: str, n: int, seed: int, prob: int, budget_s: float):
             "sample": f"oracle OpenMP port, epochs 1..{e} of the same workload"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_bff_noheads_2p17.md")
+
+
+def traffic_per_launch():
+    """HBM bytes per k_epoch launch from the committed PMC summary (FETCH_SIZE
+    + WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
+    plain copy of known size), or None when it has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 BATCH_SIZES = [4096, 16384, 65536, 131072]
 
 
@@ -218,7 +235,21 @@ def main():
                     help="timed epochs of the 64-soup batched runs (0 = skip)")
     ap.add_argument("--config-epochs", type=int, default=16,
                     help="timed epochs of the other configurations (0 = skip)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB "
+                         "(four launches), the reference for HBM byte counters")
     args = ap.parse_args()
+    if args.calibrate_copy_mib > 0:
+        import torch
+        nbytes = args.calibrate_copy_mib << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 4}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the epoch runs on a single MI355X and "
               "this build has no multi-GPU path (use --gpus 1)", file=sys.stderr)
@@ -273,7 +304,7 @@ def main():
             "peak": PEAK_LANE_OPS / 1e12,
             "unit": "T int32 lane-ops/s",
             "frac": achieved / PEAK_LANE_OPS,
-            "traffic": None,
+            "traffic": traffic_per_launch(),
             "kernel": "k_epoch (mutate + evaluate + scatter)",
             "kernel_ms_avg": kern_ms / K,
             "work_per_launch": work / K,

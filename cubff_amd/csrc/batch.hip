@@ -175,7 +175,7 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
                                                     uint64_t count, uint32_t per_lane) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<LANES>;
-  using Aux = LdsTape<LANES>;
+  using Aux = typename L::template Aux<LANES>;
   constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
   __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   const int lane = threadIdx.x;

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch(
     uint32_t *__restrict__ pair_steps, unsigned long long *__restrict__ counters) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<kEpochLanes>;
-  using Aux = LdsTape<kEpochLanes>;
+  using Aux = typename L::template Aux<kEpochLanes>;
   constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
   __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   __shared__ unsigned long long block_ops, block_steps;
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kSelfrepLanes) void k_selfrep(
   using L = typename LangOf<LANG>::T;
   constexpr int kLanes = kSelfrepLanes;  // kSelfrepProgs * 13 (+ padding)
   using Tape = typename L::template Tape<kLanes>;
-  using Aux = LdsTape<kLanes>;
+  using Aux = typename L::template Aux<kLanes>;
   constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
   __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   __shared__ uint32_t res[kSelfrepProgs][2];
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_eval_tapes(
     uint32_t *__restrict__ ops_out, uint32_t *__restrict__ steps_out) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<kEpochLanes>;
-  using Aux = LdsTape<kEpochLanes>;
+  using Aux = typename L::template Aux<kEpochLanes>;
   constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
   __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
   const int lane = threadIdx.x;

@@ -223,6 +223,7 @@ struct BffClass {
 template <int MAP, bool HEADS, int LOGIC>
 struct BffLang {
   template <int LANES> using Tape = LinTape<LANES>;
+  template <int LANES> using Aux = LdsTape<LANES>;  // unused
   using C = BffClass<MAP>;
   // Per-kind fields, 2 bits each, indexed by BffKind: head-0 delta + 1,
   // head-1 delta + 1 (bff_selfmove's `,` also advances head 1), byte delta + 1.
@@ -485,9 +486,91 @@ __device__ __forceinline__ int forth_kind(uint32_t c) {
   return fNoop;
 }
 
+// Table-driven step of the Forth family.  Every op below 16 is described
+// by nibbles of five 64-bit constants (one nibble per opcode); bytes 16..63
+// share opcode 15's descriptor (a no-op in every variant), bytes 64..127 are
+// constants and 128..255 jumps.  A step is then one straight-line sequence
+// for every lane, whatever its opcode:
+//   x, y   = the top two stack bytes (0 where the stack is shorter)
+//   m      = tape[((rsrc ? y : x) & rmask) + rbase]
+//   pop npop, push v0 (and v1), write wv = (wvsrc ? m : x) to
+//            tape[((wsrc ? y : x) & wmask) + wbase]
+// with the reference's Stack semantics: Pop on an empty stack yields 0 and
+// leaves it empty, Push on a full stack sets the overflow flag and stores
+// nothing (forth.inc.h:236-249).
+struct ForthOpDesc {
+  int npop, npush;              // values popped / pushed
+  int rsrc, rmask127, rbase64;  // read address m: source (0 x, 1 y), mask, base
+  int write, wsrc, wmask127, wbase64, wvsrc;  // tape write: address, value (0 x, 1 m)
+  int v0;                       // 0 m, 1 x, 2 x+1, 3 x-1, 4 x+y, 5 x-y, 6 x^64
+  int v1y;                      // second push: 0 x, 1 y
+  int if0, noop, reset;         // skip next if x != 0; counts as no-op; reset check
+};
+
+constexpr ForthOpDesc forth_desc(int lang, int c) {
+  // forth.inc.h:251-373 (and forthtrivial_reset.cu:123-252 for `reset`).
+  constexpr ForthOpDesc kRead0{1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr ForthOpDesc kRead1{1, 1, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr ForthOpDesc kRead{1, 1, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr ForthOpDesc kWrite{2, 0, 0, 0, 0, 1, 1, 1, 0, 0, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kWrite0{2, 0, 0, 0, 0, 1, 1, 0, 0, 0, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kWrite1{2, 0, 0, 0, 0, 1, 1, 0, 1, 0, 1, 0, 0, 0, 1};
+  constexpr ForthOpDesc kDup{1, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kDrop{1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kSwap{2, 2, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0};
+  constexpr ForthOpDesc kIf0{1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 1, 0, 0};
+  constexpr ForthOpDesc kInc{1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0};
+  constexpr ForthOpDesc kDec{1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 3, 0, 0, 0, 0};
+  constexpr ForthOpDesc kAdd{2, 1, 0, 0, 0, 0, 0, 0, 0, 0, 4, 0, 0, 0, 0};
+  constexpr ForthOpDesc kSub{2, 1, 0, 0, 0, 0, 0, 0, 0, 0, 5, 0, 0, 0, 0};
+  constexpr ForthOpDesc kCopy0{1, 0, 0, 0, 0, 1, 0, 0, 1, 1, 1, 0, 0, 0, 1};
+  constexpr ForthOpDesc kCopy1{1, 0, 0, 0, 1, 1, 0, 0, 0, 1, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kCopy{2, 0, 1, 1, 0, 1, 0, 1, 0, 1, 1, 0, 0, 0, 0};
+  constexpr ForthOpDesc kXor{1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 6, 0, 0, 0, 0};
+  constexpr ForthOpDesc kNop{0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0};
+  switch (c) {
+    case 0: return lang == kForthcopy ? kRead : kRead0;
+    case 1: return lang == kForthcopy ? kWrite : kRead1;
+    case 2: return lang == kForthcopy ? kCopy : kWrite0;
+    case 3: return lang == kForthcopy ? kXor : kWrite1;
+    case 4: return kDup;
+    case 5: return kDrop;
+    case 6: return kSwap;
+    case 7: return kIf0;
+    case 8: return kInc;
+    case 9: return kDec;
+    case 10: return kAdd;
+    case 11: return kSub;
+    case 12: return (lang == kForthtrivial || lang == kForthtrivialReset) ? kCopy0 : kNop;
+    case 13: return (lang == kForthtrivial || lang == kForthtrivialReset) ? kCopy1 : kNop;
+    default: return kNop;
+  }
+}
+
+// Nibble table: nibble c of the result is f(forth_desc(lang, c)).
+template <class F>
+constexpr uint64_t forth_nibbles(int lang, F f) {
+  uint64_t r = 0;
+  for (int c = 0; c < 16; c++) r |= (uint64_t)(f(forth_desc(lang, c)) & 15) << (4 * c);
+  return r;
+}
+
 template <int LANG>
 struct ForthLang {
-  template <int LANES> using Tape = LdsTape<LANES>;
+  template <int LANES> using Tape = LinTape<LANES>;
+  template <int LANES> using Aux = LinTape<LANES>;  // the stack; row 128 is a dummy
+  static constexpr bool kReset = LANG == kForthtrivialReset;
+  static constexpr uint64_t kNibPop = forth_nibbles(
+      LANG, [](ForthOpDesc d) { return d.npop | d.npush << 2; });
+  static constexpr uint64_t kNibRead = forth_nibbles(
+      LANG, [](ForthOpDesc d) { return d.rsrc | d.rmask127 << 1 | d.rbase64 << 2 | d.wvsrc << 3; });
+  static constexpr uint64_t kNibWrite = forth_nibbles(
+      LANG, [](ForthOpDesc d) { return d.write | d.wsrc << 1 | d.wmask127 << 2 | d.wbase64 << 3; });
+  static constexpr uint64_t kNibVal = forth_nibbles(
+      LANG, [](ForthOpDesc d) { return d.v0 | d.v1y << 3; });
+  static constexpr uint64_t kNibFlag = forth_nibbles(
+      LANG, [](ForthOpDesc d) { return d.if0 | d.noop << 1 | d.reset << 2; });
+
   // Reset of forthtrivial_reset.cu:58-68.
   template <class T>
   __device__ static void reset(const T &t, int pos) {
@@ -500,91 +583,70 @@ struct ForthLang {
   }
 
   // Evaluate (forth.inc.h:379-397) with EvaluateOne (forth.inc.h:251-373,
-  // forthtrivial_reset.cu:123-252).  The stack (forth.inc.h:236-249) lives in
-  // the lane's aux LDS region.
+  // forthcopy.cu, forthtrivial.cu, forthtrivial_reset.cu:123-252).
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &stk,
                                       uint32_t stepcount, uint32_t *steps,
                                       uint32_t *work = nullptr) {
+    constexpr int kDummy = kTape2;
     int sp = 0;
-    bool overflow = false;
     int pos = 0;
     uint32_t i = 0, nops = 0;
-    auto push = [&](uint32_t v) {
-      if (sp != 128) stk.set(sp++, (uint8_t)v);
-      else overflow = true;
-    };
-    auto pop = [&]() -> uint32_t { return sp == 0 ? 0u : stk.get(--sp); };
-    constexpr bool kReset = LANG == kForthtrivialReset;
+    bool overflow = false;
     while (i < stepcount) {
       i++;
-      const uint32_t cmd = t.get(pos);
-      switch (forth_kind<LANG>(cmd)) {
-        case fRead0: { uint32_t a = pop() % kTape; push(t.get(a)); break; }
-        case fRead1: { uint32_t a = pop() % kTape; push(t.get(kTape + a)); break; }
-        case fRead: { uint32_t a = pop() % kTape2; push(t.get(a)); break; }
-        case fWrite: {
-          uint32_t v = pop();
-          uint32_t a = pop() % kTape2;
-          t.set(a, (uint8_t)v);
-          break;
-        }
-        case fWrite0: {
-          uint32_t v = pop();
-          uint32_t a = pop() % kTape;
-          t.set(a, (uint8_t)v);
-          break;
-        }
-        case fWrite1: {
-          uint32_t v = pop();
-          uint32_t a = pop() % kTape;
-          if (kReset && t.get(kTape + a) == 0x13) reset(t, pos);
-          else t.set(kTape + a, (uint8_t)v);
-          break;
-        }
-        case fDup: { uint32_t v = pop(); push(v); push(v); break; }
-        case fDrop: pop(); break;
-        case fSwap: {
-          uint32_t a = pop();
-          uint32_t b = pop();
-          push(a);
-          push(b);
-          break;
-        }
-        case fIf0: {
-          uint32_t v = pop();
-          if (v) pos++;
-          push(v);
-          break;
-        }
-        case fInc: push(pop() + 1); break;
-        case fDec: push(pop() - 1); break;
-        case fAdd: { uint32_t a = pop(); uint32_t b = pop(); push(a + b); break; }
-        case fSub: { uint32_t a = pop(); uint32_t b = pop(); push(a - b); break; }
-        case fConst: push(cmd & 63); break;
-        case fCopy0: {
-          uint32_t a = pop() % kTape;
-          if (kReset && t.get(kTape + a) == 0x13) reset(t, pos);
-          else t.set(kTape + a, t.get(a));
-          break;
-        }
-        case fCopy1: { uint32_t a = pop() % kTape; t.set(a, t.get(kTape + a)); break; }
-        case fCopy: {
-          uint32_t to = pop() % kTape2;
-          uint32_t from = pop() % kTape2;
-          t.set(to, t.get(from));
-          break;
-        }
-        case fXor: push(pop() ^ 64); break;
-        case fJmp: {
-          int a = (int)(cmd & 63) + 1;
-          pos += (cmd & 64) ? -a : a;
-          pos--;
-          break;
-        }
-        default: nops++; break;
+      const uint32_t c = t.get(pos);
+      const uint32_t xs = stk.get(sp > 0 ? sp - 1 : kDummy);
+      const uint32_t ys = stk.get(sp > 1 ? sp - 2 : kDummy);
+      const uint32_t x = sp > 0 ? xs : 0u, y = sp > 1 ? ys : 0u;
+      const uint32_t sh = 4 * (c < 16 ? c : 15u);
+      const bool is_const = c - 64u < 64u, is_jmp = c >= 128;
+      const uint32_t np = (uint32_t)(kNibPop >> sh) & 15;
+      const uint32_t rd = (uint32_t)(kNibRead >> sh) & 15;
+      const uint32_t wr = (uint32_t)(kNibWrite >> sh) & 15;
+      const uint32_t vl = (uint32_t)(kNibVal >> sh) & 15;
+      const uint32_t fl = (uint32_t)(kNibFlag >> sh) & 15;
+      const uint32_t raddr = ((rd & 1 ? y : x) & (rd & 2 ? 127u : 63u)) + (rd & 4 ? 64u : 0u);
+      const uint32_t m = t.get(raddr);
+      // Stack effect (constants push c & 63, jumps touch nothing).
+      const int npop = (is_const || is_jmp) ? 0 : (int)(np & 3);
+      const int npush = is_jmp ? 0 : (is_const ? 1 : (int)(np >> 2));
+      uint32_t v0;
+      switch (is_const ? 7u : vl & 7) {  // selects, no branches after if-conversion
+        case 0: v0 = m; break;
+        case 1: v0 = x; break;
+        case 2: v0 = x + 1; break;
+        case 3: v0 = x - 1; break;
+        case 4: v0 = x + y; break;
+        case 5: v0 = x - y; break;
+        case 6: v0 = x ^ 64u; break;
+        default: v0 = c & 63u; break;
       }
-      pos++;
+      const uint32_t v1 = vl & 8 ? y : x;
+      const int sp1 = sp > npop ? sp - npop : 0;
+      const bool ok0 = npush >= 1 && sp1 < 128;
+      const bool ok1 = npush >= 2 && ok0 && sp1 + 1 < 128;
+      overflow = (npush >= 1 && !ok0) || (npush >= 2 && !ok1);
+      stk.set(ok0 ? sp1 : kDummy, (uint8_t)v0);
+      stk.set(ok1 ? sp1 + 1 : kDummy, (uint8_t)v1);
+      sp = sp1 + (int)ok0 + (int)ok1;
+      // Tape write.
+      const bool wr_on = (wr & 1) && !is_const && !is_jmp;
+      const uint32_t waddr = ((wr & 2 ? y : x) & (wr & 4 ? 127u : 63u)) + (wr & 8 ? 64u : 0u);
+      const uint32_t wv = rd & 8 ? m : x;
+      bool do_reset = false;
+      if constexpr (kReset) do_reset = wr_on && (fl & 4) && t.get(waddr) == 0x13;
+      if (do_reset) {
+        reset(t, pos);
+      } else {
+        t.set(wr_on ? waddr : (uint32_t)kDummy, (uint8_t)wv);
+      }
+      nops += (fl & 2) && !is_const && !is_jmp;
+      // Next position.
+      int delta = 1;
+      if (is_jmp) delta = (c & 64) ? -(int)(c & 63) - 1 : (int)(c & 63) + 1;
+      else if ((fl & 1) && !is_const && x != 0) delta = 2;
+      pos += delta;
       if (pos >= kTape2 || pos < 0 || overflow) break;
     }
     *steps = i;
@@ -599,6 +661,7 @@ struct ForthLang {
 
 struct SubleqLang {
   template <int LANES> using Tape = LdsTape<LANES>;
+  template <int LANES> using Aux = LdsTape<LANES>;  // unused
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
                                       uint32_t *steps, uint32_t *work = nullptr) {
@@ -622,6 +685,7 @@ struct SubleqLang {
 
 struct Rsubleq4Lang {
   template <int LANES> using Tape = LdsTape<LANES>;
+  template <int LANES> using Aux = LdsTape<LANES>;  // unused
   template <class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
                                       uint32_t *steps, uint32_t *work = nullptr) {

@@ -348,15 +348,14 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
 //   3. a2 = relu(a1 W2 + b2) with a1 kept in LDS as bf16 (over the histogram);
 //   4. y = a2 . w3 + b3, reduced across lanes and waves in LDS.
 //
-// Block: 4 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
-// columns 128 (w % 2) .. +128 (eight 16-column blocks).  About 71 KB of LDS,
-// so two blocks share a CU and one's staging overlaps the other's MFMAs.  Fragment layout of
+// Block: 8 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
+// columns 128 (w % 2) .. +128 (eight 16-column blocks).  Fragment layout of
 // the 16x16x32 MFMA: lane l holds A(l % 16, 8 (l / 16) + 0..7),
 // B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kInfM = 64;         // tapes per block
-constexpr int kInfThreads = 256;
+constexpr int kInfM = 128;        // tapes per block
+constexpr int kInfThreads = 512;
 constexpr int kHistStride = 520;  // bytes per histogram row (512 counters + pad)
 constexpr int kActStride = 264;   // bf16 per activation row (256 + pad)
 constexpr int kWKC = 64;          // weight chunk along K
