@@ -30,13 +30,18 @@ __host__ __device__ __forceinline__ uint64_t epoch_seed_pre(uint64_t sm_seed,
 // lane l sits at LDS dword (w * LANES + l).  Every lane therefore always hits
 // bank (l mod #banks), whatever byte it reads, so data-dependent accesses from
 // 64 lanes never conflict.  Consecutive tape bytes share a dword.
+// Pointers into LDS (address space 3): 32-bit addresses, so per-lane byte
+// addresses are one 32-bit multiply-add.
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
 template <int LANES>
 struct LdsTape {
   static constexpr int kRegionBytes = kTape2 * LANES;
-  uint8_t *p;  // LDS address of this lane's dword 0
+  lds_u8 *p;  // this lane's dword 0
 
   __device__ __forceinline__ static LdsTape at(uint8_t *region, int lane) {
-    return LdsTape{region + lane * 4};
+    return LdsTape{(lds_u8 *)region + lane * 4};
   }
   __device__ __forceinline__ uint32_t off(uint32_t i) const {
     return (i >> 2) * (LANES * 4) + (i & 3);
@@ -46,10 +51,10 @@ struct LdsTape {
     p[off(i)] = v;
   }
   __device__ __forceinline__ uint32_t word(uint32_t w) const {
-    return *reinterpret_cast<const uint32_t *>(p + w * (LANES * 4));
+    return *(const lds_u32 *)(p + w * (LANES * 4));
   }
   __device__ __forceinline__ void set_word(uint32_t w, uint32_t v) const {
-    *reinterpret_cast<uint32_t *>(p + w * (LANES * 4)) = v;
+    *(lds_u32 *)(p + w * (LANES * 4)) = v;
   }
 };
 

@@ -35,8 +35,11 @@ inline const char *const kLangNames[kNumLangs] = {
     "forthcopy",    "forthtrivial",       "forthtrivial_reset", "subleq",
     "rsubleq4"};
 
+// Languages whose interpreter uses a second per-lane LDS region: the Forth
+// stack, and the tape snapshot of the BFF cycle check.
 constexpr bool lang_needs_aux(int lang) {
-  return lang >= kForth && lang <= kForthtrivialReset;
+  return (lang >= kForth && lang <= kForthtrivialReset) || lang == kBff ||
+         lang == kBffNoheads || lang == kBffNoheads4bit || lang == kBffPerm;
 }
 
 // ---------------------------------------------------------------------------
@@ -116,10 +119,10 @@ struct LinTape {
   static constexpr int kStride = LANES + 4;
   static constexpr int kRegionBytes = (kTape2 + 1) * kStride;
   static constexpr int kDummy = kTape2;
-  uint8_t *p;
+  lds_u8 *p;
 
   __device__ __forceinline__ static LinTape at(uint8_t *region, int lane) {
-    return LinTape{region + lane};
+    return LinTape{(lds_u8 *)region + lane};
   }
   __device__ __forceinline__ uint8_t get(uint32_t i) const { return p[i * kStride]; }
   __device__ __forceinline__ void set(uint32_t i, uint8_t v) const {
@@ -144,11 +147,12 @@ struct Mask128 {
     const uint64_t b = (uint64_t)f << (p & 63);
     if (p < 64) lo ^= b; else hi ^= b;
   }
-  // Smallest set position >= p (0 <= p <= 128), or 128.
+  // Smallest set position >= p (0 <= p <= 128), or 128.  Branch-free.
   __device__ __forceinline__ int next_from(int p) const {
-    const uint64_t l = p >= 64 ? 0 : lo & (~0ULL << p);
-    const uint64_t h = p <= 64 ? hi : (p >= 128 ? 0 : hi & (~0ULL << (p - 64)));
-    return l ? __builtin_ctzll(l) : (h ? 64 + __builtin_ctzll(h) : 128);
+    const uint64_t l = lo & (p >= 64 ? 0ULL : ~0ULL << (p & 63));
+    const uint64_t h = hi & (p <= 64 ? ~0ULL : (p >= 128 ? 0ULL : ~0ULL << (p & 63)));
+    const int cl = __builtin_ctzg(l, 64), ch = __builtin_ctzg(h, 64);
+    return l ? cl : 64 + ch;
   }
   // Largest set position <= p (-1 <= p <= 127), or -1.
   __device__ __forceinline__ int prev_upto(int p) const {
@@ -247,8 +251,19 @@ struct BffLang {
   // after the target, next command from there) until a write changes any
   // byte's class.  Masks always equal the tape: every write that changes a
   // byte's class flips the affected bits.
+  // Full-state cycle check (standard jump logic): from kCycleStart steps
+  // on, the state right after a taken backward jump is compared with an
+  // anchor whose tape is kept in the aux region; anchors move to later jumps
+  // after windows of kCycleStart, 2 kCycleStart, ... steps (Brent), so a
+  // cycle shorter than the remaining budget is found.  A match (pc, heads,
+  // then every tape byte) means the run repeats with that period.
+  static constexpr uint32_t kCycleStart = 512;
+  __device__ __forceinline__ static uint32_t cycle_weight(int p) {
+    return 2654435761u * (2u * (uint32_t)p + 1u);
+  }
+
   template <class T, class A>
-  __device__ static uint32_t evaluate(const T &t, const A & /*aux*/,
+  __device__ static uint32_t evaluate(const T &t, const A &aux,
                                       uint32_t stepcount, uint32_t *steps,
                                       uint32_t *work = nullptr) {
     Mask128 mc{0, 0}, ml{0, 0}, me{0, 0};
@@ -295,11 +310,17 @@ struct BffLang {
       int aq = -1, ah0 = 0, ah1 = 0;
       uint32_t ai = 0, askip = 0, aw0 = 0, aw1 = 0;
       bool wrote = false, moved = false, branched = false;
+      // Cycle-check anchor: jump source (-1 none yet, -2 done), heads, step
+      // and no-op counts, window, and a hash of the tape changes since.
+      int cq = -1, ch0 = 0, ch1 = 0;
+      uint32_t ci = 0, cskip = 0, cwin = kCycleStart, cdelta = 0;
       for (;;) {
         iters++;
         const uint32_t c = t.get(pc);
         const uint32_t v0 = t.get(h0);
         const uint32_t v1 = t.get(h1);
+        // Keep the three loads together: one LDS round trip per command.
+        __builtin_amdgcn_sched_barrier(0);
         int fall = mc.next_from(pc + 1);
         const uint32_t k = C::kind(c);
         if constexpr (kAccel) {
@@ -314,6 +335,9 @@ struct BffLang {
             ((k == kCopy10 ? v1 : v0) + ((kTDelta >> (2 * k)) & 3) - 1) & 0xff;
         const bool changed = (k - kPlus) < 4u && wval != wold;
         t.set(changed ? waddr : T::kDummy, (uint8_t)wval);
+        if constexpr (kAccel) {
+          if (changed) cdelta += (wval - wold) * cycle_weight(waddr);
+        }
         h0 = (h0 + (int)((kTH0 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
         h1 = (h1 + (int)((kTH1 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
         if (changed) {
@@ -341,35 +365,35 @@ struct BffLang {
           const bool z = LOGIC == kLogicSelfmove ? v0 == 0 : C::null(v0);
           const bool tf = k == kLoopStart && z;
           const bool tb = k == kLoopEnd && !z;
-          if (tb && pc != bsrc) {
-            // Backward scan to the matching loop start; unmatched -> stop.
-            int depth = 1, p = pc - 1, q;
+          if ((tb && pc != bsrc) || (tf && pc != fsrc)) {
+            // Bracket scan to the matching `[` (backward, unmatched -> stop)
+            // or `]` (forward, unmatched -> 128), then cached.
+            int depth = 1, p = tb ? pc - 1 : pc + 1, q;
             for (;;) {
-              const int ql = ml.prev_upto(p), qe = me.prev_upto(p);
-              q = max(ql, qe);
-              if (q < 0) break;
-              depth += (q == qe) ? 1 : -1;
+              if (tb) {
+                const int ql = ml.prev_upto(p), qe = me.prev_upto(p);
+                q = max(ql, qe);
+                if (q < 0) break;
+                depth += (q == qe) ? 1 : -1;
+                p = q - 1;
+              } else {
+                const int ql = ml.next_from(p), qe = me.next_from(p);
+                q = min(ql, qe);
+                if (q >= kTape2) break;
+                depth += (q == ql) ? 1 : -1;
+                p = q + 1;
+              }
               if (depth == 0) break;
-              p = q - 1;
             }
-            bsrc = pc;
-            bafter = q + 1;
-            bnext = q < 0 ? -1 : mc.next_from(q + 1);
-          }
-          if (tf && pc != fsrc) {
-            // Forward scan to the matching loop end; unmatched -> 128.
-            int depth = 1, p = pc + 1, q;
-            for (;;) {
-              const int ql = ml.next_from(p), qe = me.next_from(p);
-              q = min(ql, qe);
-              if (q >= kTape2) break;
-              depth += (q == ql) ? 1 : -1;
-              if (depth == 0) break;
-              p = q + 1;
+            if (tb) {
+              bsrc = pc;
+              bafter = q + 1;
+              bnext = q < 0 ? -1 : mc.next_from(q + 1);
+            } else {
+              fsrc = pc;
+              fafter = min(q + 1, kTape2);
+              fnext = mc.next_from(fafter);
             }
-            fsrc = pc;
-            fafter = min(q + 1, kTape2);
-            fnext = mc.next_from(fafter);
           }
           after = tb ? bafter : (tf ? fafter : pc + 1);
           nextc = tb ? bnext : (tf ? fnext : fall);
@@ -417,6 +441,27 @@ struct BffLang {
                 aw0 = v0;
                 aw1 = v1;
                 wrote = moved = branched = false;
+                if (i1 >= kCycleStart && cq != -2) {
+                  bool same = pc == cq && h0 == ch0 && h1 == ch1 && cdelta == 0;
+                  for (int w = 0; same && w < kTape2 / 4; w++) same = t.word(w) == aux.word(w);
+                  if (same) {
+                    const uint32_t len = i1 - ci;
+                    const uint32_t reps = (stepcount - i1) / len;
+                    i += reps * len;
+                    nskip += reps * (nskip - cskip);
+                    cq = -2;  // fewer than one period of steps is left
+                    aq = -1;  // the loop anchor's counts predate the skip
+                  } else if (cq < 0 || i1 - ci >= cwin) {
+                    if (cq >= 0) cwin *= 2;
+                    for (int w = 0; w < kTape2 / 4; w++) aux.set_word(w, t.word(w));
+                    cq = pc;
+                    ch0 = h0;
+                    ch1 = h1;
+                    ci = i1;
+                    cskip = nskip;
+                    cdelta = 0;
+                  }
+                }
               }
             }
           }
@@ -593,12 +638,22 @@ struct ForthLang {
     int pos = 0;
     uint32_t i = 0, nops = 0;
     bool overflow = false;
+    // The top two stack bytes live in registers (0 where the stack is
+    // shorter); the stack in LDS holds every byte.  Each step issues all of
+    // its loads at once: the opcode, the two bytes below the cached pair, and
+    // every tape byte the op could read (x & 63, 64 + (x & 63), y & 127).
+    uint32_t x = 0, y = 0;
     while (i < stepcount) {
       i++;
+      const uint32_t xa = x & 63u;
       const uint32_t c = t.get(pos);
-      const uint32_t xs = stk.get(sp > 0 ? sp - 1 : kDummy);
-      const uint32_t ys = stk.get(sp > 1 ? sp - 2 : kDummy);
-      const uint32_t x = sp > 0 ? xs : 0u, y = sp > 1 ? ys : 0u;
+      const uint32_t s3m = stk.get(sp > 2 ? sp - 3 : kDummy);
+      const uint32_t s4m = stk.get(sp > 3 ? sp - 4 : kDummy);
+      const uint32_t ma = t.get(xa), mb = t.get(xa + 64), my = t.get(y & 127u);
+      uint32_t mr = 0;  // forthtrivial_reset: the byte Write1 would replace
+      if constexpr (kReset) mr = t.get((y & 63u) + 64);
+      __builtin_amdgcn_sched_barrier(0);
+      const uint32_t s3 = sp > 2 ? s3m : 0u, s4 = sp > 3 ? s4m : 0u;
       const uint32_t sh = 4 * (c < 16 ? c : 15u);
       const bool is_const = c - 64u < 64u, is_jmp = c >= 128;
       const uint32_t np = (uint32_t)(kNibPop >> sh) & 15;
@@ -606,22 +661,22 @@ struct ForthLang {
       const uint32_t wr = (uint32_t)(kNibWrite >> sh) & 15;
       const uint32_t vl = (uint32_t)(kNibVal >> sh) & 15;
       const uint32_t fl = (uint32_t)(kNibFlag >> sh) & 15;
-      const uint32_t raddr = ((rd & 1 ? y : x) & (rd & 2 ? 127u : 63u)) + (rd & 4 ? 64u : 0u);
-      const uint32_t m = t.get(raddr);
+      // m = tape[((rsrc ? y : x) & rmask) + rbase], from the loaded candidates.
+      const uint32_t m = (rd & 1) ? my : (((rd & 4) || ((rd & 2) && (x & 64))) ? mb : ma);
       // Stack effect (constants push c & 63, jumps touch nothing).
       const int npop = (is_const || is_jmp) ? 0 : (int)(np & 3);
       const int npush = is_jmp ? 0 : (is_const ? 1 : (int)(np >> 2));
-      uint32_t v0;
-      switch (is_const ? 7u : vl & 7) {  // selects, no branches after if-conversion
-        case 0: v0 = m; break;
-        case 1: v0 = x; break;
-        case 2: v0 = x + 1; break;
-        case 3: v0 = x - 1; break;
-        case 4: v0 = x + y; break;
-        case 5: v0 = x - y; break;
-        case 6: v0 = x ^ 64u; break;
-        default: v0 = c & 63u; break;
-      }
+      // First pushed value: a chain of selects (no divergent branches).
+      const uint32_t sel = is_const ? 7u : vl & 7;
+      uint32_t v0 = m;
+      v0 = sel == 1 ? x : v0;
+      v0 = sel == 2 ? x + 1 : v0;
+      v0 = sel == 3 ? x - 1 : v0;
+      v0 = sel == 4 ? x + y : v0;
+      v0 = sel == 5 ? x - y : v0;
+      v0 = sel == 6 ? x ^ 64u : v0;
+      v0 = sel == 7 ? c & 63u : v0;
+      v0 &= 255u;
       const uint32_t v1 = vl & 8 ? y : x;
       const int sp1 = sp > npop ? sp - npop : 0;
       const bool ok0 = npush >= 1 && sp1 < 128;
@@ -635,7 +690,7 @@ struct ForthLang {
       const uint32_t waddr = ((wr & 2 ? y : x) & (wr & 4 ? 127u : 63u)) + (wr & 8 ? 64u : 0u);
       const uint32_t wv = rd & 8 ? m : x;
       bool do_reset = false;
-      if constexpr (kReset) do_reset = wr_on && (fl & 4) && t.get(waddr) == 0x13;
+      if constexpr (kReset) do_reset = wr_on && (fl & 4) && ((wr & 2) ? mr : mb) == 0x13;
       if (do_reset) {
         reset(t, pos);
       } else {
@@ -647,6 +702,11 @@ struct ForthLang {
       if (is_jmp) delta = (c & 64) ? -(int)(c & 63) - 1 : (int)(c & 63) + 1;
       else if ((fl & 1) && !is_const && x != 0) delta = 2;
       pos += delta;
+      // New cached top pair: the two bytes left after the pops, then pushes.
+      const uint32_t t0 = npop == 0 ? x : (npop == 1 ? y : s3);
+      const uint32_t t1 = npop == 0 ? y : (npop == 1 ? s3 : s4);
+      x = ok1 ? v1 : (ok0 ? v0 : t0);
+      y = ok1 ? v0 : (ok0 ? t0 : t1);
       if (pos >= kTape2 || pos < 0 || overflow) break;
     }
     *steps = i;
