@@ -638,22 +638,12 @@ struct ForthLang {
     int pos = 0;
     uint32_t i = 0, nops = 0;
     bool overflow = false;
-    // The top two stack bytes live in registers (0 where the stack is
-    // shorter); the stack in LDS holds every byte.  Each step issues all of
-    // its loads at once: the opcode, the two bytes below the cached pair, and
-    // every tape byte the op could read (x & 63, 64 + (x & 63), y & 127).
-    uint32_t x = 0, y = 0;
     while (i < stepcount) {
       i++;
-      const uint32_t xa = x & 63u;
       const uint32_t c = t.get(pos);
-      const uint32_t s3m = stk.get(sp > 2 ? sp - 3 : kDummy);
-      const uint32_t s4m = stk.get(sp > 3 ? sp - 4 : kDummy);
-      const uint32_t ma = t.get(xa), mb = t.get(xa + 64), my = t.get(y & 127u);
-      uint32_t mr = 0;  // forthtrivial_reset: the byte Write1 would replace
-      if constexpr (kReset) mr = t.get((y & 63u) + 64);
-      __builtin_amdgcn_sched_barrier(0);
-      const uint32_t s3 = sp > 2 ? s3m : 0u, s4 = sp > 3 ? s4m : 0u;
+      const uint32_t xs = stk.get(sp > 0 ? sp - 1 : kDummy);
+      const uint32_t ys = stk.get(sp > 1 ? sp - 2 : kDummy);
+      const uint32_t x = sp > 0 ? xs : 0u, y = sp > 1 ? ys : 0u;
       const uint32_t sh = 4 * (c < 16 ? c : 15u);
       const bool is_const = c - 64u < 64u, is_jmp = c >= 128;
       const uint32_t np = (uint32_t)(kNibPop >> sh) & 15;
@@ -661,8 +651,8 @@ struct ForthLang {
       const uint32_t wr = (uint32_t)(kNibWrite >> sh) & 15;
       const uint32_t vl = (uint32_t)(kNibVal >> sh) & 15;
       const uint32_t fl = (uint32_t)(kNibFlag >> sh) & 15;
-      // m = tape[((rsrc ? y : x) & rmask) + rbase], from the loaded candidates.
-      const uint32_t m = (rd & 1) ? my : (((rd & 4) || ((rd & 2) && (x & 64))) ? mb : ma);
+      const uint32_t raddr = ((rd & 1 ? y : x) & (rd & 2 ? 127u : 63u)) + (rd & 4 ? 64u : 0u);
+      const uint32_t m = t.get(raddr);
       // Stack effect (constants push c & 63, jumps touch nothing).
       const int npop = (is_const || is_jmp) ? 0 : (int)(np & 3);
       const int npush = is_jmp ? 0 : (is_const ? 1 : (int)(np >> 2));
@@ -676,7 +666,6 @@ struct ForthLang {
       v0 = sel == 5 ? x - y : v0;
       v0 = sel == 6 ? x ^ 64u : v0;
       v0 = sel == 7 ? c & 63u : v0;
-      v0 &= 255u;
       const uint32_t v1 = vl & 8 ? y : x;
       const int sp1 = sp > npop ? sp - npop : 0;
       const bool ok0 = npush >= 1 && sp1 < 128;
@@ -690,7 +679,7 @@ struct ForthLang {
       const uint32_t waddr = ((wr & 2 ? y : x) & (wr & 4 ? 127u : 63u)) + (wr & 8 ? 64u : 0u);
       const uint32_t wv = rd & 8 ? m : x;
       bool do_reset = false;
-      if constexpr (kReset) do_reset = wr_on && (fl & 4) && ((wr & 2) ? mr : mb) == 0x13;
+      if constexpr (kReset) do_reset = wr_on && (fl & 4) && t.get(waddr) == 0x13;
       if (do_reset) {
         reset(t, pos);
       } else {
@@ -702,11 +691,6 @@ struct ForthLang {
       if (is_jmp) delta = (c & 64) ? -(int)(c & 63) - 1 : (int)(c & 63) + 1;
       else if ((fl & 1) && !is_const && x != 0) delta = 2;
       pos += delta;
-      // New cached top pair: the two bytes left after the pops, then pushes.
-      const uint32_t t0 = npop == 0 ? x : (npop == 1 ? y : s3);
-      const uint32_t t1 = npop == 0 ? y : (npop == 1 ? s3 : s4);
-      x = ok1 ? v1 : (ok0 ? v0 : t0);
-      y = ok1 ? v0 : (ok0 ? t0 : t1);
       if (pos >= kTape2 || pos < 0 || overflow) break;
     }
     *steps = i;

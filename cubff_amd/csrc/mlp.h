@@ -1,10 +1,10 @@
 // MLP cost model of the learned scheduler.
 //
-//   x   = [byte histogram | hashed byte-bigram histogram] / 128   (512, bf16)
-//   p   = x E                      (E: 512 x 256, pooled byte embedding)
+//   x   = byte histogram / 128     (256, bf16)
+//   p   = x E                      (E: 256 x 256, pooled byte embedding)
 //   a1  = relu(p W1 + b1)          (256)
 //   a2  = relu(a1 W2 + b2)         (256)
-//   y   = a2 . w3 + b3             predicted log2(ops + 1)
+//   y   = a2 . w3 + b3             predicted log2 length
 //
 // bf16 weights and activations, fp32 accumulation on MFMA; fp32 master
 // weights with Adam.  All GEMMs are hand-written (mlp.hip).
@@ -15,7 +15,7 @@
 
 namespace cubff {
 
-constexpr int kMlpIn = 512, kMlpHid = 256;
+constexpr int kMlpIn = 256, kMlpHid = 256;
 // Offsets of the fp32 parameter vector.
 constexpr uint64_t kOffE = 0;
 constexpr uint64_t kOffW1 = kOffE + (uint64_t)kMlpIn * kMlpHid;
@@ -32,9 +32,9 @@ struct Mlp {
   // bf16 weights: W (row-major [in][out]) and WT ([out][in]) per layer.
   uint16_t *e_w = nullptr, *e_wt = nullptr, *w1 = nullptr, *w1t = nullptr;
   uint16_t *w2 = nullptr, *w2t = nullptr;
-  uint16_t *ew1t = nullptr;            // folded E W1 as [256][512], inference only
+  uint16_t *ew1t = nullptr;            // folded E W1 / 128 as [256][256], inference only
   bool folded_valid = false;
-  uint16_t *x = nullptr;               // [capacity, 512] features
+  uint16_t *x = nullptr;               // [capacity, 256] features
   uint16_t *p = nullptr, *a1 = nullptr, *a2 = nullptr;  // [capacity, 256]
   uint16_t *d2 = nullptr, *d1 = nullptr, *dp = nullptr; // [capacity, 256]
   float *y = nullptr, *target = nullptr;                 // [capacity]

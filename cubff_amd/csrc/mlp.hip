@@ -203,11 +203,8 @@ __global__ __launch_bounds__(512) void k_gemm_fwd(const uint16_t *__restrict__ A
 
 
 // ---------------------------------------------------------------------------
-// Features: 16 tapes per block, 16 threads per tape (8 bytes each).
-__device__ __forceinline__ uint32_t bigram_code(uint32_t a, uint32_t b) {
-  return (a * 167 + b * 13 + ((a * b) >> 3)) & 255;
-}
-
+// Features: byte histogram / 128.  16 tapes per block, 16 threads per tape
+// (8 bytes each).
 __global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ tapes,
                                                   const uint32_t *__restrict__ rows,
                                                   uint64_t count, uint16_t *__restrict__ x) {
@@ -220,14 +217,8 @@ __global__ __launch_bounds__(256) void k_features(const uint8_t *__restrict__ ta
   if (r < count) {
     const uint64_t src = rows ? rows[r] : r;
     const uint8_t *tp = tapes + src * kTape2;
-    uint32_t v[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) v[q] = (c * 8 + q < kTape2) ? tp[c * 8 + q] : 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      atomicAdd(&h[ti][v[q]], 1u);
-      if (c * 8 + q < kTape2 - 1) atomicAdd(&h[ti][256 + bigram_code(v[q], v[q + 1])], 1u);
-    }
+    for (int q = 0; q < 8; q++) atomicAdd(&h[ti][tp[c * 8 + q]], 1u);
   }
   __syncthreads();
   for (int i = t; i < 16 * kMlpIn; i += 256) {
@@ -340,11 +331,11 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
 // per block, with nothing but the tapes read from and the predictions written
 // to memory.
 //
-//   1. byte and bigram counts of each tape into an LDS histogram (four u8
-//      counters per dword, LDS atomics; counts never exceed 128);
-//   2. a1 = relu(x (E W1) + b1): v_mfma_f32_16x16x32_bf16 with A fragments
-//      converted from the counts (count / 128 is exact in bf16) and the
-//      folded weights staged through LDS in K-chunks of 64;
+//   1. byte counts of each tape into an LDS histogram (four u8 counters per
+//      dword, LDS atomics; counts never exceed 128);
+//   2. a1 = relu(counts (E W1 / 128) + b1): v_mfma_f32_16x16x32_bf16 with A
+//      fragments converted from the counts (exact in bf16) and the folded,
+//      pre-scaled weights staged through LDS in K-chunks of 64;
 //   3. a2 = relu(a1 W2 + b2) with a1 kept in LDS as bf16 (over the histogram);
 //   4. y = a2 . w3 + b3, reduced across lanes and waves in LDS.
 //
@@ -356,19 +347,28 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kInfM = 128;        // tapes per block
 constexpr int kInfThreads = 512;
-constexpr int kHistStride = 520;  // bytes per histogram row (512 counters + pad)
+constexpr int kHistStride = 264;  // bytes per histogram row (256 counters + pad)
 constexpr int kActStride = 264;   // bf16 per activation row (256 + pad)
 constexpr int kWKC = 64;          // weight chunk along K
 constexpr int kWKP = 72;          // padded chunk row (bf16)
 
+// Eight u8 counts as bf16 (exact: counts are at most 128).  The 1/128
+// feature scale is folded into the first-layer weights (k_fold), so the
+// counts go to the MFMA as they are: byte -> float (v_cvt_f32_ubyteN), and
+// the high halves of two floats packed into one dword (v_perm_b32).
 __device__ __forceinline__ bf16x8 counts_to_bf16(uint64_t c8) {
-  bf16x8 r;
+  uint32_t w[4];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const float f = (float)((uint32_t)(c8 >> (8 * q)) & 255u) * (1.0f / 128);
-    r[q] = __builtin_bit_cast(__bf16, (uint16_t)(__float_as_uint(f) >> 16));
+  for (int h = 0; h < 2; h++) {
+    const uint32_t v = (uint32_t)(c8 >> (32 * h));
+    const uint32_t f0 = __float_as_uint((float)(v & 255u));
+    const uint32_t f1 = __float_as_uint((float)((v >> 8) & 255u));
+    const uint32_t f2 = __float_as_uint((float)((v >> 16) & 255u));
+    const uint32_t f3 = __float_as_uint((float)(v >> 24));
+    w[2 * h] = __builtin_amdgcn_perm(f1, f0, 0x07060302u);
+    w[2 * h + 1] = __builtin_amdgcn_perm(f3, f2, 0x07060302u);
   }
-  return r;
+  return __builtin_bit_cast(bf16x8, w);
 }
 
 // Stages rows [0, 256) x K-columns [kc, kc + 64) of WT ([256][K] bf16) into ws.
@@ -400,27 +400,19 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
   if (tid < kInfM) ypart[tid] = 0.f;
   __syncthreads();
 
-  // 1. Counts: chunk c = 16 bytes of tape c / 8; bigram (t[i], t[i + 1]) for
-  //    i < 127 goes to counter 256 + code.
+  // 1. Counts: chunk c = 16 bytes of tape c / 8.
 #pragma unroll
   for (int q = 0; q < kInfM * 8 / kInfThreads; q++) {
     const int c = tid + kInfThreads * q;
     const int r = c >> 3, part = c & 7;
     if (row0 + r < count) {
-      const uint8_t *src = tapes + (row0 + r) * kTape2 + part * 16;
-      const uint4 v = *reinterpret_cast<const uint4 *>(src);
+      const uint4 v = *reinterpret_cast<const uint4 *>(tapes + (row0 + r) * kTape2 + part * 16);
       const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-      const uint32_t after = part < 7 ? src[16] : 0u;
       uint32_t *hr = hist + r * (kHistStride / 4);
 #pragma unroll
       for (int b = 0; b < 16; b++) {
         const uint32_t cur = (wd[b >> 2] >> (8 * (b & 3))) & 255u;
         atomicAdd(&hr[cur >> 2], 1u << (8 * (cur & 3)));
-        if (b < 15 || part < 7) {
-          const uint32_t nb = b < 15 ? (wd[(b + 1) >> 2] >> (8 * ((b + 1) & 3))) & 255u : after;
-          const uint32_t code = 256 + bigram_code(cur, nb);
-          atomicAdd(&hr[code >> 2], 1u << (8 * (code & 3)));
-        }
       }
     }
   }
@@ -571,14 +563,16 @@ void mlp_free(Mlp &m) {
   m = Mlp{};
 }
 
-// Folded first layer E W1 (fp32 GEMM on the fp32 masters), stored as bf16
-// [256][512] for the inference GEMM.
+// Folded first layer E W1 / 128 (fp32 GEMM on the fp32 masters), stored as
+// bf16 [256][256] for the inference kernel, which feeds it raw counts.
 __global__ void k_fold(const float *__restrict__ params, uint16_t *__restrict__ ew1t) {
-  const int i = blockIdx.x, j = threadIdx.x;  // i: input (512), j: hidden (256)
+  const int i = blockIdx.x, j = threadIdx.x;  // i: input (256), j: hidden (256)
   const float *E = params + kOffE, *W1 = params + kOffW1;
   float s = 0.f;
   for (int k = 0; k < kMlpHid; k++) s += E[i * kMlpHid + k] * W1[k * kMlpHid + j];
-  ew1t[j * kMlpIn + i] = f2bf(s);
+  // Scaled by the feature normalisation 1/128 (exact in bf16), so inference
+  // multiplies raw counts.
+  ew1t[j * kMlpIn + i] = f2bf(s * (1.0f / 128));
 }
 
 hipError_t mlp_fold(Mlp &m, hipStream_t st) {
@@ -606,7 +600,7 @@ hipError_t mlp_init(Mlp &m, uint64_t seed, hipStream_t st) {
     const uint64_t r = splitmix64(ctr++);
     return s * (2.0f * (float)((r >> 11) * (1.0 / 9007199254740992.0)) - 1.0f);
   };
-  const float se = std::sqrt(6.0f / (kMlpIn / 8 + kMlpHid));  // inputs are sparse histograms
+  const float se = std::sqrt(6.0f / (kMlpIn / 4 + kMlpHid));  // inputs are sparse histograms
   const float sh = std::sqrt(6.0f / (2 * kMlpHid));
   for (uint64_t i = 0; i < (uint64_t)kMlpIn * kMlpHid; i++) h[kOffE + i] = uni(se);
   for (uint64_t i = 0; i < (uint64_t)kMlpHid * kMlpHid; i++) h[kOffW1 + i] = uni(sh);

@@ -20,7 +20,8 @@ struct SchedState {
   bool ready = false;
   Mlp mlp;
   uint32_t *bucket = nullptr;      // [total_slots] bucket of each slot (0 = longest)
-  uint32_t *cnt = nullptr;         // [num_lists * 2 * kBuckets] counts + cursors
+  uint32_t *cnt = nullptr;         // [num_lists * 2 * kBuckets] bucket totals per list
+  uint32_t *blk = nullptr;         // per-block bucket counts, then output offsets
   uint32_t *hist = nullptr;        // prediction histogram
   uint32_t *thr = nullptr;         // bucket thresholds (prediction bins)
   float *pred = nullptr;           // [total_slots] predicted log2 length

@@ -33,23 +33,32 @@ __global__ void k_pred_hist(const float *__restrict__ y, uint64_t count,
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
-// thr[b] = lowest prediction bin of bucket b (walking down from the top).
-// The histogram is first copied to LDS by the whole block.
-__global__ __launch_bounds__(256) void k_pred_thresholds(const uint32_t *__restrict__ hist,
-                                                         uint64_t count,
-                                                         uint32_t *__restrict__ thr) {
-  __shared__ uint32_t h[kPredBins];
-  for (int i = threadIdx.x; i < kPredBins; i += blockDim.x) h[i] = hist[i];
+// thr[b] = lowest prediction bin of bucket b: the largest bin whose suffix
+// count (predictions in that bin or above) reaches F[b] of the batch.  One
+// block of kPredBins threads: a suffix scan in LDS, then each bin checks the
+// seven crossings.
+__global__ __launch_bounds__(kPredBins) void k_pred_thresholds(const uint32_t *__restrict__ hist,
+                                                               uint64_t count,
+                                                               uint32_t *__restrict__ thr) {
+  __shared__ uint32_t suf[kPredBins + 1];
+  const int bin = threadIdx.x;
+  suf[bin] = hist[bin];
+  if (bin == 0) suf[kPredBins] = 0;
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  const float F[kBuckets - 1] = {0.005f, 0.02f, 0.05f, 0.1f, 0.2f, 0.4f, 0.7f};
-  uint64_t run = 0;
-  int b = 0;
-  for (int bin = kPredBins - 1; bin >= 0 && b < kBuckets - 1; bin--) {
-    run += h[bin];
-    while (b < kBuckets - 1 && (double)run >= F[b] * (double)count) thr[b++] = (uint32_t)bin;
+  for (int o = 1; o < kPredBins; o <<= 1) {  // suf[bin] = sum of hist[bin..]
+    const uint32_t v = bin + o < kPredBins ? suf[bin + o] : 0u;
+    __syncthreads();
+    suf[bin] += v;
+    __syncthreads();
   }
-  for (; b < kBuckets - 1; b++) thr[b] = 0;
+  constexpr double F[kBuckets - 1] = {0.005, 0.02, 0.05, 0.1, 0.2, 0.4, 0.7};
+#pragma unroll
+  for (int j = 0; j < kBuckets - 1; j++) {
+    const double need = F[j] * (double)count;
+    const bool here = (double)suf[bin] >= need && (double)suf[bin + 1] < need;
+    if (here) thr[j] = (uint32_t)bin;
+    if (bin == 0 && !(need > 0.0)) thr[j] = 0;  // empty batch
+  }
 }
 
 __device__ __forceinline__ uint32_t bucket_of(float y, const uint32_t *__restrict__ thr) {
@@ -60,13 +69,19 @@ __device__ __forceinline__ uint32_t bucket_of(float y, const uint32_t *__restric
   return k;
 }
 
-__global__ void k_sched_count(const uint32_t *__restrict__ list, uint64_t count,
-                              const float *__restrict__ y, const uint32_t *__restrict__ thr,
-                              uint32_t *__restrict__ bucket, uint32_t *__restrict__ cnt) {
+// Bucketing without contended atomics: each block of 256 slots counts its
+// slots per bucket (blk[block][bucket]), one block turns the counts into
+// output offsets (bucket-major, blocks in order), and each block scatters its
+// slots from its offsets.
+constexpr int kSchedBlock = 256;
+
+__global__ __launch_bounds__(kSchedBlock) void k_sched_count(
+    const uint32_t *__restrict__ list, uint64_t count, const float *__restrict__ y,
+    const uint32_t *__restrict__ thr, uint32_t *__restrict__ bucket, uint32_t *__restrict__ blk) {
   __shared__ uint32_t local[kBuckets];
   if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
   if (i < count) {
     const uint32_t g = list[i];
     const uint32_t b = bucket_of(y[g], thr);
@@ -74,40 +89,56 @@ __global__ void k_sched_count(const uint32_t *__restrict__ list, uint64_t count,
     atomicAdd(&local[b], 1u);
   }
   __syncthreads();
-  if (threadIdx.x < kBuckets && local[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], local[threadIdx.x]);
+  if (threadIdx.x < kBuckets) blk[(uint64_t)blockIdx.x * kBuckets + threadIdx.x] = local[threadIdx.x];
 }
 
-// cursor[b] = exclusive prefix of counts (bucket 0 first).
-__global__ void k_sched_prefix(uint32_t *__restrict__ cnt) {
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int b = 0; b < kBuckets; b++) {
-      cnt[kBuckets + b] = run;
-      run += cnt[b];
+// blk[block][b] <- first output position of that block's bucket-b slots;
+// tot[b] <- slots in bucket b.  One block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_sched_offsets(uint32_t *__restrict__ blk,
+                                                        uint32_t nblocks,
+                                                        uint32_t *__restrict__ tot) {
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  const uint32_t per = (nblocks + 1023) / 1024;
+  const uint32_t lo = min(nblocks, tid * per), hi = min(nblocks, lo + per);
+  uint32_t base = 0;
+  for (int b = 0; b < kBuckets; b++) {
+    uint32_t c = 0;
+    for (uint32_t k = lo; k < hi; k++) c += blk[(uint64_t)k * kBuckets + b];
+    part[tid] = c;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const uint32_t v = tid >= o ? part[tid - o] : 0u;
+      __syncthreads();
+      part[tid] += v;
+      __syncthreads();
     }
+    uint32_t run = base + part[tid] - c;
+    for (uint32_t k = lo; k < hi; k++) {
+      const uint32_t n = blk[(uint64_t)k * kBuckets + b];
+      blk[(uint64_t)k * kBuckets + b] = run;
+      run += n;
+    }
+    const uint32_t total = part[1023];
+    __syncthreads();
+    if (tid == 0) tot[b] = total;
+    base += total;
   }
 }
 
-// Block-aggregated scatter: ranks within the block come from LDS counters,
-// and each bucket reserves one range per block with a single global atomic.
-__global__ void k_sched_scatter(const uint32_t *__restrict__ list, uint64_t count,
-                                const uint32_t *__restrict__ bucket,
-                                uint32_t *__restrict__ cursor, uint32_t *__restrict__ order) {
-  __shared__ uint32_t local[kBuckets], base[kBuckets];
+__global__ __launch_bounds__(kSchedBlock) void k_sched_scatter(
+    const uint32_t *__restrict__ list, uint64_t count, const uint32_t *__restrict__ bucket,
+    const uint32_t *__restrict__ blk, uint32_t *__restrict__ order) {
+  __shared__ uint32_t local[kBuckets];
   if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
   __syncthreads();
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t g = 0, b = 0, rank = 0;
+  const uint64_t i = (uint64_t)blockIdx.x * kSchedBlock + threadIdx.x;
   if (i < count) {
-    g = list[i];
-    b = bucket[g];
-    rank = atomicAdd(&local[b], 1u);
+    const uint32_t g = list[i];
+    const uint32_t b = bucket[g];
+    const uint32_t rank = atomicAdd(&local[b], 1u);
+    order[blk[(uint64_t)blockIdx.x * kBuckets + b] + rank] = g;
   }
-  __syncthreads();
-  if (threadIdx.x < kBuckets && local[threadIdx.x])
-    base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], local[threadIdx.x]);
-  __syncthreads();
-  if (i < count) order[base[b] + rank] = g;
 }
 
 __global__ void k_sched_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
@@ -126,6 +157,7 @@ int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
   if (hipMalloc(&s.pred, d.total_slots * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.bucket, d.total_slots * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.cnt, (num_lists ? num_lists : 1) * 2 * kBuckets * 4) != hipSuccess) return -1;
+  if (hipMalloc(&s.blk, (d.total_slots / kSchedBlock + 2) * kBuckets * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.hist, kPredBins * 4) != hipSuccess) return -1;
   if (hipMalloc(&s.thr, kBuckets * 4) != hipSuccess) return -1;
   std::vector<uint32_t> rows(s.train_rows_n);
@@ -146,6 +178,7 @@ void sched_free(SchedState &s) {
   mlp_free(s.mlp);
   if (s.bucket) (void)hipFree(s.bucket);
   if (s.cnt) (void)hipFree(s.cnt);
+  if (s.blk) (void)hipFree(s.blk);
   if (s.hist) (void)hipFree(s.hist);
   if (s.thr) (void)hipFree(s.thr);
   if (s.pred) (void)hipFree(s.pred);
@@ -163,17 +196,18 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const u
   hipLaunchKernelGGL(k_sched_fill, dim3(kPredBins / 256), dim3(256), 0, st, s.hist, 0u,
                      (uint64_t)kPredBins);
   hipLaunchKernelGGL(k_pred_hist, dim3(256), dim3(256), 0, st, s.pred, d.total_slots, s.hist);
-  hipLaunchKernelGGL(k_pred_thresholds, dim3(1), dim3(256), 0, st, s.hist, d.total_slots, s.thr);
+  hipLaunchKernelGGL(k_pred_thresholds, dim3(1), dim3(kPredBins), 0, st, s.hist, d.total_slots,
+                     s.thr);
   for (size_t li = 0; li < num_lists; li++) {
     const uint64_t n = lists[li].count;
     if (n == 0) continue;
-    const unsigned g = (unsigned)((n + 255) / 256);
+    const unsigned g = (unsigned)((n + kSchedBlock - 1) / kSchedBlock);
     uint32_t *cnt = s.cnt + li * 2 * kBuckets;
-    hipLaunchKernelGGL(k_sched_count, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
-                       s.pred, s.thr, s.bucket, cnt);
-    hipLaunchKernelGGL(k_sched_prefix, dim3(1), dim3(64), 0, st, cnt);
-    hipLaunchKernelGGL(k_sched_scatter, dim3(g), dim3(256), 0, st, plain + lists[li].begin, n,
-                       s.bucket, cnt + kBuckets, order + lists[li].begin);
+    hipLaunchKernelGGL(k_sched_count, dim3(g), dim3(kSchedBlock), 0, st, plain + lists[li].begin,
+                       n, s.pred, s.thr, s.bucket, s.blk);
+    hipLaunchKernelGGL(k_sched_offsets, dim3(1), dim3(1024), 0, st, s.blk, g, cnt);
+    hipLaunchKernelGGL(k_sched_scatter, dim3(g), dim3(kSchedBlock), 0, st,
+                       plain + lists[li].begin, n, s.bucket, s.blk, order + lists[li].begin);
   }
   return hipGetLastError();
 }

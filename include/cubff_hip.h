@@ -212,9 +212,11 @@ int cubff_batch_sched_stats(cubff_batch *b, uint64_t *trainings,
 
 /* ------------------------------------------------------------------------ */
 /* The MLP cost model of the learned scheduler on its own.  No reference
- * counterpart.  Input: 128-byte tapes; output: predicted log2(ops + 1).
+ * counterpart.  Input: 128-byte tapes (features: byte histogram / 128);
+ * output: a predicted log2 length (the scheduler trains it on the commands
+ * each interaction ran).
  * Parameters are one fp32 vector of cubff_mlp_num_params() values:
- * E[512][256], W1[256][256], b1[256], W2[256][256], b2[256], w3[256], b3. */
+ * E[256][256], W1[256][256], b1[256], W2[256][256], b2[256], w3[256], b3. */
 typedef struct cubff_mlp cubff_mlp;
 const char *cubff_mlp_last_error(void);
 uint64_t cubff_mlp_num_params(void);

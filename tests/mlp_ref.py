@@ -7,7 +7,7 @@ function in float64 from the bf16-rounded inputs and weights.
 """
 import numpy as np
 
-IN, HID = 512, 256
+IN, HID = 256, 256
 OFF_E = 0
 OFF_W1 = OFF_E + IN * HID
 OFF_B1 = OFF_W1 + HID * HID
@@ -27,15 +27,11 @@ def bf16(x):
 
 
 def features(tapes):
-    """[byte histogram | hashed bigram histogram] / 128, shape (n, 512)."""
+    """Byte histogram / 128, shape (n, 256)."""
     t = np.asarray(tapes, dtype=np.int64).reshape(-1, 128)
     n = t.shape[0]
     x = np.zeros((n, IN), dtype=np.float64)
-    rows = np.repeat(np.arange(n), 128)
-    np.add.at(x, (rows, t.reshape(-1)), 1.0)
-    a, b = t[:, :-1], t[:, 1:]
-    code = (a * 167 + b * 13 + ((a * b) >> 3)) & 255
-    np.add.at(x, (np.repeat(np.arange(n), 127), 256 + code.reshape(-1)), 1.0)
+    np.add.at(x, (np.repeat(np.arange(n), 128), t.reshape(-1)), 1.0)
     return (x / 128).astype(np.float32)
 
 
