@@ -216,15 +216,18 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
   }
 }
 
-// Per-soup interval counters (common_language.h:192, :494): one block per
-// soup sums its slots' ops and steps; num_runs += num_indices; then the next
-// epoch.
+// Per-soup interval counters (common_language.h:192, :494): kEndSplit
+// blocks per soup each sum a share of its slots' ops and steps and add them
+// atomically; the first also adds num_runs += num_indices, and block 0
+// advances the epoch.
+constexpr int kEndSplit = 16;
 __global__ __launch_bounds__(256) void k_b_end_epoch(const BatchDev d) {
   __shared__ unsigned long long part[2][4];
-  const uint32_t s = blockIdx.x;
-  const uint64_t lo = d.slot_base[s], hi = d.slot_base[s + 1];
+  const uint32_t s = blockIdx.x / kEndSplit, piece = blockIdx.x % kEndSplit;
+  const uint64_t lo = d.slot_base[s], n = d.slot_base[s + 1] - lo;
+  const uint64_t b0 = lo + n * piece / kEndSplit, b1 = lo + n * (piece + 1) / kEndSplit;
   unsigned long long ops = 0, steps = 0;
-  for (uint64_t g = lo + threadIdx.x; g < hi; g += blockDim.x) {
+  for (uint64_t g = b0 + threadIdx.x; g < b1; g += blockDim.x) {
     ops += d.pair_ops[g];
     steps += d.pair_steps[g];
   }
@@ -238,10 +241,12 @@ __global__ __launch_bounds__(256) void k_b_end_epoch(const BatchDev d) {
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long *c = d.counters + kNumBatchCounters * (uint64_t)s;
-    c[0] += part[0][0] + part[0][1] + part[0][2] + part[0][3];
-    c[1] += d.num_indices[s];
-    c[2] += part[1][0] + part[1][1] + part[1][2] + part[1][3];
-    if (s == 0) *d.epoch += 1;
+    const unsigned long long o = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+    const unsigned long long t = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+    if (o) atomicAdd(&c[0], o);
+    if (t) atomicAdd(&c[2], t);
+    if (piece == 0) atomicAdd(&c[1], (unsigned long long)d.num_indices[s]);
+    if (blockIdx.x == 0) *d.epoch += 1;
   }
 }
 
@@ -276,7 +281,7 @@ hipError_t batch_stage(hipStream_t st, const BatchDev &d) {
 }
 
 hipError_t batch_end_epoch(hipStream_t st, const BatchDev &d) {
-  hipLaunchKernelGGL(k_b_end_epoch, dim3(d.num_soups), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_b_end_epoch, dim3(d.num_soups * kEndSplit), dim3(256), 0, st, d);
   return hipGetLastError();
 }
 

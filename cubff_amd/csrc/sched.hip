@@ -70,9 +70,9 @@ __device__ __forceinline__ uint32_t bucket_of(float y, const uint32_t *__restric
 }
 
 // Bucketing without contended atomics: each block of 256 slots counts its
-// slots per bucket (blk[block][bucket]), one block turns the counts into
-// output offsets (bucket-major, blocks in order), and each block scatters its
-// slots from its offsets.
+// slots per bucket (blk[block][bucket]), one block per bucket turns the
+// counts into offsets within the bucket, and each block scatters its slots
+// to (earlier buckets' totals + its offset + rank).
 constexpr int kSchedBlock = 256;
 
 __global__ __launch_bounds__(kSchedBlock) void k_sched_count(
@@ -92,43 +92,39 @@ __global__ __launch_bounds__(kSchedBlock) void k_sched_count(
   if (threadIdx.x < kBuckets) blk[(uint64_t)blockIdx.x * kBuckets + threadIdx.x] = local[threadIdx.x];
 }
 
-// blk[block][b] <- first output position of that block's bucket-b slots;
-// tot[b] <- slots in bucket b.  One block of 1024 threads.
+// blk[block][b] <- position of that block's first bucket-b slot within
+// bucket b; tot[b] <- slots in bucket b.  One block of 1024 threads per
+// bucket.
 __global__ __launch_bounds__(1024) void k_sched_offsets(uint32_t *__restrict__ blk,
                                                         uint32_t nblocks,
                                                         uint32_t *__restrict__ tot) {
   __shared__ uint32_t part[1024];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, b = blockIdx.x;
   const uint32_t per = (nblocks + 1023) / 1024;
   const uint32_t lo = min(nblocks, tid * per), hi = min(nblocks, lo + per);
-  uint32_t base = 0;
-  for (int b = 0; b < kBuckets; b++) {
-    uint32_t c = 0;
-    for (uint32_t k = lo; k < hi; k++) c += blk[(uint64_t)k * kBuckets + b];
-    part[tid] = c;
+  uint32_t c = 0;
+  for (uint32_t k = lo; k < hi; k++) c += blk[(uint64_t)k * kBuckets + b];
+  part[tid] = c;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = tid >= o ? part[tid - o] : 0u;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      const uint32_t v = tid >= o ? part[tid - o] : 0u;
-      __syncthreads();
-      part[tid] += v;
-      __syncthreads();
-    }
-    uint32_t run = base + part[tid] - c;
-    for (uint32_t k = lo; k < hi; k++) {
-      const uint32_t n = blk[(uint64_t)k * kBuckets + b];
-      blk[(uint64_t)k * kBuckets + b] = run;
-      run += n;
-    }
-    const uint32_t total = part[1023];
+    part[tid] += v;
     __syncthreads();
-    if (tid == 0) tot[b] = total;
-    base += total;
   }
+  uint32_t run = part[tid] - c;
+  for (uint32_t k = lo; k < hi; k++) {
+    const uint32_t n = blk[(uint64_t)k * kBuckets + b];
+    blk[(uint64_t)k * kBuckets + b] = run;
+    run += n;
+  }
+  if (tid == 1023) tot[b] = part[1023];
 }
 
 __global__ __launch_bounds__(kSchedBlock) void k_sched_scatter(
     const uint32_t *__restrict__ list, uint64_t count, const uint32_t *__restrict__ bucket,
-    const uint32_t *__restrict__ blk, uint32_t *__restrict__ order) {
+    const uint32_t *__restrict__ blk, const uint32_t *__restrict__ tot,
+    uint32_t *__restrict__ order) {
   __shared__ uint32_t local[kBuckets];
   if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
   __syncthreads();
@@ -137,7 +133,9 @@ __global__ __launch_bounds__(kSchedBlock) void k_sched_scatter(
     const uint32_t g = list[i];
     const uint32_t b = bucket[g];
     const uint32_t rank = atomicAdd(&local[b], 1u);
-    order[blk[(uint64_t)blockIdx.x * kBuckets + b] + rank] = g;
+    uint32_t base = 0;  // slots in the buckets before b
+    for (uint32_t j = 0; j < b; j++) base += tot[j];
+    order[base + blk[(uint64_t)blockIdx.x * kBuckets + b] + rank] = g;
   }
 }
 
@@ -205,9 +203,10 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const u
     uint32_t *cnt = s.cnt + li * 2 * kBuckets;
     hipLaunchKernelGGL(k_sched_count, dim3(g), dim3(kSchedBlock), 0, st, plain + lists[li].begin,
                        n, s.pred, s.thr, s.bucket, s.blk);
-    hipLaunchKernelGGL(k_sched_offsets, dim3(1), dim3(1024), 0, st, s.blk, g, cnt);
+    hipLaunchKernelGGL(k_sched_offsets, dim3(kBuckets), dim3(1024), 0, st, s.blk, g, cnt);
     hipLaunchKernelGGL(k_sched_scatter, dim3(g), dim3(kSchedBlock), 0, st,
-                       plain + lists[li].begin, n, s.bucket, s.blk, order + lists[li].begin);
+                       plain + lists[li].begin, n, s.bucket, s.blk, cnt,
+                       order + lists[li].begin);
   }
   return hipGetLastError();
 }
