@@ -111,7 +111,7 @@ def load():
         "cubff_batch_write_soup": (_i, [_vp, ctypes.c_uint32, _u8p]),
         "cubff_batch_read_counters": (_i, [_vp, ctypes.c_uint32, _u64p, _u64p, _u64p]),
         "cubff_batch_reset_counters": (_i, [_vp]),
-        "cubff_batch_read_pair_stats": (_i, [_vp, ctypes.c_uint32, _u32p, _u32p]),
+        "cubff_batch_read_pair_stats": (_i, [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p]),
         "cubff_batch_read_shuffle": (_i, [_vp, ctypes.c_uint32, _u32p]),
         "cubff_batch_sched_stats": (_i, [_vp, _u64p, ctypes.POINTER(ctypes.c_float), _u32p]),
         "cubff_mlp_last_error": (ctypes.c_char_p, []),
@@ -381,12 +381,15 @@ class Batch:
     def reset_counters(self):
         _bcheck(load().cubff_batch_reset_counters(self._h))
 
-    def read_pair_stats(self, i: int):
+    def read_pair_stats(self, i: int, with_work: bool = False):
+        """(ops, steps) of the last epoch's pairs; (ops, steps, work) with
+        with_work."""
         ops = np.zeros(self.n[i] // 2, dtype=np.uint32)
         st = np.zeros(self.n[i] // 2, dtype=np.uint32)
+        wk = np.zeros(self.n[i] // 2, dtype=np.uint32)
         _bcheck(load().cubff_batch_read_pair_stats(self._h, i, _ptr(ops, _u32p),
-                                                   _ptr(st, _u32p)))
-        return ops, st
+                                                   _ptr(st, _u32p), _ptr(wk, _u32p)))
+        return (ops, st, wk) if with_work else (ops, st)
 
     def read_shuffle(self, i: int) -> np.ndarray:
         s = np.zeros(self.n[i], dtype=np.uint32)

@@ -469,13 +469,14 @@ int cubff_batch_reset_counters(cubff_batch *b) {
 }
 
 int cubff_batch_read_pair_stats(cubff_batch *b, uint32_t soup, uint32_t *ops,
-                                uint32_t *steps) {
+                                uint32_t *steps, uint32_t *work) {
   int r;
   if ((r = check(b))) return r;
   if (soup >= b->num_soups) return fail(CUBFF_EINVAL, "bad soup index");
   const uint64_t s0 = b->slot_base[soup], ns = b->slot_base[soup + 1] - s0;
   if (ops) HIP_TRY(hipMemcpyAsync(ops, b->d.pair_ops + s0, ns * 4, hipMemcpyDeviceToHost, b->stream));
   if (steps) HIP_TRY(hipMemcpyAsync(steps, b->d.pair_steps + s0, ns * 4, hipMemcpyDeviceToHost, b->stream));
+  if (work) HIP_TRY(hipMemcpyAsync(work, b->d.pair_work + s0, ns * 4, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return CUBFF_OK;
 }

@@ -627,6 +627,22 @@ struct ForthLang {
       t.set(i, (uint8_t)splitmix64(base + (uint64_t)i));
   }
 
+  // Cycle check: the state of a run is (position, live stack [0, sp),
+  // tape); stack entries at or above sp are never read before they are
+  // written.  The anchor is the state right after a backward jump, with the
+  // top kSnap stack entries kept in a register.  If a later backward jump
+  // lands on the anchor's position with the same sp, no write since the
+  // anchor changed a tape byte, pushes since then stayed within the top
+  // kSnap slots of the anchor's stack and those slots hold the anchor's
+  // values again, the whole state equals the anchor's: the run repeats with
+  // that period and whole periods are skipped.  Anchors move to later jumps
+  // after windows of kCycleWin, 2 kCycleWin, ... steps (Brent), so every
+  // cycle of this kind shorter than the remaining budget is found.  Runs
+  // that modify and restore the tape, or rewrite deeper stack slots, are not
+  // skipped (still exact).
+  static constexpr uint32_t kCycleWin = 16;
+  static constexpr int kSnap = 4;
+
   // Evaluate (forth.inc.h:379-397) with EvaluateOne (forth.inc.h:251-373,
   // forthcopy.cu, forthtrivial.cu, forthtrivial_reset.cu:123-252).
   template <class T, class A>
@@ -636,8 +652,14 @@ struct ForthLang {
     constexpr int kDummy = kTape2;
     int sp = 0;
     int pos = 0;
-    uint32_t i = 0, nops = 0;
+    uint32_t i = 0, nops = 0, skipped = 0;
     bool overflow = false;
+    // Anchor: position (-1 none yet, -2 cycle already skipped), sp, step
+    // and no-op counts, window; since the anchor: tape changed, lowest
+    // stack slot written.
+    int apos = -1, asp = 0, minsp = 0;
+    uint32_t ai = 0, anops = 0, awin = kCycleWin, snap = 0;
+    bool tchg = false;
     while (i < stepcount) {
       i++;
       const uint32_t c = t.get(pos);
@@ -652,7 +674,9 @@ struct ForthLang {
       const uint32_t vl = (uint32_t)(kNibVal >> sh) & 15;
       const uint32_t fl = (uint32_t)(kNibFlag >> sh) & 15;
       const uint32_t raddr = ((rd & 1 ? y : x) & (rd & 2 ? 127u : 63u)) + (rd & 4 ? 64u : 0u);
+      const uint32_t waddr = ((wr & 2 ? y : x) & (wr & 4 ? 127u : 63u)) + (wr & 8 ? 64u : 0u);
       const uint32_t m = t.get(raddr);
+      const uint32_t wold = t.get(waddr);
       // Stack effect (constants push c & 63, jumps touch nothing).
       const int npop = (is_const || is_jmp) ? 0 : (int)(np & 3);
       const int npush = is_jmp ? 0 : (is_const ? 1 : (int)(np >> 2));
@@ -676,15 +700,16 @@ struct ForthLang {
       sp = sp1 + (int)ok0 + (int)ok1;
       // Tape write.
       const bool wr_on = (wr & 1) && !is_const && !is_jmp;
-      const uint32_t waddr = ((wr & 2 ? y : x) & (wr & 4 ? 127u : 63u)) + (wr & 8 ? 64u : 0u);
       const uint32_t wv = rd & 8 ? m : x;
       bool do_reset = false;
-      if constexpr (kReset) do_reset = wr_on && (fl & 4) && t.get(waddr) == 0x13;
+      if constexpr (kReset) do_reset = wr_on && (fl & 4) && wold == 0x13;
       if (do_reset) {
         reset(t, pos);
       } else {
         t.set(wr_on ? waddr : (uint32_t)kDummy, (uint8_t)wv);
       }
+      tchg |= do_reset || (wr_on && (wv & 0xff) != wold);
+      minsp = min(minsp, sp1);
       nops += (fl & 2) && !is_const && !is_jmp;
       // Next position.
       int delta = 1;
@@ -692,9 +717,32 @@ struct ForthLang {
       else if ((fl & 1) && !is_const && x != 0) delta = 2;
       pos += delta;
       if (pos >= kTape2 || pos < 0 || overflow) break;
+      if (is_jmp && (c & 64) && apos != -2) {
+        bool same = pos == apos && sp == asp && !tchg && asp - minsp <= kSnap;
+        for (int q = minsp; same && q < asp; q++)
+          same = stk.get(q) == ((snap >> (8 * (asp - 1 - q))) & 0xff);
+        if (same) {
+          const uint32_t len = i - ai;
+          const uint32_t reps = (stepcount - i) / len;
+          i += reps * len;
+          skipped += reps * len;
+          nops += reps * (nops - anops);
+          apos = -2;  // fewer than one period of steps is left
+        } else if (apos < 0 || i - ai >= awin) {
+          if (apos >= 0) awin *= 2;
+          apos = pos;
+          asp = sp;
+          ai = i;
+          anops = nops;
+          tchg = false;
+          minsp = sp;
+          snap = 0;
+          for (int q = 0; q < kSnap && q < sp; q++) snap |= stk.get(sp - 1 - q) << (8 * q);
+        }
+      }
     }
     *steps = i;
-    if (work) *work = i;
+    if (work) *work = i - skipped;
     return i - nops;
   }
 };

@@ -201,8 +201,13 @@ int cubff_batch_write_soup(cubff_batch *b, uint32_t soup, const uint8_t *src);
 int cubff_batch_read_counters(cubff_batch *b, uint32_t soup, uint64_t *ops,
                               uint64_t *num_runs, uint64_t *steps);
 int cubff_batch_reset_counters(cubff_batch *b);
+/* Per-interaction results of the last epoch of one soup, in pair order
+ * (each pointer nullable, n/2 values): ops and steps as RunSingleProgram
+ * reports them (common_language.h:232-242), and work = the commands the
+ * interpreter actually executed (skipped loop iterations and no-op runs not
+ * counted; the learned scheduler's training target). */
 int cubff_batch_read_pair_stats(cubff_batch *b, uint32_t soup, uint32_t *ops,
-                                uint32_t *steps);
+                                uint32_t *steps, uint32_t *work);
 int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s);
 /* Learned-scheduler statistics: trainings so far, the loss of the last
  * training round, and the last epoch's slot count per length bucket

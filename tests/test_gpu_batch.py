@@ -53,8 +53,9 @@ def test_batch_matches_oracle(capi, wg, tpl, graph):
                 cum[i] += tot
                 assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, e)), (i, e)
                 assert np.array_equal(b.read_soup(i), soup), (i, e)
-                ops, steps = b.read_pair_stats(i)
+                ops, steps, work = b.read_pair_stats(i, with_work=True)
                 assert np.array_equal(ops, po) and np.array_equal(steps, ps), (i, e)
+                assert (work <= steps).all(), (i, e)  # executed commands
                 assert b.read_counters(i)[:2] == (cum[i], (e + 1) * (n // 2)), (i, e)
 
 
