@@ -253,11 +253,17 @@ struct BffLang {
   // byte's class flips the affected bits.
   // Full-state cycle check (standard jump logic): from kCycleStart steps
   // on, the state right after a taken backward jump is compared with an
-  // anchor whose tape is kept in the aux region; anchors move to later jumps
-  // after windows of kCycleStart, 2 kCycleStart, ... steps (Brent), so a
-  // cycle shorter than the remaining budget is found.  A match (pc, heads,
-  // then every tape byte) means the run repeats with that period.
-  static constexpr uint32_t kCycleStart = 512;
+  // anchor whose tape is kept in the aux region.  The first anchor stays for
+  // kCycleFirstWin steps: the longest runs are mostly cycles that start
+  // within ~150 steps with periods of 1.5K-4K steps (a head sweeping the
+  // whole tape in a loop), and half the 8192-step budget is the longest
+  // period that can still be skipped.  Later anchors move after windows of
+  // kCycleWin, 2 kCycleWin, ... steps (Brent), so a cycle that starts late
+  // and is shorter than the remaining budget is found too.  A match (pc,
+  // heads, then every tape byte) means the run repeats with that period.
+  static constexpr uint32_t kCycleStart = 128;
+  static constexpr uint32_t kCycleFirstWin = 4096;
+  static constexpr uint32_t kCycleWin = 1024;
   __device__ __forceinline__ static uint32_t cycle_weight(int p) {
     return 2654435761u * (2u * (uint32_t)p + 1u);
   }
@@ -313,7 +319,8 @@ struct BffLang {
       // Cycle-check anchor: jump source (-1 none yet, -2 done), heads, step
       // and no-op counts, window, and a hash of the tape changes since.
       int cq = -1, ch0 = 0, ch1 = 0;
-      uint32_t ci = 0, cskip = 0, cwin = kCycleStart, cdelta = 0;
+      uint32_t ci = 0, cskip = 0, cwin = kCycleFirstWin, cdelta = 0;
+      bool cfirst = true;
       for (;;) {
         iters++;
         const uint32_t c = t.get(pc);
@@ -452,7 +459,10 @@ struct BffLang {
                     cq = -2;  // fewer than one period of steps is left
                     aq = -1;  // the loop anchor's counts predate the skip
                   } else if (cq < 0 || i1 - ci >= cwin) {
-                    if (cq >= 0) cwin *= 2;
+                    if (cq >= 0) {
+                      cwin = cfirst ? kCycleWin : min(cwin * 2, 1u << 30);
+                      cfirst = false;
+                    }
                     for (int w = 0; w < kTape2 / 4; w++) aux.set_word(w, t.word(w));
                     cq = pc;
                     ch0 = h0;
@@ -630,17 +640,20 @@ struct ForthLang {
   // Cycle check: the state of a run is (position, live stack [0, sp),
   // tape); stack entries at or above sp are never read before they are
   // written.  The anchor is the state right after a backward jump, with the
-  // top kSnap stack entries kept in a register.  If a later backward jump
-  // lands on the anchor's position with the same sp, no write since the
-  // anchor changed a tape byte, pushes since then stayed within the top
-  // kSnap slots of the anchor's stack and those slots hold the anchor's
-  // values again, the whole state equals the anchor's: the run repeats with
-  // that period and whole periods are skipped.  Anchors move to later jumps
-  // after windows of kCycleWin, 2 kCycleWin, ... steps (Brent), so every
-  // cycle of this kind shorter than the remaining budget is found.  Runs
-  // that modify and restore the tape, or rewrite deeper stack slots, are not
-  // skipped (still exact).
+  // top kSnap stack entries kept in a register.  Since the anchor, the first
+  // two tape bytes that writes changed are logged with their anchor values.
+  // If a later backward jump lands on the anchor's position with the same
+  // sp, no third tape byte changed and the logged ones hold their anchor
+  // values again, pushes stayed within the top kSnap slots of the anchor's
+  // stack and those slots hold the anchor's values again, the whole state
+  // equals the anchor's: the run repeats with that period and whole periods
+  // are skipped.  Anchors move to later jumps after windows of kCycleWin,
+  // 8 kCycleWin, 64 kCycleWin, ... steps (Brent's scheme with factor 8:
+  // cycles here start within a few dozen steps, so long windows early
+  // catch long periods sooner), so every cycle of this kind shorter than the
+  // remaining budget is found.  Other runs are not skipped (still exact).
   static constexpr uint32_t kCycleWin = 16;
+  static constexpr uint32_t kCycleGrow = 8;
   static constexpr int kSnap = 4;
 
   // Evaluate (forth.inc.h:379-397) with EvaluateOne (forth.inc.h:251-373,
@@ -660,6 +673,8 @@ struct ForthLang {
     int apos = -1, asp = 0, minsp = 0;
     uint32_t ai = 0, anops = 0, awin = kCycleWin, snap = 0;
     bool tchg = false;
+    int la0 = -1, la1 = -1;  // logged tape addresses (-1 free) and values
+    uint32_t lv0 = 0, lv1 = 0;
     while (i < stepcount) {
       i++;
       const uint32_t c = t.get(pos);
@@ -708,7 +723,18 @@ struct ForthLang {
       } else {
         t.set(wr_on ? waddr : (uint32_t)kDummy, (uint8_t)wv);
       }
-      tchg |= do_reset || (wr_on && (wv & 0xff) != wold);
+      if (wr_on && (wv & 0xff) != wold && (int)waddr != la0 && (int)waddr != la1) {
+        if (la0 < 0) {
+          la0 = (int)waddr;
+          lv0 = wold;
+        } else if (la1 < 0) {
+          la1 = (int)waddr;
+          lv1 = wold;
+        } else {
+          tchg = true;
+        }
+      }
+      tchg |= do_reset;
       minsp = min(minsp, sp1);
       nops += (fl & 2) && !is_const && !is_jmp;
       // Next position.
@@ -719,6 +745,8 @@ struct ForthLang {
       if (pos >= kTape2 || pos < 0 || overflow) break;
       if (is_jmp && (c & 64) && apos != -2) {
         bool same = pos == apos && sp == asp && !tchg && asp - minsp <= kSnap;
+        if (same && la0 >= 0) same = t.get(la0) == lv0;
+        if (same && la1 >= 0) same = t.get(la1) == lv1;
         for (int q = minsp; same && q < asp; q++)
           same = stk.get(q) == ((snap >> (8 * (asp - 1 - q))) & 0xff);
         if (same) {
@@ -729,12 +757,13 @@ struct ForthLang {
           nops += reps * (nops - anops);
           apos = -2;  // fewer than one period of steps is left
         } else if (apos < 0 || i - ai >= awin) {
-          if (apos >= 0) awin *= 2;
+          if (apos >= 0) awin = min(awin * kCycleGrow, 1u << 30);
           apos = pos;
           asp = sp;
           ai = i;
           anops = nops;
           tchg = false;
+          la0 = la1 = -1;
           minsp = sp;
           snap = 0;
           for (int q = 0; q < kSnap && q < sp; q++) snap |= stk.get(sp - 1 - q) << (8 * q);
