@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libcubff_hip.so")
 
 PAIR_SHUFFLE, PAIR_FIXED, PAIR_CYCLIC, PAIR_2D, PAIR_HOST = 0, 1, 2, 3, 4
+FLOW_OFF, FLOW_ON, FLOW_AUTO = 0, 1, 2  # cubff_set_pipeline modes
 
 # Every symbol include/cubff_hip.h declares.
 EXPORTS = [
@@ -22,7 +23,8 @@ EXPORTS = [
     "cubff_ctx_create", "cubff_ctx_destroy", "cubff_init_programs",
     "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
     "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
-    "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_read_counters", "cubff_reset_counters",
+    "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats",
+    "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
     "cubff_splitmix64", "cubff_epoch_seed",
@@ -83,6 +85,8 @@ def load():
         "cubff_run_epochs_timed": (_i, [_vp, _i, _u64, _u64, _u64, ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_float),
                                         ctypes.POINTER(ctypes.c_float)]),
+        "cubff_set_pipeline": (_i, [_vp, _i, ctypes.c_double]),
+        "cubff_flow_stats": (_i, [_vp, _u64p, _u64p]),
         "cubff_read_counters": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_reset_counters": (_i, [_vp]),
         "cubff_read_pair_ops": (_i, [_vp, _u32p, _u32p, _u64]),
@@ -242,6 +246,17 @@ class Soup:
                    count: int, mutation_prob: int):
         _check(load().cubff_run_epochs(self._h, mode, params_seed, first_epoch,
                                        count, mutation_prob))
+
+    def set_pipeline(self, mode, wait_limit_s: float = 10.0):
+        """Multi-epoch runs: FLOW_ON / True (dataflow kernel), FLOW_OFF /
+        False (one launch per epoch) or FLOW_AUTO (the default)."""
+        _check(load().cubff_set_pipeline(self._h, int(mode), float(wait_limit_s)))
+
+    def flow_stats(self):
+        """(interactions, claims) of the dataflow kernel since creation."""
+        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load().cubff_flow_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return int(a.value), int(b.value)
 
     def run_epochs_timed(self, mode: int, params_seed: int, first_epoch: int,
                          count: int, mutation_prob: int):

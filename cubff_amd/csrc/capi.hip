@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -28,6 +29,15 @@ struct cubff_ctx {
   G2Scratch g2{};
   uint64_t *row_off = nullptr;
   uint32_t *nbr = nullptr;
+  // Multi-epoch dataflow path (flow.hip): CUBFF_FLOW_* mode; scratch for
+  // flow_cap epochs; a single wait may last flow_wait_s seconds.
+  int pipeline = CUBFF_FLOW_AUTO;
+  double flow_wait_s = 10.0;
+  uint64_t flow_cap = 0;
+  uint32_t *flow_shuf = nullptr, *flow_pairof = nullptr, *flow_cnt = nullptr,
+           *flow_queue = nullptr;
+  FlowCtl *flow_ctl = nullptr;
+  uint64_t flow_claims = 0, flow_items = 0;  // since creation
 };
 
 namespace {
@@ -85,6 +95,185 @@ int ensure_2d_scratch(cubff_ctx *c) {
       (r = dev_alloc(&c->g2.rest_total, 1)))
     return r;
   return CUBFF_OK;
+}
+
+// Multi-epoch segments: at most kFlowMaxEpochs epochs and kFlowBytes of
+// scratch (pairings, pair-of-program maps, pending counts, queue: 12 bytes
+// per program and epoch).
+constexpr uint64_t kFlowMaxEpochs = 64;
+constexpr uint64_t kFlowBytes = 1ULL << 30;
+constexpr double kFlowPatienceS = 2e-6;
+
+// Where the dataflow kernel beats one launch per epoch (measured on MI355X
+// at 2^17 programs, 128 epochs): languages whose longest interactions run
+// far longer than a typical one, up to 2^18 programs.  At 2^22 programs the
+// epochs are throughput-bound and the per-launch loop is faster.
+bool flow_pays(int lang, uint64_t n) {
+  if (n > (1ULL << 18)) return false;
+  switch (lang) {
+    case kBff8: case kBff8Noheads: case kBffNoheads4bit: case kBffSelfmove:
+    case kForth: case kForthcopy: case kForthtrivial: case kForthtrivialReset:
+    case kRsubleq4:
+      return true;
+    default:
+      return false;
+  }
+}
+
+uint64_t flow_segment_epochs(uint64_t n) {
+  return std::min(kFlowBytes / (12 * n), kFlowMaxEpochs);
+}
+
+int ensure_flow_scratch(cubff_ctx *c, uint64_t epochs) {
+  if (c->flow_cap >= epochs) return CUBFF_OK;
+  dev_free(c->flow_shuf);
+  dev_free(c->flow_pairof);
+  dev_free(c->flow_cnt);
+  dev_free(c->flow_queue);
+  c->flow_cap = 0;
+  const uint64_t n = c->n;
+  int r;
+  if ((r = dev_alloc(&c->flow_shuf, epochs * n)) || (r = dev_alloc(&c->flow_pairof, epochs * n)) ||
+      (r = dev_alloc(&c->flow_cnt, epochs * (n / 2))) ||
+      (r = dev_alloc(&c->flow_queue, epochs * (n / 2))))
+    return r;
+  if (!c->flow_ctl && (r = dev_alloc(&c->flow_ctl, 1))) return r;
+  c->flow_cap = epochs;
+  return CUBFF_OK;
+}
+
+int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint32_t *dst) {
+  switch (mode) {
+    case CUBFF_PAIR_SHUFFLE:
+      HIP_TRY(launch_pair_shuffle(c->stream, c->fy, dst, c->n, params_seed, epoch));
+      return CUBFF_OK;
+    case CUBFF_PAIR_FIXED:
+      HIP_TRY(launch_pair_fixed(c->stream, dst, c->n, epoch));
+      return CUBFF_OK;
+    case CUBFF_PAIR_CYCLIC:
+      HIP_TRY(launch_pair_cyclic(c->stream, dst, c->n, epoch));
+      return CUBFF_OK;
+    default:
+      return fail(CUBFF_EINVAL, "pairing mode without a closed form or shuffle");
+  }
+}
+
+// Epochs [first, first + epochs) as one dataflow launch; leaves the last
+// epoch's pairing in c->shuf and its per-pair results in pair_ops/steps.
+// With ev, records ev[0]/ev[1] around the dataflow kernel.  Blocks until
+// done to check the kernel's abort flag.
+int run_flow_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first,
+                     uint32_t epochs, uint32_t prob, hipEvent_t *ev) {
+  int r;
+  if ((r = ensure_flow_scratch(c, epochs))) return r;
+  const uint64_t n = c->n;
+  for (uint32_t j = 0; j < epochs; j++)
+    if ((r = pair_into(c, mode, params_seed, first + j, c->flow_shuf + (uint64_t)j * n))) return r;
+  HIP_TRY(launch_flow_pairof(c->stream, c->flow_shuf, c->flow_pairof, n, epochs));
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  FlowArgs a{};
+  a.soup = c->soup;
+  a.shuf = c->flow_shuf;
+  a.pairof = c->flow_pairof;
+  a.cnt = c->flow_cnt;
+  a.queue = c->flow_queue;
+  a.ctl = c->flow_ctl;
+  a.n = n;
+  a.slots = (uint32_t)(n / 2);
+  a.epochs = epochs;
+  a.sm_seed = splitmix64(params_seed);
+  a.first_epoch = first;
+  a.prob = prob;
+  a.pair_ops = c->pair_ops;
+  a.pair_steps = c->pair_steps;
+  a.counters = c->counters;
+  a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
+  a.patience_ticks = (uint64_t)(kFlowPatienceS * khz * 1000.0);
+  if (ev) HIP_TRY(hipEventRecord(ev[0], c->stream));
+  HIP_TRY(launch_flow(c->stream, c->lang, a));
+  if (ev) HIP_TRY(hipEventRecord(ev[1], c->stream));
+  HIP_TRY(hipMemcpyAsync(c->shuf, c->flow_shuf + (uint64_t)(epochs - 1) * n, n * 4,
+                         hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
+  FlowCtl ctl{};
+  HIP_TRY(hipMemcpyAsync(&ctl, c->flow_ctl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->flow_claims += ctl.claims;
+  c->flow_items += (uint64_t)epochs * (n / 2);
+  if (ctl.abort)
+    return fail(CUBFF_EHIP, "multi-epoch kernel stopped: a wait exceeded " +
+                                std::to_string(c->flow_wait_s) + " s; soup state is undefined");
+  return CUBFF_OK;
+}
+
+// The body of cubff_run_epochs / cubff_run_epochs_timed.  Closed-form and
+// shuffle pairings of two or more epochs go through the dataflow kernel in
+// segments; otherwise one pairing + epoch launch per epoch.  timing (2
+// floats, nullable) receives the total and the epoch-kernel milliseconds.
+int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first_epoch,
+                    uint64_t count, uint32_t prob, float *timing) {
+  const uint64_t seg = flow_segment_epochs(c->n);
+  const bool flow = (c->pipeline == CUBFF_FLOW_ON ||
+                     (c->pipeline == CUBFF_FLOW_AUTO && flow_pays(c->lang, c->n))) &&
+                    count >= 2 && seg >= 2 &&
+                    (mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
+                     mode == CUBFF_PAIR_CYCLIC);
+  const uint64_t sm = splitmix64(params_seed);
+  std::vector<hipEvent_t> ev;
+  auto destroy = [&]() {
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  };
+  int r = CUBFF_OK;
+  if (timing) {
+    const uint64_t launches = flow ? (count + seg - 1) / seg : count;
+    ev.resize(2 + 2 * launches);
+    for (auto &e : ev) {
+      hipError_t he = hipEventCreate(&e);
+      if (he != hipSuccess) {
+        destroy();
+        return fail(CUBFF_EHIP, std::string("hipEventCreate failed: ") + hipGetErrorString(he));
+      }
+    }
+    hipError_t he = hipEventRecord(ev[0], c->stream);
+    if (he != hipSuccess) r = fail(CUBFF_EHIP, hipGetErrorString(he));
+  }
+  uint64_t launch = 0;
+  for (uint64_t e = first_epoch; r == CUBFF_OK && e < first_epoch + count; launch++) {
+    hipEvent_t *kev = timing ? &ev[2 + 2 * launch] : nullptr;
+    const uint64_t left = first_epoch + count - e;
+    if (flow && left >= 2) {
+      const uint32_t ep = (uint32_t)std::min(left, seg);
+      r = run_flow_segment(c, mode, params_seed, e, ep, prob, kev);
+      e += ep;
+      continue;
+    }
+    if ((r = cubff_pair(c, mode, params_seed, e))) break;
+    hipError_t he = hipSuccess;
+    if (kev) he = hipEventRecord(kev[0], c->stream);
+    if (he == hipSuccess)
+      he = launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
+                        epoch_seed_pre(sm, e), prob, c->pair_ops, c->pair_steps, c->counters);
+    if (he == hipSuccess && kev) he = hipEventRecord(kev[1], c->stream);
+    if (he != hipSuccess) r = fail(CUBFF_EHIP, std::string("epoch launch failed: ") + hipGetErrorString(he));
+    e++;
+  }
+  if (r == CUBFF_OK && timing) {
+    float tot = 0, kern = 0;
+    hipError_t he = hipEventRecord(ev[1], c->stream);
+    if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
+    if (he == hipSuccess) he = hipEventElapsedTime(&tot, ev[0], ev[1]);
+    for (uint64_t i = 0; he == hipSuccess && i < launch; i++) {
+      float ms = 0;
+      he = hipEventElapsedTime(&ms, ev[2 + 2 * i], ev[3 + 2 * i]);
+      kern += ms;
+    }
+    if (he != hipSuccess) r = fail(CUBFF_EHIP, std::string("event timing failed: ") + hipGetErrorString(he));
+    timing[0] = tot;
+    timing[1] = kern;
+  }
+  destroy();
+  return r;
 }
 
 }  // namespace
@@ -196,6 +385,11 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   dev_free(c->g2.rest_total);
   dev_free(c->row_off);
   dev_free(c->nbr);
+  dev_free(c->flow_shuf);
+  dev_free(c->flow_pairof);
+  dev_free(c->flow_cnt);
+  dev_free(c->flow_queue);
+  dev_free(c->flow_ctl);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUBFF_OK;
@@ -312,14 +506,7 @@ int cubff_run_epochs(cubff_ctx *c, int mode, uint64_t params_seed,
   int r;
   if ((r = check_ctx(c))) return r;
   if (mode == CUBFF_PAIR_HOST) return fail(CUBFF_EINVAL, "run_epochs needs a device pairing mode");
-  const uint64_t sm = splitmix64(params_seed);
-  for (uint64_t e = first_epoch; e < first_epoch + count; e++) {
-    if ((r = cubff_pair(c, mode, params_seed, e))) return r;
-    HIP_TRY(launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
-                         epoch_seed_pre(sm, e), prob, c->pair_ops, c->pair_steps,
-                         c->counters));
-  }
-  return CUBFF_OK;
+  return run_epochs_impl(c, mode, params_seed, first_epoch, count, prob, nullptr);
 }
 
 int cubff_run_epochs_timed(cubff_ctx *c, int mode, uint64_t params_seed,
@@ -329,36 +516,29 @@ int cubff_run_epochs_timed(cubff_ctx *c, int mode, uint64_t params_seed,
   if ((r = check_ctx(c))) return r;
   if (mode == CUBFF_PAIR_HOST) return fail(CUBFF_EINVAL, "run_epochs needs a device pairing mode");
   if (mode == CUBFF_PAIR_2D) return fail(CUBFF_EINVAL, "timed runs support the closed-form and shuffle pairings");
-  const uint64_t sm = splitmix64(params_seed);
-  hipEvent_t ev0, ev1;
-  HIP_TRY(hipEventCreate(&ev0));
-  HIP_TRY(hipEventCreate(&ev1));
-  std::vector<hipEvent_t> ks(2 * count);
-  for (auto &e : ks) HIP_TRY(hipEventCreate(&e));
-  HIP_TRY(hipEventRecord(ev0, c->stream));
-  for (uint64_t i = 0; i < count; i++) {
-    const uint64_t e = first_epoch + i;
-    if ((r = cubff_pair(c, mode, params_seed, e))) return r;
-    HIP_TRY(hipEventRecord(ks[2 * i], c->stream));
-    HIP_TRY(launch_epoch(c->stream, c->lang, c->soup, c->shuf, c->n, c->num_indices,
-                         epoch_seed_pre(sm, e), prob, c->pair_ops, c->pair_steps,
-                         c->counters));
-    HIP_TRY(hipEventRecord(ks[2 * i + 1], c->stream));
-  }
-  HIP_TRY(hipEventRecord(ev1, c->stream));
-  HIP_TRY(hipEventSynchronize(ev1));
-  float tot = 0, kern = 0;
-  HIP_TRY(hipEventElapsedTime(&tot, ev0, ev1));
-  for (uint64_t i = 0; i < count; i++) {
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, ks[2 * i], ks[2 * i + 1]));
-    kern += ms;
-  }
-  for (auto &e : ks) (void)hipEventDestroy(e);
-  (void)hipEventDestroy(ev0);
-  (void)hipEventDestroy(ev1);
-  if (total_ms) *total_ms = tot;
-  if (epoch_kernel_ms) *epoch_kernel_ms = kern;
+  float t[2] = {0, 0};
+  if ((r = run_epochs_impl(c, mode, params_seed, first_epoch, count, prob, t))) return r;
+  if (total_ms) *total_ms = t[0];
+  if (epoch_kernel_ms) *epoch_kernel_ms = t[1];
+  return CUBFF_OK;
+}
+
+int cubff_flow_stats(cubff_ctx *c, uint64_t *items, uint64_t *claims) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (items) *items = c->flow_items;
+  if (claims) *claims = c->flow_claims;
+  return CUBFF_OK;
+}
+
+int cubff_set_pipeline(cubff_ctx *c, int enable, double wait_limit_s) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (!(wait_limit_s >= 0)) return fail(CUBFF_EINVAL, "wait limit must be >= 0 seconds");
+  if (enable != CUBFF_FLOW_OFF && enable != CUBFF_FLOW_ON && enable != CUBFF_FLOW_AUTO)
+    return fail(CUBFF_EINVAL, "pipeline mode must be CUBFF_FLOW_OFF, _ON or _AUTO");
+  c->pipeline = enable;
+  c->flow_wait_s = wait_limit_s;
   return CUBFF_OK;
 }
 

@@ -25,6 +25,35 @@ struct G2Scratch {
   uint64_t *rest_total;
 };
 
+// Control block of the multi-epoch dataflow kernel: queue head (next entry
+// to claim) and tail (next free entry), adjacent and 8-byte aligned so they
+// can be read together; the abort flag; claims made; items done.
+struct alignas(16) FlowCtl {
+  uint32_t head, tail, abort, claims, done, pad[3];
+};
+
+// Arguments of the multi-epoch dataflow kernel (k_flow in flow.hip) for a
+// segment of `epochs` epochs starting at first_epoch: shuf and pairof hold
+// epochs x n entries (pairing of segment epoch j, and the pair of each
+// program in it), cnt and queue epochs x n/2.
+struct FlowArgs {
+  uint8_t *soup;
+  const uint32_t *shuf;
+  const uint32_t *pairof;
+  uint32_t *cnt;
+  uint32_t *queue;
+  FlowCtl *ctl;
+  uint64_t n;
+  uint32_t slots, epochs;
+  uint64_t sm_seed, first_epoch;
+  uint32_t prob;
+  uint32_t *pair_ops, *pair_steps;  // written for the segment's last epoch
+  unsigned long long *counters;
+  uint64_t spin_ticks;      // wall-clock ticks any single wait may take
+  uint64_t patience_ticks;  // wait for a full run of 64 items this long
+  uint32_t fixed;           // epoch-0 items handed to the waves directly (set at launch)
+};
+
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
                        uint64_t seed_value, uint64_t count, int zero_init);
 hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
@@ -52,5 +81,9 @@ hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,
                           const uint32_t *nbr, uint32_t *s, uint64_t n,
                           uint64_t params_seed, uint64_t epoch,
                           uint64_t *num_indices);
+
+hipError_t launch_flow_pairof(hipStream_t st, const uint32_t *shuf, uint32_t *pairof,
+                              uint64_t n, uint32_t epochs);
+hipError_t launch_flow(hipStream_t st, int lang, const FlowArgs &a);
 
 }  // namespace cubff

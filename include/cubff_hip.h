@@ -103,6 +103,25 @@ int cubff_run_epochs(cubff_ctx *ctx, int mode, uint64_t params_seed,
  * duration of the epoch (mutate + evaluate) launches alone.  Blocks until
  * done.  No reference counterpart: the reference times wall-clock intervals
  * between callbacks (common_language.h:497, :508-517). */
+/* Multi-epoch runs (cubff_run_epochs / _timed with the shuffle, fixed or
+ * cyclic pairing and count >= 2) can go through a dataflow kernel: the
+ * pairings of up to 64 epochs are drawn first, then one launch runs every
+ * interaction as soon as the interactions of the previous epoch that wrote
+ * its two programs are done.  Results are bit-identical to the
+ * epoch-by-epoch loop.  mode CUBFF_FLOW_OFF selects that loop, _ON the
+ * dataflow kernel, _AUTO (the default) the dataflow kernel for the
+ * languages and sizes where it measured faster.  wait_limit_s bounds any
+ * single wait inside the kernel (default 10 s), past which the run stops
+ * with CUBFF_EHIP and an undefined soup.  No reference counterpart. */
+#define CUBFF_FLOW_OFF 0
+#define CUBFF_FLOW_ON 1
+#define CUBFF_FLOW_AUTO 2
+int cubff_set_pipeline(cubff_ctx *ctx, int mode, double wait_limit_s);
+/* Interactions run by the dataflow kernel since the context was created,
+ * and the runs of queued interactions (up to 64, one wavefront) claimed to
+ * run them. */
+int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims);
+
 int cubff_run_epochs_timed(cubff_ctx *ctx, int mode, uint64_t params_seed,
                            uint64_t first_epoch, uint64_t count,
                            uint32_t mutation_prob, float *total_ms,

@@ -1,0 +1,108 @@
+"""GPU parity of the multi-epoch dataflow kernel (cubff_run_epochs with
+count >= 2): every interaction starts once the interactions of the previous
+epoch that wrote its programs are done, and the result must equal the
+epoch-by-epoch loop bit for bit -- soups, the last epoch's pairing and
+per-pair results, and the interval counters -- for every language, pairing
+mode, soup size and segment split.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+# The dataflow kernel is forced on (FLOW_ON) in every check below; the
+# default (FLOW_AUTO) only picks it where it measured faster.
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from cubff_amd import capi
+    capi.load()
+    return capi
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def _pairing(capi, mode, n, seed, e):
+    if mode == capi.PAIR_SHUFFLE:
+        return O.pairing_shuffle(n, seed, e)
+    if mode == capi.PAIR_FIXED:
+        return O.pairing_fixed(n, e)
+    return O.pairing_cyclic(n, e)
+
+
+def _check_run(capi, lang, n, seed, P, mode, first, count, nthreads=0):
+    """count epochs from `first` in one call against the oracle's loop."""
+    print("flow:", lang, n, seed, mode, first, count, flush=True)
+    ref = O.Soup(lang, n, seed, P, nthreads=nthreads)
+    with capi.Soup(lang, n) as g:
+        g.set_pipeline(capi.FLOW_ON)
+        g.init_programs(O.epoch_seed(seed, 0))
+        for e in range(first):  # oracle catches up; the GPU gets the soup
+            ref.step(_pairing(capi, mode, n, seed, e))
+        g.write_soup(ref.soup)
+        g.reset_counters()
+        tot = 0
+        for e in range(first, first + count):
+            t, po, ps = ref.step(_pairing(capi, mode, n, seed, e))
+            tot += t
+        g.run_epochs(mode, seed, first, count, P)
+        assert _sha(g.read_soup()) == _sha(ref.soup), (lang, n, mode, count)
+        s, k = g.read_shuffle()
+        assert k == n // 2 and np.array_equal(s, _pairing(capi, mode, n, seed, first + count - 1))
+        ops, steps = g.read_pair_ops()
+        assert np.array_equal(ops, po) and np.array_equal(steps, ps)
+        assert g.read_counters() == (tot, count * (n // 2))
+
+
+@pytest.mark.parametrize("lang", O.LANGS)
+def test_flow_every_language(capi, lang):
+    _check_run(capi, lang, 2048, 10248, 1 << 18, capi.PAIR_SHUFFLE, 0, 12)
+
+
+@pytest.mark.parametrize("mode", ["shuffle", "fixed", "cyclic"])
+def test_flow_segments_and_modes(capi, mode):
+    # 150 epochs = segments of 64, 64 and 22 epochs.
+    m = {"shuffle": capi.PAIR_SHUFFLE, "fixed": capi.PAIR_FIXED, "cyclic": capi.PAIR_CYCLIC}[mode]
+    _check_run(capi, "bff_noheads", 256, 7, 1 << 20, m, 5, 150)
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 130])
+def test_flow_tiny_soups(capi, n):
+    # One or a few pairs per epoch: every item waits on the one before.
+    _check_run(capi, "bff_noheads", n, 3, 1 << 22, capi.PAIR_SHUFFLE, 0, 70)
+    _check_run(capi, "forth", n, 4, 1 << 22, capi.PAIR_SHUFFLE, 2, 9)
+
+
+def test_flow_equals_epoch_loop_at_2p17(capi):
+    n, seed, P = 131072, 1, 1 << 18
+    print("flow vs loop at", n, flush=True)
+    out = []
+    for pipeline in (capi.FLOW_ON, capi.FLOW_OFF):
+        with capi.Soup("bff_noheads", n) as g:
+            g.set_pipeline(pipeline)
+            g.init_programs(O.epoch_seed(seed, 0))
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 40, P)
+            out.append((_sha(g.read_soup()), g.read_counters(), _sha(g.read_pair_ops()[0])))
+    assert out[0] == out[1]
+    _check_run(capi, "bff_noheads", n, seed, P, capi.PAIR_SHUFFLE, 0, 3, nthreads=16)
+
+
+def test_flow_wait_limit_stops_cleanly(capi):
+    # A zero wait limit: the first wave that finds no queued item raises the
+    # abort flag, every wave leaves, and the host reports the error.  The
+    # context stays usable.
+    n, seed, P = 4096, 11, 1 << 18
+    with capi.Soup("bff_noheads", n) as g:
+        g.init_programs(O.epoch_seed(seed, 0))
+        g.set_pipeline(capi.FLOW_ON, 0.0)
+        with pytest.raises(RuntimeError, match="wait exceeded"):
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 8, P)
+        g.set_pipeline(capi.FLOW_ON, 10.0)
+    _check_run(capi, "bff_noheads", n, seed, P, capi.PAIR_SHUFFLE, 0, 8)
