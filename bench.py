@@ -8,14 +8,20 @@ epoch: device pairing + mutate + evaluate + write back over the whole soup.
 
 Prints one JSON line.  `value` is interpreted instructions per second (the
 reference's ops: executed non-no-op instructions) over the timed epochs, with
-the soup resident in HBM.  The roofline object prices the epoch kernel against
-the integer-VALU peak with the frozen work model
+the soup resident in HBM.  The roofline object prices the epoch kernel
+(k_epoch, one launch per epoch) with the frozen integer work model
 
     W_epoch = (N/2) * 128 * C_mut + sum(steps) * C_step,
     C_mut = 32 lane-ops per mutation draw, C_step = 8 lane-ops per BFF step,
 
-where steps counts every interpreter loop iteration including no-ops, and
-peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 int32 lane-ops/s.
+where steps counts every interpreter loop iteration including no-ops, over
+the kernel's average launch time from HIP events on the context stream,
+against 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 int32 lane-ops/s.  `traffic`
+is the measured HBM bytes per launch (committed rocprofv3 PMC summary) and
+`hbm` the algorithmic bytes per launch (per pair slot: 128 gathered, 128
+scattered, 8 of pairing, 8 of per-slot results) over the launch time,
+against 8 TB/s, as evidence.  The kernel is bound by neither: an epoch lasts
+as long as its longest interaction.
 
 The CPU baseline runs the reference build from oracle/_ref (compiled from the
 reference sources) on a bounded sample of the same workload on the host's
@@ -35,6 +41,8 @@ sys.path.insert(0, ROOT)
 METRIC = ("interpreted instr/s + epochs/s, 2^17×64B bff_noheads; "
           "batched+sched vs plain")
 PEAK_LANE_OPS = 256 * 64 * 2.4e9
+PEAK_HBM = 8e12          # bytes/s, MI355X HBM3E
+BYTES_PER_SLOT = 272     # k_epoch: 2 x 64 B gathered + 2 x 64 B scattered + s[2k..2k+1] + ops, steps
 C_MUT, C_STEP = 32, {"bff": 8, "forth": 10, "subleq": 10, "rsubleq4": 12}
 
 
@@ -145,10 +153,13 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
         return out
 
 
-def run_soup_timed(capi, lang, n, seed, prob, warmup, epochs, mode=None, pattern=None):
+def run_soup_timed(capi, lang, n, seed, prob, warmup, epochs, mode=None, pattern=None,
+                   flow=None):
     """Single-soup epochs timed with HIP events; returns rates."""
     mode = capi.PAIR_SHUFFLE if mode is None else mode
     with capi.Soup(lang, n) as g:
+        if flow is not None:
+            g.set_pipeline(flow)
         g.init_programs(capi.epoch_seed(seed, 0))
         if pattern is not None:
             g.set_interactions(*pattern)
@@ -178,7 +189,14 @@ def run_other_configs(capi, epochs):
     P = 1 << 18
     langs = {}
     for lang in ("bff", "bff8", "forth", "forthtrivial", "subleq", "rsubleq4"):
-        langs[lang] = run_soup_timed(capi, lang, 131072, 1, P, 4, epochs)
+        # One launch per epoch, the multi-epoch dataflow kernel, and the
+        # default (which picks the dataflow kernel for the Forth family).
+        loop = run_soup_timed(capi, lang, 131072, 1, P, 4, epochs, flow=capi.FLOW_OFF)
+        flow = run_soup_timed(capi, lang, 131072, 1, P, 4, epochs, flow=capi.FLOW_ON)
+        auto = run_soup_timed(capi, lang, 131072, 1, P, 4, epochs)
+        auto["loop_ms_per_epoch"] = loop["ms_per_epoch"]
+        auto["dataflow_ms_per_epoch"] = flow["ms_per_epoch"]
+        langs[lang] = auto
     langs["bff_noheads_grid_512x256"] = run_soup_timed(
         capi, "bff_noheads", 512 * 256, 1, P, 4, epochs, mode=capi.PAIR_2D,
         pattern=grid_pattern(512, 256, 2))
@@ -273,7 +291,9 @@ def main():
     K = args.steps
     dt = total_ms / 1e3
     work = (n // 2) * 128 * C_MUT * K + steps * c_step(lang)
-    achieved = work / (kern_ms / 1e3)
+    lane_ops = work / (kern_ms / 1e3)
+    algo_bytes = (n // 2) * BYTES_PER_SLOT
+    achieved = algo_bytes / (kern_ms / 1e3 / K)
     line = {
         "metric": METRIC,
         "value": ops / dt,
@@ -300,15 +320,19 @@ def main():
         "wall_s": wall,
         "roofline": {
             "bound": "valu",
-            "achieved": achieved / 1e12,
+            "achieved": lane_ops / 1e12,
             "peak": PEAK_LANE_OPS / 1e12,
             "unit": "T int32 lane-ops/s",
-            "frac": achieved / PEAK_LANE_OPS,
+            "frac": lane_ops / PEAK_LANE_OPS,
             "traffic": traffic_per_launch(),
-            "kernel": "k_epoch (mutate + evaluate + scatter)",
+            "kernel": "k_epoch (gather + mutate + evaluate + scatter), one launch per epoch",
             "kernel_ms_avg": kern_ms / K,
             "work_per_launch": work / K,
             "model": "W = (N/2)*128*32 + sum(steps)*C_step",
+            "limiter": "latency of the longest interaction of the epoch",
+            "hbm": {"achieved": achieved / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                    "frac": achieved / PEAK_HBM,
+                    "algorithmic_bytes_per_launch": algo_bytes},
         },
         "cpu_baseline": None,
     }

@@ -105,15 +105,16 @@ constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 
 // Where the dataflow kernel beats one launch per epoch (measured on MI355X
-// at 2^17 programs, 128 epochs): languages whose longest interactions run
-// far longer than a typical one, up to 2^18 programs.  At 2^22 programs the
-// epochs are throughput-bound and the per-launch loop is faster.
+// at 2^17 programs over 16 and 64 epochs): languages whose longest
+// interactions run far longer than a typical one (Forth family: 1.6-2.8x;
+// bff_selfmove, bff_noheads_4bit: 1.2x), up to 2^18 programs.  At 2^22
+// programs the epochs are throughput-bound and the per-launch loop is
+// faster.
 bool flow_pays(int lang, uint64_t n) {
   if (n > (1ULL << 18)) return false;
   switch (lang) {
-    case kBff8: case kBff8Noheads: case kBffNoheads4bit: case kBffSelfmove:
+    case kBffNoheads4bit: case kBffSelfmove:
     case kForth: case kForthcopy: case kForthtrivial: case kForthtrivialReset:
-    case kRsubleq4:
       return true;
     default:
       return false;

@@ -12,17 +12,25 @@
 // MutateAndRunPrograms (common_language.h:157-193) per epoch in order: every
 // program is read by exactly the interaction that follows its last writer.
 //
-// Work items are "pair k of segment epoch j" (id j * slots + k).  Epoch 0's
-// items start in the ready queue; an item that finishes decrements the
-// pending count of the epoch-(j+1) pairs holding its two programs, and the
-// one that brings a count to zero appends that pair to the queue.  Waves
-// claim runs of up to 64 queued items (a lane each), wait until each item's
-// count reads zero, take an acquire fence, run the interactions, take a
-// release fence and notify.  Pending counts are read-modify-written after
-// the writer's release fence, so reading zero synchronizes with both
-// writers.  Every wait is bounded by a wall-clock limit: past it the kernel
-// raises the abort flag, every wave leaves at its next claim and the host
-// reports the error.
+// Work items are "pair k of segment epoch j" (id j * slots + k).  A wave
+// runs up to 64 items at a time, one per lane, then takes a release fence
+// and decrements the pending count of the epoch-(j+1) pairs holding each
+// item's two programs.  An item whose count this wave brought to zero is
+// ready: the wave keeps the first 64 such items for its next round (no
+// queue traffic on the common path) and appends the rest to a global ready
+// queue, from which waves with free lanes claim runs of items.  Epoch 0's
+// items are handed to the waves directly, 64 each, and the rest queued.  A
+// queued item is run after its pending count reads zero and an acquire
+// fence: the counts are read-modify-written after each writer's release
+// fence, so reading zero synchronizes with both writers.  Every wait is
+// bounded by a wall-clock limit: past it the kernel raises the abort flag,
+// every wave leaves at its next claim and the host reports the error.
+//
+// The gain is largest where single interactions run far longer than the
+// rest (the Forth family at 8192 steps): the rest of the soup moves on
+// while they run.  Where the longest interaction is short compared with the
+// fences, queue operations and global-memory latency of a round (about
+// 100-150 us per wave), one launch per epoch stays faster.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
