@@ -327,61 +327,45 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: features, both hidden layers and the output of 128 tapes
-// per block, with nothing but the tapes read from and the predictions written
-// to memory.
+// Fused inference, weight-stationary: one persistent workgroup per CU holds
+// both hidden layers' weights in registers for the whole launch and streams
+// the tapes through, 64 per pass, so no weight is read from memory more than
+// once per workgroup.
 //
-//   1. byte counts of each tape into an LDS histogram (four u8 counters per
-//      dword, LDS atomics; counts never exceed 128);
-//   2. a1 = relu(counts (E W1 / 128) + b1): v_mfma_f32_16x16x32_bf16 with A
-//      fragments converted from the counts (exact in bf16) and the folded,
-//      pre-scaled weights staged through LDS in K-chunks of 64;
-//   3. a2 = relu(a1 W2 + b2) with a1 kept in LDS as bf16 (over the histogram);
-//   4. y = a2 . w3 + b3, reduced across lanes and waves in LDS.
-//
-// Block: 8 waves; wave w owns rows 32 (w / 2) .. +32 (two 16-row blocks) and
-// columns 128 (w % 2) .. +128 (eight 16-column blocks).  Fragment layout of
-// the 16x16x32 MFMA: lane l holds A(l % 16, 8 (l / 16) + 0..7),
-// B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).
+// Workgroup: 8 waves; wave w owns output columns 32 w .. 32 w + 31 of both
+// layers, held as MFMA B fragments (2 column tiles x 8 K-steps of 32, four
+// VGPRs each: 64 VGPRs per layer).  Fragment layout of
+// v_mfma_f32_16x16x32_bf16: lane l holds A(l % 16, 8 (l / 16) + 0..7),
+// B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).  A pass:
+//   1. byte counts of 64 tapes into an LDS histogram (four u8 counters per
+//      dword, LDS integer atomics; counts never exceed 128), from tapes
+//      loaded into registers during the previous pass, then converted once,
+//      cooperatively, to a bf16 copy (exact: integers up to 256);
+//   2. a1 = relu(counts (E W1 / 128) + b1): A fragments are 16-byte LDS
+//      reads of the bf16 counts; a1 written to LDS as bf16
+//      (v_cvt_pk_bf16_f32, round to nearest even);
+//   3. a2 = relu(a1 W2 + b2), rounded to bf16 as in the unfused path, and
+//      y = a2 . w3 + b3 reduced across lanes and waves in LDS.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-constexpr int kInfM = 128;        // tapes per block
+constexpr int kInfRows = 64;       // tapes per pass
 constexpr int kInfThreads = 512;
-constexpr int kHistStride = 264;  // bytes per histogram row (256 counters + pad)
-constexpr int kActStride = 264;   // bf16 per activation row (256 + pad)
-constexpr int kWKC = 64;          // weight chunk along K
-constexpr int kWKP = 72;          // padded chunk row (bf16)
+constexpr int kRowStride = 264;    // bf16 per histogram / activation row (256 + pad)
 
-// Eight u8 counts as bf16 (exact: counts are at most 128).  The 1/128
-// feature scale is folded into the first-layer weights (k_fold), so the
-// counts go to the MFMA as they are: byte -> float (v_cvt_f32_ubyteN), and
-// the high halves of two floats packed into one dword (v_perm_b32).
-__device__ __forceinline__ bf16x8 counts_to_bf16(uint64_t c8) {
-  uint32_t w[4];
-#pragma unroll
-  for (int h = 0; h < 2; h++) {
-    const uint32_t v = (uint32_t)(c8 >> (32 * h));
-    const uint32_t f0 = __float_as_uint((float)(v & 255u));
-    const uint32_t f1 = __float_as_uint((float)((v >> 8) & 255u));
-    const uint32_t f2 = __float_as_uint((float)((v >> 16) & 255u));
-    const uint32_t f3 = __float_as_uint((float)(v >> 24));
-    w[2 * h] = __builtin_amdgcn_perm(f1, f0, 0x07060302u);
-    w[2 * h + 1] = __builtin_amdgcn_perm(f3, f2, 0x07060302u);
-  }
-  return __builtin_bit_cast(bf16x8, w);
+constexpr int kCountStride = 264;  // bytes per u8 count row (256 + pad)
+
+// Workgroup barrier ordering LDS accesses only: unlike __syncthreads() it
+// does not wait for outstanding global loads, so the next pass's tapes stay
+// in flight across the barriers of the current pass.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Stages rows [0, 256) x K-columns [kc, kc + 64) of WT ([256][K] bf16) into ws.
-template <int K>
-__device__ __forceinline__ void stage_weights(const uint16_t *__restrict__ WT, int kc,
-                                              uint16_t *__restrict__ ws) {
-#pragma unroll
-  for (int c = 0; c < 2048 / kInfThreads; c++) {
-    const int idx = threadIdx.x + kInfThreads * c;  // 2048 chunks of 8 bf16
-    const int n = idx >> 3, part = idx & 7;
-    *reinterpret_cast<uint4 *>(ws + n * kWKP + part * 8) =
-        *reinterpret_cast<const uint4 *>(WT + (int64_t)n * K + kc + part * 8);
-  }
+// Round to bf16 (nearest even) with the hardware conversion.
+__device__ __forceinline__ uint16_t to_bf16(float x) {
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 
 __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restrict__ tapes,
@@ -390,124 +374,136 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
                                                  const uint16_t *__restrict__ w2t,
                                                  const float *__restrict__ params,
                                                  float *__restrict__ y) {
-  __shared__ __align__(16) uint8_t act_u[kInfM * kActStride * 2];  // histogram, then a1
-  __shared__ __align__(16) uint16_t ws[kMlpHid * kWKP];
-  __shared__ float ypart[kInfM];
+  __shared__ __align__(16) uint32_t cnt8[kInfRows * kCountStride / 4];
+  __shared__ __align__(16) uint16_t hist[kInfRows * kRowStride];
+  __shared__ __align__(16) uint16_t act[kInfRows * kRowStride];
+  __shared__ float ypart[kInfRows];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const uint64_t row0 = (uint64_t)blockIdx.x * kInfM;
-  uint32_t *hist = reinterpret_cast<uint32_t *>(act_u);
-  for (int i = tid; i < kInfM * kHistStride / 4; i += kInfThreads) hist[i] = 0;
-  if (tid < kInfM) ypart[tid] = 0.f;
-  __syncthreads();
+  const int r16 = lane & 15, kg = lane >> 4;
+  const int col0 = 32 * w;
 
-  // 1. Counts: chunk c = 16 bytes of tape c / 8.
+  // This wave's weights, bias and output weights, for the whole launch.
+  bf16x8 wb1[2][8], wb2[2][8];
 #pragma unroll
-  for (int q = 0; q < kInfM * 8 / kInfThreads; q++) {
-    const int c = tid + kInfThreads * q;
-    const int r = c >> 3, part = c & 7;
-    if (row0 + r < count) {
-      const uint4 v = *reinterpret_cast<const uint4 *>(tapes + (row0 + r) * kTape2 + part * 16);
-      const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
-      uint32_t *hr = hist + r * (kHistStride / 4);
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int64_t off = (int64_t)(col0 + 16 * c + r16) * kMlpHid + 32 * s + 8 * kg;
+      wb1[c][s] = *reinterpret_cast<const bf16x8 *>(ew1t + off);
+      wb2[c][s] = *reinterpret_cast<const bf16x8 *>(w2t + off);
+    }
+  float b1[2], b2[2], w3[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++) {
+    b1[c] = params[kOffB1 + col0 + 16 * c + r16];
+    b2[c] = params[kOffB2 + col0 + 16 * c + r16];
+    w3[c] = params[kOffW3 + col0 + 16 * c + r16];
+  }
+  const float b3 = params[kOffB3];
+
+  // Thread tid loads 16 bytes (chunk tid % 8) of tape tid / 8 of each pass.
+  const int tr = tid >> 3, tpart = tid & 7;
+  const uint64_t passes = (count + kInfRows - 1) / kInfRows;
+  uint4 next = make_uint4(0, 0, 0, 0);
+  uint64_t p = blockIdx.x;
+  if (p < passes && p * kInfRows + tr < count)
+    next = *reinterpret_cast<const uint4 *>(tapes + (p * kInfRows + tr) * kTape2 + tpart * 16);
+  for (; p < passes; p += gridDim.x) {
+    const uint64_t row0 = p * kInfRows;
+    const bool have = row0 + tr < count;
+    const uint4 cur = next;
+    // 1. Counts (the previous pass is done with hist, act and ypart).
+    for (int i = tid; i < kInfRows * kCountStride / 4; i += kInfThreads) cnt8[i] = 0;
+    if (tid < kInfRows) ypart[tid] = 0.f;
+    lds_barrier();
+    const uint64_t pn = p + gridDim.x;
+    if (pn < passes && pn * kInfRows + tr < count)
+      next = *reinterpret_cast<const uint4 *>(tapes + (pn * kInfRows + tr) * kTape2 + tpart * 16);
+    if (have) {
+      const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
+      uint32_t *hr = cnt8 + tr * (kCountStride / 4);
 #pragma unroll
       for (int b = 0; b < 16; b++) {
-        const uint32_t cur = (wd[b >> 2] >> (8 * (b & 3))) & 255u;
-        atomicAdd(&hr[cur >> 2], 1u << (8 * (cur & 3)));
+        const uint32_t v = (wd[b >> 2] >> (8 * (b & 3))) & 255u;
+        atomicAdd(&hr[v >> 2], 1u << (8 * (v & 3)));
       }
     }
-  }
-
-  const int r16 = lane & 15, kg = lane >> 4;
-  const int rb = 32 * (w >> 1), cb = 128 * (w & 1);
-  f32x4 acc[2][8];
+    lds_barrier();
+    // Counts to bf16, once for all waves: thread tid converts 32 counts
+    // (row tid / 8, bins 32 (tid % 8) .. +31).
+    {
+      const uint32_t *src = cnt8 + tr * (kCountStride / 4) + 8 * tpart;
+      uint32_t *dst = reinterpret_cast<uint32_t *>(hist + tr * kRowStride + 32 * tpart);
 #pragma unroll
-  for (int t = 0; t < 2; t++)
-#pragma unroll
-    for (int b = 0; b < 8; b++) acc[t][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-  // 2. a1 = relu(x (E W1) + b1).
-  for (int kc = 0; kc < kMlpIn; kc += kWKC) {
-    __syncthreads();  // counts complete / previous chunk consumed
-    stage_weights<kMlpIn>(ew1t, kc, ws);
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < kWKC; ks += 32) {
-      bf16x8 fa[2];
-#pragma unroll
-      for (int t = 0; t < 2; t++)
-        fa[t] = counts_to_bf16(*reinterpret_cast<const uint64_t *>(
-            act_u + (rb + 16 * t + r16) * kHistStride + kc + ks + 8 * kg));
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const bf16x8 fb =
-            *reinterpret_cast<const bf16x8 *>(ws + (cb + 16 * b + r16) * kWKP + ks + 8 * kg);
-#pragma unroll
-        for (int t = 0; t < 2; t++)
-          acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[t], fb, acc[t][b], 0, 0, 0);
+      for (int q = 0; q < 8; q++) {
+        const uint32_t v = src[q];
+        const __bf16 h0 = (__bf16)(float)(v & 255u), h1 = (__bf16)(float)((v >> 8) & 255u);
+        const __bf16 h2 = (__bf16)(float)((v >> 16) & 255u), h3 = (__bf16)(float)(v >> 24);
+        dst[2 * q] = (uint32_t)__builtin_bit_cast(uint16_t, h0) |
+                     (uint32_t)__builtin_bit_cast(uint16_t, h1) << 16;
+        dst[2 * q + 1] = (uint32_t)__builtin_bit_cast(uint16_t, h2) |
+                         (uint32_t)__builtin_bit_cast(uint16_t, h3) << 16;
       }
     }
-  }
-  __syncthreads();  // every histogram read is done before a1 overwrites it
-  uint16_t *act = reinterpret_cast<uint16_t *>(act_u);
+    lds_barrier();
+
+    // 2. a1 = relu(x (E W1) + b1).
+    f32x4 acc[4][2];
 #pragma unroll
-  for (int t = 0; t < 2; t++)
+    for (int t = 0; t < 4; t++)
 #pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const int col = cb + 16 * b + r16;
-      const float bias = params[kOffB1 + col];
+      for (int c = 0; c < 2; c++) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const bf16x8 fa = *reinterpret_cast<const bf16x8 *>(hist + (16 * t + r16) * kRowStride +
+                                                            32 * s + 8 * kg);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, wb1[c][s], acc[t][c], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const int col = col0 + 16 * c + r16;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          act[(16 * t + 4 * kg + r) * kRowStride + col] = to_bf16(fmaxf(acc[t][c][r] + b1[c], 0.f));
+          acc[t][c][r] = 0.f;
+        }
+      }
+    lds_barrier();
+
+    // 3. a2 = relu(a1 W2 + b2); y = a2 . w3 + b3.
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const bf16x8 fa = *reinterpret_cast<const bf16x8 *>(act + (16 * t + r16) * kRowStride +
+                                                            32 * s + 8 * kg);
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, wb2[c][s], acc[t][c], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        act[(rb + 16 * t + 4 * kg + r) * kActStride + col] = f2bf(fmaxf(acc[t][b][r] + bias, 0.f));
-        acc[t][b][r] = 0.f;
+        float v = 0.f;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+          v += (float)(__bf16)fmaxf(acc[t][c][r] + b2[c], 0.f) * w3[c];
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+        if (r16 == 0) atomicAdd(&ypart[16 * t + 4 * kg + r], v);
       }
-    }
-
-  // 3. a2 = relu(a1 W2 + b2).
-  for (int kc = 0; kc < kMlpHid; kc += kWKC) {
-    __syncthreads();  // a1 complete / previous chunk consumed
-    stage_weights<kMlpHid>(w2t, kc, ws);
-    __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < kWKC; ks += 32) {
-      bf16x8 fa[2];
-#pragma unroll
-      for (int t = 0; t < 2; t++)
-        fa[t] = *reinterpret_cast<const bf16x8 *>(act + (rb + 16 * t + r16) * kActStride + kc +
-                                                  ks + 8 * kg);
-#pragma unroll
-      for (int b = 0; b < 8; b++) {
-        const bf16x8 fb =
-            *reinterpret_cast<const bf16x8 *>(ws + (cb + 16 * b + r16) * kWKP + ks + 8 * kg);
-#pragma unroll
-        for (int t = 0; t < 2; t++)
-          acc[t][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[t], fb, acc[t][b], 0, 0, 0);
-      }
-    }
+    lds_barrier();
+    if (tid < kInfRows && row0 + tid < count) y[row0 + tid] = ypart[tid] + b3;
   }
-
-  // 4. y = relu(.) . w3 + b3; a2 is rounded to bf16 as in the unfused path.
-  float part[2][4] = {};
-#pragma unroll
-  for (int t = 0; t < 2; t++)
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const int col = cb + 16 * b + r16;
-      const float bias = params[kOffB2 + col], w3 = params[kOffW3 + col];
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        part[t][r] += bf2f(f2bf(fmaxf(acc[t][b][r] + bias, 0.f))) * w3;
-    }
-#pragma unroll
-  for (int t = 0; t < 2; t++)
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      float v = part[t][r];
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-      if (r16 == 0) atomicAdd(&ypart[rb + 16 * t + 4 * kg + r], v);
-    }
-  __syncthreads();
-  if (tid < kInfM && row0 + tid < count) y[row0 + tid] = ypart[tid] + params[kOffB3];
 }
 
 hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
@@ -517,9 +513,15 @@ hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
     hipError_t e = mlp_fold(m, st);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_predict, dim3((unsigned)((count + kInfM - 1) / kInfM)), dim3(kInfThreads),
-                     0, st,
-                     tapes, count, m.ew1t, m.w2t, m.params, y);
+  int dev = 0, cus = 0;
+  hipError_t e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+    return e;
+  const uint64_t passes = (count + kInfRows - 1) / kInfRows;
+  const unsigned grid = (unsigned)(passes < (uint64_t)cus ? passes : (uint64_t)cus);
+  hipLaunchKernelGGL(k_predict, dim3(grid), dim3(kInfThreads), 0, st, tapes, count, m.ew1t,
+                     m.w2t, m.params, y);
   return hipGetLastError();
 }
 
