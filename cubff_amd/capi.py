@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libcubff_hip.so")
 
 PAIR_SHUFFLE, PAIR_FIXED, PAIR_CYCLIC, PAIR_2D, PAIR_HOST = 0, 1, 2, 3, 4
-FLOW_OFF, FLOW_ON, FLOW_AUTO = 0, 1, 2  # cubff_set_pipeline modes
+FLOW_OFF, FLOW_ON, FLOW_AUTO, FLOW_DEFER = 0, 1, 2, 3  # cubff_set_pipeline modes
 
 # Every symbol include/cubff_hip.h declares.
 EXPORTS = [
@@ -23,7 +23,7 @@ EXPORTS = [
     "cubff_ctx_create", "cubff_ctx_destroy", "cubff_init_programs",
     "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
     "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
-    "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats",
+    "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats", "cubff_set_defer_budget",
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
@@ -86,7 +86,8 @@ def load():
                                         ctypes.POINTER(ctypes.c_float),
                                         ctypes.POINTER(ctypes.c_float)]),
         "cubff_set_pipeline": (_i, [_vp, _i, ctypes.c_double]),
-        "cubff_flow_stats": (_i, [_vp, _u64p, _u64p]),
+        "cubff_flow_stats": (_i, [_vp, _u64p, _u64p, _u64p]),
+        "cubff_set_defer_budget": (_i, [_vp, ctypes.c_uint32]),
         "cubff_read_counters": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_reset_counters": (_i, [_vp]),
         "cubff_read_pair_ops": (_i, [_vp, _u32p, _u32p, _u64]),
@@ -253,10 +254,15 @@ class Soup:
         _check(load().cubff_set_pipeline(self._h, int(mode), float(wait_limit_s)))
 
     def flow_stats(self):
-        """(interactions, claims) of the dataflow kernel since creation."""
-        a, b = ctypes.c_uint64(0), ctypes.c_uint64(0)
-        _check(load().cubff_flow_stats(self._h, ctypes.byref(a), ctypes.byref(b)))
-        return int(a.value), int(b.value)
+        """(interactions, claims) of the dataflow kernel and interactions
+        deferred to the resolver, since creation."""
+        a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(load().cubff_flow_stats(self._h, ctypes.byref(a), ctypes.byref(b),
+                                       ctypes.byref(c)))
+        return int(a.value), int(b.value), int(c.value)
+
+    def set_defer_budget(self, budget: int):
+        _check(load().cubff_set_defer_budget(self._h, budget))
 
     def run_epochs_timed(self, mode: int, params_seed: int, first_epoch: int,
                          count: int, mutation_prob: int):

@@ -13,6 +13,9 @@
 
 using namespace cubff;
 
+// Command budget of an interaction in the deferring epoch kernel.
+constexpr uint32_t kDeferBudget = 768;
+
 struct cubff_ctx {
   int lang = 0;
   int device = 0;
@@ -38,6 +41,16 @@ struct cubff_ctx {
            *flow_queue = nullptr;
   FlowCtl *flow_ctl = nullptr;
   uint64_t flow_claims = 0, flow_items = 0;  // since creation
+  // Deferred-interaction path (defer.hip): second stream, program owners,
+  // the list (defer_cap entries), run generation, command budget.
+  hipStream_t stream2 = nullptr;
+  uint32_t *owner = nullptr;
+  DeferItem *ditems = nullptr;
+  DeferCtl *dctl = nullptr;
+  uint64_t defer_cap = 0;
+  uint32_t defer_gen = 0;
+  uint32_t defer_budget = kDeferBudget;
+  uint64_t defer_items = 0;  // interactions deferred since creation
 };
 
 namespace {
@@ -103,22 +116,36 @@ int ensure_2d_scratch(cubff_ctx *c) {
 constexpr uint64_t kFlowMaxEpochs = 64;
 constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
+constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
 
-// Where the dataflow kernel beats one launch per epoch (measured on MI355X
-// at 2^17 programs over 16 and 64 epochs): languages whose longest
-// interactions run far longer than a typical one (Forth family: 1.6-2.8x;
-// bff_selfmove, bff_noheads_4bit: 1.2x), up to 2^18 programs.  At 2^22
-// programs the epochs are throughput-bound and the per-launch loop is
-// faster.
+// Which multi-epoch path the default picks (measured on MI355X at 2^17
+// programs, 64 epochs, against one launch per epoch).  Deferring epoch
+// kernels: bff 2.0x, bff_perm 1.8x, bff_noheads 1.4-1.6x, subleq 2.2x --
+// languages whose interactions rarely exceed the budget.  Dataflow kernel:
+// the Forth family 1.6-2.2x, bff_selfmove 1.3x, bff_noheads_4bit 1.3x,
+// bff8_noheads 1.1x -- long interactions are common there and would mostly
+// be deferred.  Up to 2^18 programs; at 2^22 the epochs are throughput-bound
+// and one launch per epoch is used.
+bool defer_pays(int lang, uint64_t n) {
+  if (n > (1ULL << 18)) return false;
+  return lang == kBff || lang == kBffNoheads || lang == kBffPerm || lang == kSubleq;
+}
+
 bool flow_pays(int lang, uint64_t n) {
   if (n > (1ULL << 18)) return false;
   switch (lang) {
-    case kBffNoheads4bit: case kBffSelfmove:
+    case kBffNoheads4bit: case kBffSelfmove: case kBff8Noheads:
     case kForth: case kForthcopy: case kForthtrivial: case kForthtrivialReset:
       return true;
     default:
       return false;
   }
+}
+
+// Deferred-interaction segments: the list holds every pair slot of the
+// segment in the worst case (32 bytes each), within kFlowBytes.
+uint64_t defer_segment_epochs(uint64_t n) {
+  return std::max<uint64_t>(2, std::min(kFlowBytes / (16 * n), kFlowMaxEpochs));
 }
 
 uint64_t flow_segment_epochs(uint64_t n) {
@@ -208,18 +235,118 @@ int run_flow_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   return CUBFF_OK;
 }
 
+int ensure_defer_scratch(cubff_ctx *c, uint64_t epochs) {
+  const uint64_t need = epochs * (c->n / 2);
+  int r;
+  if (!c->stream2) {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+    if (e != hipSuccess)
+      return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+  }
+  if (!c->owner && (r = dev_alloc(&c->owner, c->n))) return r;
+  if (!c->dctl && (r = dev_alloc(&c->dctl, 1))) return r;
+  if (c->defer_cap < need) {
+    dev_free(c->ditems);
+    c->defer_cap = 0;
+    if ((r = dev_alloc(&c->ditems, need))) return r;
+    // Generation words start at 0; runs use generations >= 1.
+    HIP_TRY(hipMemsetAsync(c->ditems, 0, need * sizeof(DeferItem), c->stream));
+    c->defer_cap = need;
+  }
+  return CUBFF_OK;
+}
+
+// Epochs [first, first + epochs) with the deferring epoch kernel on the
+// context stream and the resolver on the second stream.  With ev, records
+// ev[0]/ev[1] around the epoch kernels (their summed time is not separable
+// from the pairing launches in between, so ev brackets the whole sequence).
+int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first,
+                      uint32_t epochs, uint32_t prob, hipEvent_t *ev) {
+  int r;
+  if ((r = ensure_defer_scratch(c, epochs))) return r;
+  const uint64_t n = c->n;
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+  DeferArgs a{};
+  a.soup = c->soup;
+  a.owner = c->owner;
+  a.items = c->ditems;
+  a.ctl = c->dctl;
+  a.n = n;
+  a.slots = (uint32_t)(n / 2);
+  a.epochs = epochs;
+  a.capacity = (uint32_t)(epochs * (n / 2));
+  a.budget = c->defer_budget;
+  a.max_backlog = 2 * kResolveWaves;
+  a.gen = ++c->defer_gen;
+  if (a.gen == 0) a.gen = c->defer_gen = 1;  // 0 marks never-written entries
+  a.prob = prob;
+  a.sm_seed = splitmix64(params_seed);
+  a.first_epoch = first;
+  a.pair_ops = c->pair_ops;
+  a.pair_steps = c->pair_steps;
+  a.counters = c->counters;
+  a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
+  hipEvent_t started = nullptr, resolved = nullptr;
+  auto cleanup = [&]() {
+    if (started) (void)hipEventDestroy(started);
+    if (resolved) (void)hipEventDestroy(resolved);
+  };
+  hipError_t e = hipEventCreateWithFlags(&started, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&resolved, hipEventDisableTiming);
+  if (e == hipSuccess) e = launch_fill_u32(c->stream, c->owner, kNone, n);
+  if (e == hipSuccess) e = launch_defer_start(c->stream, c->dctl);
+  if (e == hipSuccess) e = hipEventRecord(started, c->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream2, started, 0);
+  if (e == hipSuccess) e = launch_resolve(c->stream2, c->lang, a, kResolveWaves);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(CUBFF_EHIP, std::string("deferred run setup failed: ") + hipGetErrorString(e));
+  }
+  if (ev) e = hipEventRecord(ev[0], c->stream);
+  r = CUBFF_OK;
+  for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
+    if ((r = cubff_pair(c, mode, params_seed, first + j))) break;
+    e = launch_epoch_defer(c->stream, c->lang, a, c->shuf, j);
+  }
+  // Close the list whatever happened, so the resolver leaves.
+  hipError_t e2 = launch_defer_close(c->stream, c->dctl);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess) e = hipEventRecord(resolved, c->stream2);
+  if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, resolved, 0);
+  DeferCtl ctl{};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&ctl, c->dctl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream);
+  hipError_t e3 = hipStreamSynchronize(c->stream2);
+  if (e == hipSuccess) e = e3;
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (r) return r;
+  if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
+  c->defer_items += ctl.final_tail;
+  if (ctl.abort)
+    return fail(CUBFF_EHIP, "deferred-interaction resolver stopped: a wait exceeded " +
+                                std::to_string(c->flow_wait_s) + " s; soup state is undefined");
+  return CUBFF_OK;
+}
+
 // The body of cubff_run_epochs / cubff_run_epochs_timed.  Closed-form and
 // shuffle pairings of two or more epochs go through the dataflow kernel in
 // segments; otherwise one pairing + epoch launch per epoch.  timing (2
 // floats, nullable) receives the total and the epoch-kernel milliseconds.
 int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first_epoch,
                     uint64_t count, uint32_t prob, float *timing) {
-  const uint64_t seg = flow_segment_epochs(c->n);
-  const bool flow = (c->pipeline == CUBFF_FLOW_ON ||
+  const bool closed_form = mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
+                           mode == CUBFF_PAIR_CYCLIC;
+  const bool defer = (c->pipeline == CUBFF_FLOW_DEFER ||
+                      (c->pipeline == CUBFF_FLOW_AUTO && defer_pays(c->lang, c->n))) &&
+                     count >= 2 && closed_form;
+  const uint64_t seg = defer ? defer_segment_epochs(c->n) : flow_segment_epochs(c->n);
+  const bool flow = !defer &&
+                    (c->pipeline == CUBFF_FLOW_ON ||
                      (c->pipeline == CUBFF_FLOW_AUTO && flow_pays(c->lang, c->n))) &&
-                    count >= 2 && seg >= 2 &&
-                    (mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
-                     mode == CUBFF_PAIR_CYCLIC);
+                    count >= 2 && seg >= 2 && closed_form;
   const uint64_t sm = splitmix64(params_seed);
   std::vector<hipEvent_t> ev;
   auto destroy = [&]() {
@@ -227,7 +354,7 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
   };
   int r = CUBFF_OK;
   if (timing) {
-    const uint64_t launches = flow ? (count + seg - 1) / seg : count;
+    const uint64_t launches = (flow || defer) ? (count + seg - 1) / seg : count;
     ev.resize(2 + 2 * launches);
     for (auto &e : ev) {
       hipError_t he = hipEventCreate(&e);
@@ -246,6 +373,12 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
     if (flow && left >= 2) {
       const uint32_t ep = (uint32_t)std::min(left, seg);
       r = run_flow_segment(c, mode, params_seed, e, ep, prob, kev);
+      e += ep;
+      continue;
+    }
+    if (defer && left >= 2) {
+      const uint32_t ep = (uint32_t)std::min(left, seg);
+      r = run_defer_segment(c, mode, params_seed, e, ep, prob, kev);
       e += ep;
       continue;
     }
@@ -391,6 +524,11 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   dev_free(c->flow_cnt);
   dev_free(c->flow_queue);
   dev_free(c->flow_ctl);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  dev_free(c->owner);
+  dev_free(c->ditems);
+  dev_free(c->dctl);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUBFF_OK;
@@ -524,11 +662,20 @@ int cubff_run_epochs_timed(cubff_ctx *c, int mode, uint64_t params_seed,
   return CUBFF_OK;
 }
 
-int cubff_flow_stats(cubff_ctx *c, uint64_t *items, uint64_t *claims) {
+int cubff_flow_stats(cubff_ctx *c, uint64_t *items, uint64_t *claims, uint64_t *deferred) {
   int r;
   if ((r = check_ctx(c))) return r;
   if (items) *items = c->flow_items;
   if (claims) *claims = c->flow_claims;
+  if (deferred) *deferred = c->defer_items;
+  return CUBFF_OK;
+}
+
+int cubff_set_defer_budget(cubff_ctx *c, uint32_t budget) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (budget == 0) return fail(CUBFF_EINVAL, "budget must be > 0 commands");
+  c->defer_budget = budget;
   return CUBFF_OK;
 }
 
@@ -536,8 +683,9 @@ int cubff_set_pipeline(cubff_ctx *c, int enable, double wait_limit_s) {
   int r;
   if ((r = check_ctx(c))) return r;
   if (!(wait_limit_s >= 0)) return fail(CUBFF_EINVAL, "wait limit must be >= 0 seconds");
-  if (enable != CUBFF_FLOW_OFF && enable != CUBFF_FLOW_ON && enable != CUBFF_FLOW_AUTO)
-    return fail(CUBFF_EINVAL, "pipeline mode must be CUBFF_FLOW_OFF, _ON or _AUTO");
+  if (enable != CUBFF_FLOW_OFF && enable != CUBFF_FLOW_ON && enable != CUBFF_FLOW_AUTO &&
+      enable != CUBFF_FLOW_DEFER)
+    return fail(CUBFF_EINVAL, "pipeline mode must be CUBFF_FLOW_OFF, _ON, _AUTO or _DEFER");
   c->pipeline = enable;
   c->flow_wait_s = wait_limit_s;
   return CUBFF_OK;

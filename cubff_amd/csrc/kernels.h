@@ -54,6 +54,36 @@ struct FlowArgs {
   uint32_t fixed;           // epoch-0 items handed to the waves directly (set at launch)
 };
 
+// Deferred-interaction path (defer.hip).  Control block: list entries
+// claimed by the resolver (head), appended (tail), the closed flag and the
+// list length at closing, the abort flag.
+struct alignas(16) DeferCtl {
+  uint32_t head, tail, closed, final_tail, abort, resolved, pad[2];
+};
+
+// One deferred interaction: segment epoch, pair slot, its programs, the list
+// entries that owe them (kNone if none), and the run generation once the
+// entry is complete (ready) and once its result is written (done).
+struct DeferItem {
+  uint32_t e, k, p1, p2, dep1, dep2, ready, done;
+};
+
+constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
+
+struct DeferArgs {
+  uint8_t *soup;
+  uint32_t *owner;  // n entries
+  DeferItem *items;
+  DeferCtl *ctl;
+  uint64_t n;
+  uint32_t slots, epochs, capacity, budget, gen, prob;
+  uint32_t max_backlog;  // entries not yet resolved past which nothing more is given up
+  uint64_t sm_seed, first_epoch;
+  uint32_t *pair_ops, *pair_steps;  // written for the segment's last epoch
+  unsigned long long *counters;
+  uint64_t spin_ticks;
+};
+
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
                        uint64_t seed_value, uint64_t count, int zero_init);
 hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
@@ -85,5 +115,10 @@ hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,
 hipError_t launch_flow_pairof(hipStream_t st, const uint32_t *shuf, uint32_t *pairof,
                               uint64_t n, uint32_t epochs);
 hipError_t launch_flow(hipStream_t st, int lang, const FlowArgs &a);
+hipError_t launch_defer_start(hipStream_t st, DeferCtl *c);
+hipError_t launch_defer_close(hipStream_t st, DeferCtl *c);
+hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,
+                              const uint32_t *shuf, uint32_t e_rel);
+hipError_t launch_resolve(hipStream_t st, int lang, const DeferArgs &a, unsigned waves);
 
 }  // namespace cubff

@@ -35,6 +35,11 @@ inline const char *const kLangNames[kNumLangs] = {
     "forthcopy",    "forthtrivial",       "forthtrivial_reset", "subleq",
     "rsubleq4"};
 
+// evaluate<true>(..., budget) stops once it has executed more than `budget`
+// commands and reports *steps = kOverBudget (the tape is then partly run and
+// must be discarded).
+constexpr uint32_t kOverBudget = 0xffffffffu;
+
 // Languages whose interpreter uses a second per-lane LDS region: the Forth
 // stack, and the tape snapshot of the BFF cycle check.
 constexpr bool lang_needs_aux(int lang) {
@@ -268,10 +273,10 @@ struct BffLang {
     return 2654435761u * (2u * (uint32_t)p + 1u);
   }
 
-  template <class T, class A>
+  template <bool kBudget = false, class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &aux,
                                       uint32_t stepcount, uint32_t *steps,
-                                      uint32_t *work = nullptr) {
+                                      uint32_t *work = nullptr, uint32_t budget = 0) {
     Mask128 mc{0, 0}, ml{0, 0}, me{0, 0};
     for (int w = 0; w < kTape2 / 8; w++) {
       uint64_t bc = 0, bl = 0, be = 0;
@@ -323,6 +328,12 @@ struct BffLang {
       bool cfirst = true;
       for (;;) {
         iters++;
+        if constexpr (kBudget) {
+          if (iters > budget) {
+            *steps = kOverBudget;
+            return 0;
+          }
+        }
         const uint32_t c = t.get(pc);
         const uint32_t v0 = t.get(h0);
         const uint32_t v1 = t.get(h1);
@@ -658,10 +669,10 @@ struct ForthLang {
 
   // Evaluate (forth.inc.h:379-397) with EvaluateOne (forth.inc.h:251-373,
   // forthcopy.cu, forthtrivial.cu, forthtrivial_reset.cu:123-252).
-  template <class T, class A>
+  template <bool kBudget = false, class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &stk,
                                       uint32_t stepcount, uint32_t *steps,
-                                      uint32_t *work = nullptr) {
+                                      uint32_t *work = nullptr, uint32_t budget = 0) {
     constexpr int kDummy = kTape2;
     int sp = 0;
     int pos = 0;
@@ -677,6 +688,12 @@ struct ForthLang {
     uint32_t lv0 = 0, lv1 = 0;
     while (i < stepcount) {
       i++;
+      if constexpr (kBudget) {
+        if (i - skipped > budget) {
+          *steps = kOverBudget;
+          return 0;
+        }
+      }
       const uint32_t c = t.get(pos);
       const uint32_t xs = stk.get(sp > 0 ? sp - 1 : kDummy);
       const uint32_t ys = stk.get(sp > 1 ? sp - 2 : kDummy);
@@ -783,13 +800,20 @@ struct ForthLang {
 struct SubleqLang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <int LANES> using Aux = LdsTape<LANES>;  // unused
-  template <class T, class A>
+  template <bool kBudget = false, class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
-                                      uint32_t *steps, uint32_t *work = nullptr) {
+                                      uint32_t *steps, uint32_t *work = nullptr,
+                                      uint32_t budget = 0) {
     int pos = 0;
     uint32_t i = 0;
     while (i < stepcount) {
       i++;
+      if constexpr (kBudget) {
+        if (i > budget) {
+          *steps = kOverBudget;
+          return 0;
+        }
+      }
       uint32_t sa = t.get(pos) % kTape2;
       uint32_t sb = t.get(pos + 1) % kTape2;
       uint8_t r = (uint8_t)(t.get(sa) - t.get(sb));
@@ -807,13 +831,20 @@ struct SubleqLang {
 struct Rsubleq4Lang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <int LANES> using Aux = LdsTape<LANES>;  // unused
-  template <class T, class A>
+  template <bool kBudget = false, class T, class A>
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
-                                      uint32_t *steps, uint32_t *work = nullptr) {
+                                      uint32_t *steps, uint32_t *work = nullptr,
+                                      uint32_t budget = 0) {
     int pos = 0;
     uint32_t i = 0;
     while (i < stepcount) {
       i++;
+      if constexpr (kBudget) {
+        if (i > budget) {
+          *steps = kOverBudget;
+          return 0;
+        }
+      }
       uint32_t sa = (uint32_t)(pos + t.get(pos)) % kTape2;
       uint32_t sb = (uint32_t)(pos + t.get(pos + 1)) % kTape2;
       uint32_t sc = (uint32_t)(pos + t.get(pos + 2)) % kTape2;

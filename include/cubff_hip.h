@@ -110,17 +110,24 @@ int cubff_run_epochs(cubff_ctx *ctx, int mode, uint64_t params_seed,
  * its two programs are done.  Results are bit-identical to the
  * epoch-by-epoch loop.  mode CUBFF_FLOW_OFF selects that loop, _ON the
  * dataflow kernel, _AUTO (the default) the dataflow kernel for the
- * languages and sizes where it measured faster.  wait_limit_s bounds any
+ * languages and sizes where it measured faster, _DEFER one epoch kernel
+ * per epoch that gives up on interactions running longer than a command
+ * budget (and on those whose programs they still owe) and hands them to a
+ * resolver kernel running concurrently on a second stream.  wait_limit_s bounds any
  * single wait inside the kernel (default 10 s), past which the run stops
  * with CUBFF_EHIP and an undefined soup.  No reference counterpart. */
 #define CUBFF_FLOW_OFF 0
 #define CUBFF_FLOW_ON 1
 #define CUBFF_FLOW_AUTO 2
+#define CUBFF_FLOW_DEFER 3  /* epoch kernels defer long interactions to a resolver */
 int cubff_set_pipeline(cubff_ctx *ctx, int mode, double wait_limit_s);
-/* Interactions run by the dataflow kernel since the context was created,
- * and the runs of queued interactions (up to 64, one wavefront) claimed to
- * run them. */
-int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims);
+/* Since the context was created: interactions run by the dataflow kernel,
+ * the runs of queued interactions (up to 64, one wavefront) claimed to run
+ * them, and interactions handed to the resolver by CUBFF_FLOW_DEFER runs. */
+int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims, uint64_t *deferred);
+/* Command budget of an interaction in CUBFF_FLOW_DEFER epoch kernels
+ * (default 768). */
+int cubff_set_defer_budget(cubff_ctx *ctx, uint32_t budget);
 
 int cubff_run_epochs_timed(cubff_ctx *ctx, int mode, uint64_t params_seed,
                            uint64_t first_epoch, uint64_t count,

@@ -14,8 +14,9 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-# The dataflow kernel is forced on (FLOW_ON) in every check below; the
-# default (FLOW_AUTO) only picks it where it measured faster.
+# The dataflow kernel (FLOW_ON) or the deferring epoch kernels (FLOW_DEFER)
+# are forced in every check below; the default (FLOW_AUTO) only picks them
+# where they measured faster.
 
 
 @pytest.fixture(scope="module")
@@ -37,12 +38,16 @@ def _pairing(capi, mode, n, seed, e):
     return O.pairing_cyclic(n, e)
 
 
-def _check_run(capi, lang, n, seed, P, mode, first, count, nthreads=0):
+def _check_run(capi, lang, n, seed, P, mode, first, count, nthreads=0, pipeline=None,
+               budget=None):
     """count epochs from `first` in one call against the oracle's loop."""
-    print("flow:", lang, n, seed, mode, first, count, flush=True)
+    pipeline = capi.FLOW_ON if pipeline is None else pipeline
+    print("flow:", lang, n, seed, mode, first, count, pipeline, budget, flush=True)
     ref = O.Soup(lang, n, seed, P, nthreads=nthreads)
     with capi.Soup(lang, n) as g:
-        g.set_pipeline(capi.FLOW_ON)
+        g.set_pipeline(pipeline)
+        if budget is not None:
+            g.set_defer_budget(budget)
         g.init_programs(O.epoch_seed(seed, 0))
         for e in range(first):  # oracle catches up; the GPU gets the soup
             ref.step(_pairing(capi, mode, n, seed, e))
@@ -59,6 +64,7 @@ def _check_run(capi, lang, n, seed, P, mode, first, count, nthreads=0):
         ops, steps = g.read_pair_ops()
         assert np.array_equal(ops, po) and np.array_equal(steps, ps)
         assert g.read_counters() == (tot, count * (n // 2))
+        return g.flow_stats()[2]
 
 
 @pytest.mark.parametrize("lang", O.LANGS)
@@ -106,3 +112,47 @@ def test_flow_wait_limit_stops_cleanly(capi):
             g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 8, P)
         g.set_pipeline(capi.FLOW_ON, 10.0)
     _check_run(capi, "bff_noheads", n, seed, P, capi.PAIR_SHUFFLE, 0, 8)
+
+
+# Deferring epoch kernels + concurrent resolver.
+
+@pytest.mark.parametrize("lang", O.LANGS)
+def test_defer_every_language(capi, lang):
+    # A budget of 16 commands defers many interactions and long chains.
+    d = _check_run(capi, lang, 2048, 10248, 1 << 18, capi.PAIR_SHUFFLE, 0, 12,
+                   pipeline=capi.FLOW_DEFER, budget=16)
+    assert d > 0
+
+
+@pytest.mark.parametrize("mode", ["shuffle", "fixed", "cyclic"])
+def test_defer_segments_and_modes(capi, mode):
+    m = {"shuffle": capi.PAIR_SHUFFLE, "fixed": capi.PAIR_FIXED, "cyclic": capi.PAIR_CYCLIC}[mode]
+    _check_run(capi, "bff_noheads", 256, 7, 1 << 20, m, 5, 150, pipeline=capi.FLOW_DEFER,
+               budget=8)
+
+
+@pytest.mark.parametrize("n", [2, 4, 6, 130])
+def test_defer_tiny_soups(capi, n):
+    for budget in (1, 64):
+        _check_run(capi, "bff_noheads", n, 3, 1 << 22, capi.PAIR_SHUFFLE, 0, 70,
+                   pipeline=capi.FLOW_DEFER, budget=budget)
+        _check_run(capi, "forth", n, 4, 1 << 22, capi.PAIR_SHUFFLE, 2, 9,
+                   pipeline=capi.FLOW_DEFER, budget=budget)
+
+
+def test_defer_equals_epoch_loop_at_2p17(capi):
+    n, seed, P = 131072, 1, 1 << 18
+    print("defer vs loop at", n, flush=True)
+    out = []
+    for pipeline in (capi.FLOW_DEFER, capi.FLOW_OFF):
+        with capi.Soup("bff_noheads", n) as g:
+            g.set_pipeline(pipeline)
+            g.init_programs(O.epoch_seed(seed, 0))
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 40, P)
+            out.append((_sha(g.read_soup()), g.read_counters(), _sha(g.read_pair_ops()[0]),
+                        _sha(g.read_pair_ops()[1])))
+            if pipeline == capi.FLOW_DEFER:
+                assert g.flow_stats()[2] > 0
+    assert out[0] == out[1]
+    _check_run(capi, "bff_noheads", n, seed, P, capi.PAIR_SHUFFLE, 0, 3, nthreads=16,
+               pipeline=capi.FLOW_DEFER)
