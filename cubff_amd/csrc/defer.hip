@@ -80,6 +80,8 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
     const uint32_t backlog = ld_agent(&a.ctl->tail) - ld_agent(&a.ctl->resolved);
     block_budget = backlog > a.max_backlog ? 0xffffffffu : a.budget;
   }
+  __syncthreads();
+  const uint32_t budget = block_budget;
   const uint64_t k = (uint64_t)blockIdx.x * kEpochLanes + lane;
   const auto t = Tape::at(lds, lane);
   const auto aux = Aux::at(lds + Tape::kRegionBytes, lane);
@@ -90,12 +92,20 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
     o1 = ld_agent(a.owner + p1);
     o2 = ld_agent(a.owner + p2);
     // An owing entry the resolver has finished (it hands the program back
-    // right after): the program is resolved.
-    if (o1 < kResolved && ld_agent(&a.items[o1].done) == a.gen) o1 = kResolved;
-    if (o2 < kResolved && ld_agent(&a.items[o2].done) == a.gen) o2 = kResolved;
+    // right after): the program is resolved.  While the resolver is behind
+    // (no budget), wait for owing entries instead of deferring more: the
+    // resolver never waits on epoch kernels, so the wait ends, and the
+    // list cannot grow without bound.
+    const bool wait = budget == 0xffffffffu;
+    if (o1 < kResolved &&
+        (ld_agent(&a.items[o1].done) == a.gen ||
+         (wait && wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks))))
+      o1 = kResolved;
+    if (o2 < kResolved &&
+        (ld_agent(&a.items[o2].done) == a.gen ||
+         (wait && wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks))))
+      o2 = kResolved;
   }
-  __syncthreads();
-  const uint32_t budget = block_budget;
   // A program last written by the resolver: see its bytes.
   if (__ballot(o1 == kResolved || o2 == kResolved))
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
