@@ -117,6 +117,7 @@ constexpr uint64_t kFlowMaxEpochs = 64;
 constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
+constexpr double kResolverIdleS = 0.1;
 
 // Which multi-epoch path the default picks (measured on MI355X at 2^17
 // programs, 64 epochs, against one launch per epoch).  Deferring epoch
@@ -257,11 +258,10 @@ int ensure_defer_scratch(cubff_ctx *c, uint64_t epochs) {
 }
 
 // Epochs [first, first + epochs) with the deferring epoch kernel on the
-// context stream and the resolver on the second stream.  With ev, records
-// ev[0]/ev[1] around the epoch kernels (their summed time is not separable
-// from the pairing launches in between, so ev brackets the whole sequence).
+// context stream and the resolver on the second stream.  With kern_ms, adds
+// the summed duration of the epoch kernels (HIP events around each).
 int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first,
-                      uint32_t epochs, uint32_t prob, hipEvent_t *ev) {
+                      uint32_t epochs, uint32_t prob, float *kern_ms) {
   int r;
   if ((r = ensure_defer_scratch(c, epochs))) return r;
   const uint64_t n = c->n;
@@ -287,12 +287,19 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.pair_steps = c->pair_steps;
   a.counters = c->counters;
   a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
+  a.idle_ticks = (uint64_t)(std::min(kResolverIdleS, c->flow_wait_s) * khz * 1000.0);
   hipEvent_t started = nullptr, resolved = nullptr;
+  std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
   auto cleanup = [&]() {
     if (started) (void)hipEventDestroy(started);
     if (resolved) (void)hipEventDestroy(resolved);
+    for (auto &x : kev)
+      if (x) (void)hipEventDestroy(x);
   };
-  hipError_t e = hipEventCreateWithFlags(&started, hipEventDisableTiming);
+  hipError_t e = hipSuccess;
+  for (auto &x : kev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&started, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&resolved, hipEventDisableTiming);
   if (e == hipSuccess) e = launch_fill_u32(c->stream, c->owner, kNone, n);
   if (e == hipSuccess) e = launch_defer_start(c->stream, c->dctl);
@@ -303,16 +310,16 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
     cleanup();
     return fail(CUBFF_EHIP, std::string("deferred run setup failed: ") + hipGetErrorString(e));
   }
-  if (ev) e = hipEventRecord(ev[0], c->stream);
   r = CUBFF_OK;
   for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
     if ((r = cubff_pair(c, mode, params_seed, first + j))) break;
-    e = launch_epoch_defer(c->stream, c->lang, a, c->shuf, j);
+    if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
+    if (e == hipSuccess) e = launch_epoch_defer(c->stream, c->lang, a, c->shuf, j);
+    if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
   }
   // Close the list whatever happened, so the resolver leaves.
   hipError_t e2 = launch_defer_close(c->stream, c->dctl);
   if (e == hipSuccess) e = e2;
-  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], c->stream);
   if (e == hipSuccess) e = hipEventRecord(resolved, c->stream2);
   if (e == hipSuccess) e = hipStreamWaitEvent(c->stream, resolved, 0);
   DeferCtl ctl{};
@@ -321,11 +328,20 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   hipError_t e3 = hipStreamSynchronize(c->stream2);
   if (e == hipSuccess) e = e3;
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && r == CUBFF_OK && kern_ms) {
+    for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
+      float ms = 0;
+      e = hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]);
+      *kern_ms += ms;
+    }
+  }
   cleanup();
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
   c->defer_items += ctl.final_tail;
-  if (ctl.abort)
+  // The resolver may leave early when it cannot run beside the epoch
+  // kernels; the run is complete as long as every deferred entry was run.
+  if (ctl.resolved != ctl.final_tail)
     return fail(CUBFF_EHIP, "deferred-interaction resolver stopped: a wait exceeded " +
                                 std::to_string(c->flow_wait_s) + " s; soup state is undefined");
   return CUBFF_OK;
@@ -367,6 +383,8 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
     if (he != hipSuccess) r = fail(CUBFF_EHIP, hipGetErrorString(he));
   }
   uint64_t launch = 0;
+  float defer_kern = 0;
+  std::vector<bool> untimed(timing ? ev.size() / 2 : 0, false);
   for (uint64_t e = first_epoch; r == CUBFF_OK && e < first_epoch + count; launch++) {
     hipEvent_t *kev = timing ? &ev[2 + 2 * launch] : nullptr;
     const uint64_t left = first_epoch + count - e;
@@ -378,7 +396,8 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
     }
     if (defer && left >= 2) {
       const uint32_t ep = (uint32_t)std::min(left, seg);
-      r = run_defer_segment(c, mode, params_seed, e, ep, prob, kev);
+      r = run_defer_segment(c, mode, params_seed, e, ep, prob, timing ? &defer_kern : nullptr);
+      if (timing) untimed[launch] = true;  // timed inside
       e += ep;
       continue;
     }
@@ -398,10 +417,12 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
     if (he == hipSuccess) he = hipEventSynchronize(ev[1]);
     if (he == hipSuccess) he = hipEventElapsedTime(&tot, ev[0], ev[1]);
     for (uint64_t i = 0; he == hipSuccess && i < launch; i++) {
+      if (untimed[i]) continue;
       float ms = 0;
       he = hipEventElapsedTime(&ms, ev[2 + 2 * i], ev[3 + 2 * i]);
       kern += ms;
     }
+    kern += defer_kern;
     if (he != hipSuccess) r = fail(CUBFF_EHIP, std::string("event timing failed: ") + hipGetErrorString(he));
     timing[0] = tot;
     timing[1] = kern;

@@ -77,8 +77,9 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
     // While the resolver is behind, run every interaction to the end here:
     // only interactions whose programs are still owed are deferred, which
     // keeps the list (and the chains hanging off it) bounded.
+    // Without a resolver (it left: see k_resolve), nothing is given up.
     const uint32_t backlog = ld_agent(&a.ctl->tail) - ld_agent(&a.ctl->resolved);
-    block_budget = backlog > a.max_backlog ? 0xffffffffu : a.budget;
+    block_budget = backlog > a.max_backlog || ld_agent(&a.ctl->abort) ? 0xffffffffu : a.budget;
   }
   __syncthreads();
   const uint32_t budget = block_budget;
@@ -200,6 +201,14 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
         if (b2 != beat) {
           beat = b2;
           t0 = wall_clock64();
+        }
+        // Nothing deferred yet and no epoch kernel finished for a while: the
+        // epoch kernels cannot start beside this kernel (dispatches are
+        // serialized, as under some profilers).  Leave; the epoch kernels
+        // then run every interaction themselves.
+        if (tl == 0 && wall_clock64() - t0 > a.idle_ticks) {
+          st_agent(&c->abort, 1);
+          break;
         }
         if (wall_clock64() - t0 > a.spin_ticks) {
           st_agent(&c->abort, 1);

@@ -82,6 +82,7 @@ struct DeferArgs {
   uint32_t *pair_ops, *pair_steps;  // written for the segment's last epoch
   unsigned long long *counters;
   uint64_t spin_ticks;
+  uint64_t idle_ticks;  // resolver: leave if nothing arrives and no epoch ends this long
 };
 
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,

@@ -156,3 +156,34 @@ def test_defer_equals_epoch_loop_at_2p17(capi):
     assert out[0] == out[1]
     _check_run(capi, "bff_noheads", n, seed, P, capi.PAIR_SHUFFLE, 0, 3, nthreads=16,
                pipeline=capi.FLOW_DEFER)
+
+
+def test_defer_with_serialized_dispatch(tmp_path):
+    # When kernels cannot run side by side (AMD_SERIALIZE_KERNEL=3, as under
+    # some profilers), the resolver leaves after a short idle wait and the
+    # epoch kernels run every interaction themselves: same results.
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    script = tmp_path / "serial.py"
+    script.write_text(
+        "import sys, hashlib\n"
+        f"sys.path.insert(0, {root!r})\n"
+        "from cubff_amd import capi\n"
+        "from oracle import oracle as O\n"
+        "n, seed, P = 2048, 5, 1 << 20\n"
+        "ref = O.Soup('bff_noheads', n, seed, P)\n"
+        "for e in range(6):\n"
+        "    ref.step()\n"
+        "with capi.Soup('bff_noheads', n) as g:\n"
+        "    g.set_pipeline(capi.FLOW_DEFER)\n"
+        "    g.set_defer_budget(8)\n"
+        "    g.init_programs(O.epoch_seed(seed, 0))\n"
+        "    g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 6, P)\n"
+        "    assert (g.read_soup() == ref.soup).all()\n"
+        "print('serialized ok', flush=True)\n")
+    env = dict(os.environ, AMD_SERIALIZE_KERNEL="3")
+    p = subprocess.run([sys.executable, str(script)], env=env, capture_output=True, timeout=110)
+    print(p.stdout.decode()[-500:], p.stderr.decode()[-2000:])
+    assert p.returncode == 0 and b"serialized ok" in p.stdout
