@@ -42,7 +42,9 @@ METRIC = ("interpreted instr/s + epochs/s, 2^17×64B bff_noheads; "
           "batched+sched vs plain")
 PEAK_LANE_OPS = 256 * 64 * 2.4e9
 PEAK_HBM = 8e12          # bytes/s, MI355X HBM3E
-BYTES_PER_SLOT = 272     # k_epoch: 2 x 64 B gathered + 2 x 64 B scattered + s[2k..2k+1] + ops, steps
+# Epoch kernel per pair slot: 2 x 64 B gathered + 2 x 64 B scattered +
+# s[2k..2k+1] + ops, steps (+ the two programs' owner words when deferring).
+BYTES_PER_SLOT = {"loop": 272, "defer": 280, "dataflow": 280}
 C_MUT, C_STEP = 32, {"bff": 8, "forth": 10, "subleq": 10, "rsubleq4": 12}
 
 
@@ -91,7 +93,7 @@ def cpu_baseline(lang: str, n: int, seed: int, prob: int, budget_s: float):
             "sample": f"oracle OpenMP port, epochs 1..{e} of the same workload"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_bff_noheads_2p17.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_defer_bff_noheads_2p17.md")
 
 
 def traffic_per_launch():
@@ -253,6 +255,9 @@ def main():
                     help="timed epochs of the 64-soup batched runs (0 = skip)")
     ap.add_argument("--config-epochs", type=int, default=16,
                     help="timed epochs of the other configurations (0 = skip)")
+    ap.add_argument("--pipeline", choices=["auto", "loop", "dataflow", "defer"], default="auto",
+                    help="multi-epoch path of the single-soup run (auto: the library's choice "
+                         "for the language)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(four launches), the reference for HBM byte counters")
@@ -276,23 +281,36 @@ def main():
     from cubff_amd import capi
     prob = 1 << 18  # round(1/4096 * 2^30), main.cc:183, :226
     n, seed, lang = args.num, args.seed, args.lang
+    modes = {"auto": capi.FLOW_AUTO, "loop": capi.FLOW_OFF, "dataflow": capi.FLOW_ON,
+             "defer": capi.FLOW_DEFER}
     with capi.Soup(lang, n) as g:
+        g.set_pipeline(modes[args.pipeline])
         g.init_programs(capi.epoch_seed(seed, 0))
         g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, args.warmup, prob)
         g.sync()
         g.reset_counters()
+        _, _, d0 = g.flow_stats()
         t0 = time.perf_counter()
         total_ms, kern_ms = g.run_epochs_timed(capi.PAIR_SHUFFLE, seed, args.warmup,
                                                args.steps, prob)
         g.sync()
         wall = time.perf_counter() - t0
         ops, runs, steps = g.read_counters(with_steps=True)
+        items, claims, d1 = g.flow_stats()
+    deferred = d1 - d0
+    path = ("defer" if deferred > 0 or args.pipeline == "defer" else
+            "dataflow" if items > 0 else "loop")
+    loop = None
+    if path != "loop":
+        # The same epochs with one launch per epoch, for comparison.
+        loop = run_soup_timed(capi, lang, n, seed, prob, args.warmup, args.steps,
+                              flow=capi.FLOW_OFF)
 
     K = args.steps
     dt = total_ms / 1e3
     work = (n // 2) * 128 * C_MUT * K + steps * c_step(lang)
     lane_ops = work / (kern_ms / 1e3)
-    algo_bytes = (n // 2) * BYTES_PER_SLOT
+    algo_bytes = (n // 2) * BYTES_PER_SLOT[path]
     achieved = algo_bytes / (kern_ms / 1e3 / K)
     line = {
         "metric": METRIC,
@@ -313,6 +331,10 @@ def main():
                          f"epochs {args.warmup}..{args.warmup + K - 1}"),
             "lang": lang, "num_programs": n, "seed": seed,
             "parallelism": "single soup, one lane per pair slot",
+            "path": {"loop": "one launch per epoch (k_epoch)",
+                     "defer": "k_epoch_defer per epoch (command budget 768) + resolver "
+                              "kernel on a second stream",
+                     "dataflow": "multi-epoch dataflow kernel (k_flow)"}[path],
         },
         "epochs_per_s": K / dt,
         "interp_steps_per_s": steps / dt,
@@ -325,7 +347,10 @@ def main():
             "unit": "T int32 lane-ops/s",
             "frac": lane_ops / PEAK_LANE_OPS,
             "traffic": traffic_per_launch(),
-            "kernel": "k_epoch (gather + mutate + evaluate + scatter), one launch per epoch",
+            "kernel": {"loop": "k_epoch (gather + mutate + evaluate + scatter)",
+                       "defer": "k_epoch_defer (gather + mutate + evaluate up to the budget "
+                                "+ scatter)",
+                       "dataflow": "k_flow"}[path],
             "kernel_ms_avg": kern_ms / K,
             "work_per_launch": work / K,
             "model": "W = (N/2)*128*32 + sum(steps)*C_step",
@@ -335,6 +360,8 @@ def main():
                     "algorithmic_bytes_per_launch": algo_bytes},
         },
         "cpu_baseline": None,
+        "deferred_per_epoch": deferred / K,
+        "loop_one_launch_per_epoch": loop,
     }
     if args.batched_epochs > 0:
         plain = run_batched(capi, capi.SCHED_PLAIN, args.batched_epochs, 8, autotune=True)
