@@ -44,6 +44,8 @@ struct cubff_ctx {
   // Deferred-interaction path (defer.hip): second stream, program owners,
   // the list (defer_cap entries), run generation, command budget.
   hipStream_t stream2 = nullptr;
+  hipStream_t stream3 = nullptr;  // pairings drawn ahead
+  uint32_t *shuf2 = nullptr;      // second pairing buffer
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
   DeferCtl *dctl = nullptr;
@@ -118,6 +120,7 @@ constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
 constexpr double kResolverIdleS = 0.1;
+constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
 
 // Which multi-epoch path the default picks (measured on MI355X at 2^17
 // programs, 64 epochs, against one launch per epoch).  Deferring epoch
@@ -171,20 +174,129 @@ int ensure_flow_scratch(cubff_ctx *c, uint64_t epochs) {
   return CUBFF_OK;
 }
 
-int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint32_t *dst) {
+int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint32_t *dst,
+              hipStream_t st = nullptr) {
+  if (!st) st = c->stream;
   switch (mode) {
     case CUBFF_PAIR_SHUFFLE:
-      HIP_TRY(launch_pair_shuffle(c->stream, c->fy, dst, c->n, params_seed, epoch));
+      HIP_TRY(launch_pair_shuffle(st, c->fy, dst, c->n, params_seed, epoch));
       return CUBFF_OK;
     case CUBFF_PAIR_FIXED:
-      HIP_TRY(launch_pair_fixed(c->stream, dst, c->n, epoch));
+      HIP_TRY(launch_pair_fixed(st, dst, c->n, epoch));
       return CUBFF_OK;
     case CUBFF_PAIR_CYCLIC:
-      HIP_TRY(launch_pair_cyclic(c->stream, dst, c->n, epoch));
+      HIP_TRY(launch_pair_cyclic(st, dst, c->n, epoch));
       return CUBFF_OK;
     default:
       return fail(CUBFF_EINVAL, "pairing mode without a closed form or shuffle");
   }
+}
+
+// Pairings drawn one epoch ahead on a third stream into two buffers (the
+// context's shuf and shuf2), so that drawing epoch j+1's pairing overlaps
+// epoch j's kernel: the pairings do not depend on the soup.  Epoch j's
+// kernel waits for its pairing; the pairing of epoch j+2 waits for epoch j's
+// kernel, which reads the same buffer.
+struct PairAhead {
+  cubff_ctx *c;
+  int mode;
+  uint64_t seed, first;
+  uint32_t *buf[2] = {nullptr, nullptr};
+  hipEvent_t drawn[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
+
+  int init() {
+    int r;
+    if (!c->stream3) {
+      hipError_t e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking);
+      if (e != hipSuccess)
+        return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+    }
+    if (!c->shuf2 && (r = dev_alloc(&c->shuf2, c->n))) return r;
+    buf[0] = c->shuf;
+    buf[1] = c->shuf2;
+    for (int i = 0; i < 2; i++) {
+      HIP_TRY(hipEventCreateWithFlags(&drawn[i], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&used[i], hipEventDisableTiming));
+    }
+    // The stream of the first pairing must see everything queued so far.
+    HIP_TRY(hipEventRecord(used[1], c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream3, used[1], 0));
+    return CUBFF_OK;
+  }
+  int draw(uint32_t j) {
+    int r;
+    if (j >= 2) HIP_TRY(hipStreamWaitEvent(c->stream3, used[j % 2], 0));
+    if ((r = pair_into(c, mode, seed, first + j, buf[j % 2], c->stream3))) return r;
+    HIP_TRY(hipEventRecord(drawn[j % 2], c->stream3));
+    return CUBFF_OK;
+  }
+  int before_kernel(uint32_t j) {
+    HIP_TRY(hipStreamWaitEvent(c->stream, drawn[j % 2], 0));
+    return CUBFF_OK;
+  }
+  int after_kernel(uint32_t j) {
+    HIP_TRY(hipEventRecord(used[j % 2], c->stream));
+    return CUBFF_OK;
+  }
+  // The last epoch's pairing becomes the context's current one.
+  void finish(uint32_t epochs) {
+    if (epochs && (epochs - 1) % 2 == 1) std::swap(c->shuf, c->shuf2);
+  }
+  ~PairAhead() {
+    for (int i = 0; i < 2; i++) {
+      if (drawn[i]) (void)hipEventDestroy(drawn[i]);
+      if (used[i]) (void)hipEventDestroy(used[i]);
+    }
+  }
+};
+
+// Epochs [first, first + epochs), one launch each, pairings drawn ahead.
+int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first,
+                     uint32_t epochs, uint32_t prob, float *kern_ms) {
+  int r;
+  PairAhead pa{c, mode, params_seed, first};
+  if ((r = pa.init())) return r;
+  std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
+  auto cleanup = [&]() {
+    for (auto &x : kev)
+      if (x) (void)hipEventDestroy(x);
+  };
+  hipError_t e = hipSuccess;
+  for (auto &x : kev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = launch_set_u64(c->stream, c->num_indices, c->n / 2);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(CUBFF_EHIP, std::string("epoch setup failed: ") + hipGetErrorString(e));
+  }
+  const uint64_t sm = splitmix64(params_seed);
+  r = pa.draw(0);
+  for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
+    if (j + 1 < epochs && (r = pa.draw(j + 1))) break;
+    if ((r = pa.before_kernel(j))) break;
+    if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
+    if (e == hipSuccess)
+      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % 2], c->n, c->num_indices,
+                       epoch_seed_pre(sm, first + j), prob, c->pair_ops, c->pair_steps,
+                       c->counters);
+    if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
+    if (e == hipSuccess) r = pa.after_kernel(j);
+  }
+  hipError_t e2 = hipStreamSynchronize(c->stream3);
+  if (e == hipSuccess) e = e2;
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && r == CUBFF_OK && kern_ms) {
+    for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
+      float ms = 0;
+      e = hipEventElapsedTime(&ms, kev[2 * j], kev[2 * j + 1]);
+      *kern_ms += ms;
+    }
+  }
+  cleanup();
+  if (r) return r;
+  if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("epoch run failed: ") + hipGetErrorString(e));
+  pa.finish(epochs);
+  return CUBFF_OK;
 }
 
 // Epochs [first, first + epochs) as one dataflow launch; leaves the last
@@ -310,12 +422,16 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
     cleanup();
     return fail(CUBFF_EHIP, std::string("deferred run setup failed: ") + hipGetErrorString(e));
   }
-  r = CUBFF_OK;
-  for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
-    if ((r = cubff_pair(c, mode, params_seed, first + j))) break;
+  PairAhead pa{c, mode, params_seed, first};
+  r = pa.init();
+  if (r == CUBFF_OK) r = pa.draw(0);
+  for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
+    if (j + 1 < epochs && (r = pa.draw(j + 1))) break;
+    if ((r = pa.before_kernel(j))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
-    if (e == hipSuccess) e = launch_epoch_defer(c->stream, c->lang, a, c->shuf, j);
+    if (e == hipSuccess) e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % 2], j);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
+    if (e == hipSuccess) r = pa.after_kernel(j);
   }
   // Close the list whatever happened, so the resolver leaves.
   hipError_t e2 = launch_defer_close(c->stream, c->dctl);
@@ -327,6 +443,10 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
     e = hipMemcpyAsync(&ctl, c->dctl, sizeof(ctl), hipMemcpyDeviceToHost, c->stream);
   hipError_t e3 = hipStreamSynchronize(c->stream2);
   if (e == hipSuccess) e = e3;
+  if (c->stream3) {
+    e3 = hipStreamSynchronize(c->stream3);
+    if (e == hipSuccess) e = e3;
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess && r == CUBFF_OK && kern_ms) {
     for (uint32_t j = 0; e == hipSuccess && j < epochs; j++) {
@@ -338,6 +458,8 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   cleanup();
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
+  pa.finish(epochs);
+  HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
   c->defer_items += ctl.final_tail;
   // The resolver may leave early when it cannot run beside the epoch
   // kernels; the run is complete as long as every deferred entry was run.
@@ -370,7 +492,7 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
   };
   int r = CUBFF_OK;
   if (timing) {
-    const uint64_t launches = (flow || defer) ? (count + seg - 1) / seg : count;
+    const uint64_t launches = (flow || defer) ? (count + seg - 1) / seg : count;  // upper bound
     ev.resize(2 + 2 * launches);
     for (auto &e : ev) {
       hipError_t he = hipEventCreate(&e);
@@ -397,6 +519,13 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
     if (defer && left >= 2) {
       const uint32_t ep = (uint32_t)std::min(left, seg);
       r = run_defer_segment(c, mode, params_seed, e, ep, prob, timing ? &defer_kern : nullptr);
+      if (timing) untimed[launch] = true;  // timed inside
+      e += ep;
+      continue;
+    }
+    if (!flow && !defer && closed_form && left >= 2) {
+      const uint32_t ep = (uint32_t)std::min<uint64_t>(left, kLoopSegmentEpochs);
+      r = run_loop_segment(c, mode, params_seed, e, ep, prob, timing ? &defer_kern : nullptr);
       if (timing) untimed[launch] = true;  // timed inside
       e += ep;
       continue;
@@ -550,6 +679,9 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   dev_free(c->ditems);
   dev_free(c->dctl);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->stream3) (void)hipStreamSynchronize(c->stream3);
+  if (c->stream3) (void)hipStreamDestroy(c->stream3);
+  dev_free(c->shuf2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUBFF_OK;
