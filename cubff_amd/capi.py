@@ -26,7 +26,7 @@ EXPORTS = [
     "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats", "cubff_set_defer_budget",
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
-    "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_sync",
+    "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_trace_bff", "cubff_sync",
     "cubff_splitmix64", "cubff_epoch_seed",
     "cubff_batch_last_error", "cubff_batch_create", "cubff_batch_destroy",
     "cubff_batch_init", "cubff_batch_set_launch", "cubff_batch_get_launch",
@@ -94,6 +94,7 @@ def load():
         "cubff_read_shuffle": (_i, [_vp, _u32p, _u64p]),
         "cubff_selfrep": (_i, [_vp, _u64, _u64, _u64p]),
         "cubff_eval_tapes": (_i, [_i, _i, _u8p, _u64, _u64, _u32p, _u32p]),
+        "cubff_trace_bff": (_i, [_i, _i, _u8p, _u64, _u8p, _u64, _u64p, _u64p]),
         "cubff_eval_selfrep": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p]),
         "cubff_sync": (_i, [_vp]),
         "cubff_splitmix64": (_u64, [_u64]),
@@ -182,6 +183,18 @@ def eval_selfrep(lang: str, prog: bytes | np.ndarray, epoch: int, seed: int,
     _check(load().cubff_eval_selfrep(lang_id(lang), device, _ptr(p, _u8p), p.size,
                                      epoch, seed, ctypes.byref(out)))
     return int(out.value)
+
+
+def trace_bff(lang: str, tape: bytes | np.ndarray, steps: int = 8192, device: int = 0):
+    """Step trace of one BFF-family tape on the GPU: (file image bytes, ops)."""
+    t = np.frombuffer(bytes(tape), dtype=np.uint8).copy()
+    if t.size != 128:
+        raise ValueError("tape must be exactly 128 bytes long")
+    out = np.zeros(12 + 140 * steps, dtype=np.uint8)
+    nbytes, ops = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(load().cubff_trace_bff(lang_id(lang), device, _ptr(t, _u8p), steps, _ptr(out, _u8p),
+                                  out.size, ctypes.byref(nbytes), ctypes.byref(ops)))
+    return out[:nbytes.value].tobytes(), int(ops.value)
 
 
 class Soup:

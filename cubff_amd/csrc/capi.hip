@@ -930,6 +930,44 @@ int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
   return CUBFF_OK;
 }
 
+int cubff_trace_bff(int lang, int device, const uint8_t *tape, uint64_t steps, uint8_t *out,
+                    uint64_t out_bytes, uint64_t *file_bytes, uint64_t *ops) {
+  if (lang < kBff || lang > kBffSelfmove) return fail(CUBFF_EINVAL, "step traces cover the BFF family");
+  if (!tape || !out) return fail(CUBFF_EINVAL, "null tape or output");
+  if (steps > 0xffffffffULL) return fail(CUBFF_EINVAL, "steps must fit in 32 bits");
+  const uint64_t rec = 12 + kTape2;
+  if (out_bytes < 12 + rec * steps)
+    return fail(CUBFF_EINVAL, "output must hold 12 + 140 * steps bytes");
+  int r;
+  if ((r = set_device(device))) return r;
+  uint8_t *d_t = nullptr, *d_o = nullptr;
+  uint64_t *d_m = nullptr;
+  if ((r = dev_alloc(&d_t, kTape2)) || (r = dev_alloc(&d_o, rec * std::max<uint64_t>(steps, 1))) ||
+      (r = dev_alloc(&d_m, 3))) {
+    dev_free(d_t);
+    dev_free(d_o);
+    return r;
+  }
+  uint64_t meta[3] = {0, 0, 0};
+  hipError_t e = hipMemcpy(d_t, tape, kTape2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_trace_bff(nullptr, lang, d_t, (uint32_t)steps, d_o, steps, d_m);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(meta, d_m, sizeof(meta), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && meta[0] > steps) e = hipErrorUnknown;
+  if (e == hipSuccess && meta[0]) e = hipMemcpy(out + 12, d_o, rec * meta[0], hipMemcpyDeviceToHost);
+  dev_free(d_t);
+  dev_free(d_o);
+  dev_free(d_m);
+  if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("trace failed: ") + hipGetErrorString(e));
+  // Header: magic "BFF\0", format version 1, tape size (u32 little-endian).
+  const uint32_t head[3] = {0x00464642u, 1u, (uint32_t)kTape2};
+  for (int q = 0; q < 3; q++)
+    for (int b = 0; b < 4; b++) out[4 * q + b] = (uint8_t)(head[q] >> (8 * b));
+  if (file_bytes) *file_bytes = 12 + rec * meta[0];
+  if (ops) *ops = meta[1];
+  return CUBFF_OK;
+}
+
 int cubff_eval_selfrep(int lang, int device, const uint8_t *prog, uint64_t len,
                        uint64_t epoch, uint64_t seed, uint64_t *result) {
   if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");

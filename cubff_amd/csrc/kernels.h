@@ -116,6 +116,8 @@ hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,
 hipError_t launch_flow_pairof(hipStream_t st, const uint32_t *shuf, uint32_t *pairof,
                               uint64_t n, uint32_t epochs);
 hipError_t launch_flow(hipStream_t st, int lang, const FlowArgs &a);
+hipError_t launch_trace_bff(hipStream_t st, int lang, const uint8_t *tape, uint32_t steps,
+                            uint8_t *out, uint64_t cap, uint64_t *meta);
 hipError_t launch_defer_start(hipStream_t st, DeferCtl *c);
 hipError_t launch_defer_close(hipStream_t st, DeferCtl *c);
 hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,

@@ -166,6 +166,15 @@ int cubff_selfrep(cubff_ctx *ctx, uint64_t seed_value, uint64_t count,
 int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
                      uint64_t steps, uint32_t *ops, uint32_t *steps_out);
 
+/* Step trace of one BFF-family tape for the reference's visualiser: the
+ * file image python/bff_interpreter.py:evaluate_and_save (:327-393) writes
+ * -- "BFF\0", u32 version 1, u32 tape size 128, then before every step u32
+ * pc, head0, head1 and the 128 tape bytes (little-endian).  out must hold
+ * 12 + 140 * steps bytes; *file_bytes receives the image's length and *ops
+ * the Evaluate result (bff.inc.h:356-389). */
+int cubff_trace_bff(int lang, int device, const uint8_t *tape, uint64_t steps, uint8_t *out,
+                    uint64_t out_bytes, uint64_t *file_bytes, uint64_t *ops);
+
 /* Single-program self-replication check of EvalParsedSelfrep
  * (common_language.h:326-342): prog is zero-padded to 64 bytes and scored
  * with seed_value = S(epoch) computed from `seed`. */

@@ -199,6 +199,46 @@ static int bff_step(const BffVariant *v, uint8_t *tape, int *h0, int *h1,
   }
 }
 
+/* Step trace of the BFF family: before every step, pc, both heads (masked)
+ * and the tape, 140 bytes per state (u32 little-endian fields), the layout
+ * of python/bff_interpreter.py:evaluate_and_save (:327-393) after its 12-byte
+ * header.  Same loop as bff_evaluate below; returns ops, *states = records
+ * written (at most cap). */
+uint64_t orc_trace_bff(int lang, uint8_t *tape, uint64_t stepcount, uint8_t *out,
+                       uint64_t cap, uint64_t *states, uint64_t *steps_out) {
+  BffVariant v = bff_variant(lang);
+  int h0, h1, pc;
+  if (v.heads) {
+    h0 = tape[0] % TAPE2;
+    h1 = tape[1] % TAPE2;
+    pc = 2;
+  } else {
+    h0 = TAPE2;
+    h1 = TAPE2;
+    pc = 0;
+  }
+  uint64_t i = 0, nskip = 0, rec = 0;
+  for (; i < stepcount; i++) {
+    h0 &= TAPE2 - 1;
+    h1 &= TAPE2 - 1;
+    if (rec < cap) {
+      uint8_t *o = out + rec * (12 + TAPE2);
+      const uint32_t f[3] = {(uint32_t)pc, (uint32_t)h0, (uint32_t)h1};
+      for (int q = 0; q < 3; q++)
+        for (int b = 0; b < 4; b++) o[4 * q + b] = (uint8_t)(f[q] >> (8 * b));
+      memcpy(o + 12, tape, TAPE2);
+      rec++;
+    }
+    if (!bff_step(&v, tape, &h0, &h1, &pc)) nskip++;
+    if (pc < 0) { i++; break; }
+    pc++;
+    if (pc >= TAPE2) { i++; break; }
+  }
+  if (states) *states = rec;
+  if (steps_out) *steps_out = i;
+  return i - nskip;
+}
+
 /* Evaluate: bff.inc.h:356-389, InitialState :269-283. */
 static uint64_t bff_evaluate(int lang, uint8_t *tape, uint64_t stepcount,
                              uint64_t *steps_out) {

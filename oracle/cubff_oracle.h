@@ -49,6 +49,8 @@ uint64_t orc_epoch_seed(uint64_t params_seed, uint64_t x);
 /* Runs one interaction in place; returns ops (steps minus no-op steps).
  * If steps_out is non-null, the loop-iteration count (steps incl. no-ops) is
  * stored there. */
+uint64_t orc_trace_bff(int lang, uint8_t *tape, uint64_t stepcount, uint8_t *out,
+                       uint64_t cap, uint64_t *states, uint64_t *steps_out);
 uint64_t orc_evaluate(int lang, uint8_t *tape, uint64_t stepcount,
                       uint64_t *steps_out);
 

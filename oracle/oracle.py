@@ -48,6 +48,8 @@ def lib():
         L.orc_lang_id.argtypes = [ctypes.c_char_p]
         L.orc_evaluate.restype = u64
         L.orc_evaluate.argtypes = [i32, _u8p, u64, _u64p]
+        L.orc_trace_bff.restype = u64
+        L.orc_trace_bff.argtypes = [i32, _u8p, u64, _u8p, u64, _u64p, _u64p]
         L.orc_init_programs.restype = None
         L.orc_init_programs.argtypes = [_u8p, u64, u64, i32]
         L.orc_pairing_shuffle.restype = None
@@ -93,6 +95,19 @@ def evaluate(lang: str, tape: np.ndarray, steps: int = 8192):
     st = ctypes.c_uint64(0)
     ops = lib().orc_evaluate(lang_id(lang), _p(t, _u8p), steps, ctypes.byref(st))
     return t, int(ops), int(st.value)
+
+
+def trace_bff(lang: str, tape: np.ndarray, steps: int = 8192):
+    """Step trace of a BFF-family tape: (states uint8[k, 140], ops, steps),
+    each state = u32 pc, head0, head1 (little-endian) + the 128-byte tape
+    before the step."""
+    t = np.ascontiguousarray(tape, dtype=np.uint8).copy()
+    assert t.size == 128
+    out = np.zeros((max(steps, 1), 140), dtype=np.uint8)
+    k, st = ctypes.c_uint64(0), ctypes.c_uint64(0)
+    ops = lib().orc_trace_bff(lang_id(lang), _p(t, _u8p), steps, _p(out, _u8p), steps,
+                              ctypes.byref(k), ctypes.byref(st))
+    return out[:k.value].copy(), int(ops), int(st.value)
 
 
 def init_programs(n: int, seed_value: int, zero_init: bool = False) -> np.ndarray:
