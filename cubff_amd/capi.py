@@ -24,6 +24,7 @@ EXPORTS = [
     "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
     "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
     "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats", "cubff_set_defer_budget",
+    "cubff_byte_histogram",
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_trace_bff", "cubff_sync",
@@ -88,6 +89,7 @@ def load():
         "cubff_set_pipeline": (_i, [_vp, _i, ctypes.c_double]),
         "cubff_flow_stats": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_set_defer_budget": (_i, [_vp, ctypes.c_uint32]),
+        "cubff_byte_histogram": (_i, [_vp, _u64p]),
         "cubff_read_counters": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_reset_counters": (_i, [_vp]),
         "cubff_read_pair_ops": (_i, [_vp, _u32p, _u32p, _u64]),
@@ -273,6 +275,11 @@ class Soup:
         _check(load().cubff_flow_stats(self._h, ctypes.byref(a), ctypes.byref(b),
                                        ctypes.byref(c)))
         return int(a.value), int(b.value), int(c.value)
+
+    def byte_histogram(self) -> np.ndarray:
+        c = np.zeros(256, dtype=np.uint64)
+        _check(load().cubff_byte_histogram(self._h, _ptr(c, _u64p)))
+        return c
 
     def set_defer_budget(self, budget: int):
         _check(load().cubff_set_defer_budget(self._h, budget))

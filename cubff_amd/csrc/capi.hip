@@ -46,6 +46,7 @@ struct cubff_ctx {
   hipStream_t stream2 = nullptr;
   hipStream_t stream3 = nullptr;  // pairings drawn ahead
   uint32_t *shuf2 = nullptr;      // second pairing buffer
+  unsigned long long *hist = nullptr;  // byte histogram, 256 entries
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
   DeferCtl *dctl = nullptr;
@@ -682,6 +683,7 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   if (c->stream3) (void)hipStreamSynchronize(c->stream3);
   if (c->stream3) (void)hipStreamDestroy(c->stream3);
   dev_free(c->shuf2);
+  dev_free(c->hist);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CUBFF_OK;
@@ -812,6 +814,18 @@ int cubff_run_epochs_timed(cubff_ctx *c, int mode, uint64_t params_seed,
   if ((r = run_epochs_impl(c, mode, params_seed, first_epoch, count, prob, t))) return r;
   if (total_ms) *total_ms = t[0];
   if (epoch_kernel_ms) *epoch_kernel_ms = t[1];
+  return CUBFF_OK;
+}
+
+int cubff_byte_histogram(cubff_ctx *c, uint64_t *counts) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (!counts) return fail(CUBFF_EINVAL, "null output");
+  if (!c->hist && (r = dev_alloc(&c->hist, 256))) return r;
+  HIP_TRY(launch_fill_u32(c->stream, reinterpret_cast<uint32_t *>(c->hist), 0, 512));
+  HIP_TRY(launch_byte_hist(c->stream, c->soup, c->n * kTape, c->hist));
+  HIP_TRY(hipMemcpyAsync(counts, c->hist, 256 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CUBFF_OK;
 }
 

@@ -98,6 +98,8 @@ hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
 hipError_t launch_eval_tapes(hipStream_t st, int lang, uint8_t *tapes,
                              uint64_t count, uint32_t steps, uint32_t *ops,
                              uint32_t *steps_out);
+hipError_t launch_byte_hist(hipStream_t st, const uint8_t *soup, uint64_t bytes,
+                            unsigned long long *counts);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v,
                            uint64_t count);
 hipError_t launch_set_u64(hipStream_t st, uint64_t *p, uint64_t v);

@@ -7,6 +7,8 @@
 //   pairing kernels   host pairing block      common_language.h:418-486
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 #include "languages.h"
@@ -445,6 +447,25 @@ __global__ void k_g2_emit_rest(const uint8_t *__restrict__ used,
 }
 
 // ---------------------------------------------------------------------------
+// Byte histogram of the soup (the h0 and frequent-byte metrics,
+// common_language.h:519-540): per-block LDS counters, then 256 global adds.
+__global__ __launch_bounds__(256) void k_byte_hist(const uint4 *__restrict__ soup, uint64_t units,
+                                                   unsigned long long *__restrict__ counts) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < units;
+       u += (uint64_t)gridDim.x * blockDim.x) {
+    const uint4 v = soup[u];
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int b = 0; b < 16; b++) atomicAdd(&h[(w[b >> 2] >> (8 * (b & 3))) & 255u], 1u);
+  }
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&counts[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
 // Host-side launch helpers (declared in kernels.h).
 
 namespace {
@@ -472,6 +493,16 @@ inline unsigned blocks_for(uint64_t n, unsigned b) {
   }                                                                       \
   return hipGetLastError();
 }  // namespace
+
+hipError_t launch_byte_hist(hipStream_t st, const uint8_t *soup, uint64_t bytes,
+                            unsigned long long *counts) {
+  const uint64_t units = bytes / 16;  // the soup is 64 N bytes
+  if (units == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<uint64_t>(blocks_for(units, 256), 2048);
+  hipLaunchKernelGGL(k_byte_hist, dim3(grid), dim3(256), 0, st,
+                     reinterpret_cast<const uint4 *>(soup), units, counts);
+  return hipGetLastError();
+}
 
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
                        uint64_t seed_value, uint64_t count, int zero_init) {

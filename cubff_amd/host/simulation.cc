@@ -18,6 +18,7 @@
 #include <cmath>
 #include <map>
 #include <memory>
+#include <thread>
 
 #include "../../include/cubff_hip.h"
 #include "brotli/encode.h"
@@ -189,91 +190,130 @@ void HipLanguage::RunSimulation(const SimulationParams &params,
   // programs (common_language.h:565, :592).
   const size_t launched = num_programs / 32 * 32;
 
+  // Epochs run in batches up to the next callback or reset epoch
+  // (cubff_run_epochs picks the fastest multi-epoch path).  At a callback
+  // epoch the soup, pairing, counters, byte histogram (on the GPU) and
+  // self-rep scores are taken first; then the next batch of epochs starts
+  // on a worker thread while this thread compresses the soup, fills the
+  // state and runs the callback, so brotli overlaps the GPU.  The reset
+  // re-init is queued before that batch, as the reference applies it after
+  // the callback and before the next epoch.  If the callback stops the run,
+  // the batch already under way is finished and dropped.
+  CHECK(params.callback_interval > 0);
+  const size_t interval = params.callback_interval;
+  const bool resets = params.reset_interval.has_value() && *params.reset_interval > 0;
+  auto next_event = [&](size_t e) {
+    size_t n = (e + interval - 1) / interval * interval;
+    if (resets) {
+      const size_t r = *params.reset_interval;
+      n = std::min(n, (e + r - 1) / r * r);
+    }
+    return n;
+  };
+  auto run_through = [&](size_t from, size_t to) {  // epochs from..to
+    hip_check(cubff_run_epochs(ctx, mode, params.seed, from, to - from + 1, params.mutation_prob),
+              "cubff_run_epochs");
+  };
+
   auto start = std::chrono::high_resolution_clock::now();
   const auto simulation_start = start;
-  for (;; epoch++) {
-    hip_check(cubff_pair(ctx, mode, params.seed, epoch), "cubff_pair");
-    hip_check(cubff_run_epoch(ctx, seed(epoch), params.mutation_prob), "cubff_run_epoch");
-
-    if (epoch % params.callback_interval == 0) {  // common_language.h:496-588
-      const auto stop = std::chrono::high_resolution_clock::now();
-      uint64_t insn = 0, num_runs = 0, nidx = 0;
+  // The first batch starts at epoch 0, or at the epoch a checkpoint stored.
+  const size_t first_epoch = epoch;
+  epoch = next_event(first_epoch);
+  run_through(first_epoch, epoch);
+  for (;;) {
+    const bool at_callback = epoch % interval == 0;
+    const bool at_reset = resets && epoch % *params.reset_interval == 0;
+    uint64_t insn = 0, num_runs = 0, nidx = 0;
+    uint64_t counts[256] = {};
+    const size_t reset_index_here = reset_index;
+    std::chrono::high_resolution_clock::time_point stop;
+    if (at_callback) {  // common_language.h:496-588
+      stop = std::chrono::high_resolution_clock::now();
       hip_check(cubff_read_counters(ctx, &insn, &num_runs, nullptr), "cubff_read_counters");
       total_ops += insn;
       hip_check(cubff_read_soup(ctx, 0, state.soup.data(), state.soup.size()), "cubff_read_soup");
       hip_check(cubff_read_shuffle(ctx, state.shuffle_idx.data(), &nidx), "cubff_read_shuffle");
-      size_t brotli_size = brotlified.size();
-      BrotliEncoderCompress(2, 24, BROTLI_MODE_GENERIC, state.soup.size(), state.soup.data(),
-                            &brotli_size, brotlified.data());
-      const float elapsed_s =
-          std::chrono::duration_cast<std::chrono::microseconds>(stop - start).count() * 1e-6;
-      const float mops_s = insn * 1.0 / elapsed_s * 1e-6;
-      const float sim_elapsed_s =
-          std::chrono::duration_cast<std::chrono::microseconds>(stop - simulation_start)
-              .count() *
-          1e-6;
-
-      size_t counts[256] = {};
-      for (uint8_t c : state.soup) counts[c]++;
-      std::vector<uint8_t> sorted(256);
-      double h0 = 0;
-      for (size_t i = 0; i < 256; i++) {
-        sorted[i] = (uint8_t)i;
-        const double frac = counts[i] * 1.0 / state.soup.size();
-        h0 -= counts[i] ? frac * std::log2(frac) : 0.0;
-      }
-      std::sort(sorted.begin(), sorted.end(), [&](uint8_t a, uint8_t b) {
-        return std::make_pair(counts[b], b) < std::make_pair(counts[a], a);
-      });
-      const double brotli_bpb = brotli_size * 8.0 / (num_programs * kSingleTapeSize);
-
-      state.elapsed_s = sim_elapsed_s;
-      state.total_ops = total_ops;
-      state.mops_s = mops_s;
-      state.epoch = epoch + 1;
-      state.ops_per_run = insn * 1.0 / num_runs;
-      state.brotli_size = brotli_size;
-      state.brotli_bpb = brotli_bpb;
-      state.bytes_per_prog = brotli_size * 1.0 / num_programs;
-      state.h0 = h0;
-      state.higher_entropy = h0 - brotli_bpb;
-      for (size_t i = 0; i < state.frequent_bytes.size(); i++) {
-        const uint8_t c = sorted[i];
-        char chmem[32];
-        state.frequent_bytes[i].first = p_->MapChar(c, chmem);
-        state.frequent_bytes[i].second = counts[c] * 1.0 / state.soup.size();
-      }
-      for (size_t i = 0; i < state.uncommon_bytes.size(); i++) {
-        const uint8_t c = sorted[256 - state.uncommon_bytes.size() + i];
-        char chmem[32];
-        state.uncommon_bytes[i].first = p_->MapChar(c, chmem);
-        state.uncommon_bytes[i].second = counts[c] * 1.0 / state.soup.size();
-      }
+      hip_check(cubff_byte_histogram(ctx, counts), "cubff_byte_histogram");
       if (params.eval_selfrep) {
         hip_check(cubff_selfrep(ctx, seed(epoch), launched, selfrep.data()), "cubff_selfrep");
         for (size_t i = 0; i < launched; i++) state.replication_per_prog[i] = selfrep[i];
       }
-      if (params.save_to.has_value() && (epoch % params.save_interval == 0)) {
-        std::vector<char> path(params.save_to->size() + 20);
-        snprintf(path.data(), path.size(), "%s/%010zu.dat", params.save_to->c_str(), epoch);
-        FILE *f = CheckFopen(path.data(), "w");
-        const size_t epoch_to_save = epoch + 1;
-        fwrite(&reset_index, sizeof(reset_index), 1, f);
-        fwrite(&num_programs, sizeof(num_programs), 1, f);
-        fwrite(&epoch_to_save, sizeof(epoch_to_save), 1, f);
-        fwrite(state.soup.data(), 1, state.soup.size(), f);
-        fclose(f);
-      }
-      if (callback(state)) break;
-      start = std::chrono::high_resolution_clock::now();
       hip_check(cubff_reset_counters(ctx), "cubff_reset_counters");
     }
-
-    if (params.reset_interval.has_value() && epoch % *params.reset_interval == 0) {
+    if (at_reset) {  // common_language.h:590-595
       hip_check(cubff_init_programs(ctx, seed(reset_index), params.zero_init, launched),
                 "cubff_init_programs");
       reset_index++;
     }
+    const size_t next = next_event(epoch + 1);
+    if (!at_callback) {
+      run_through(epoch + 1, next);
+      epoch = next;
+      continue;
+    }
+    const auto batch_start = std::chrono::high_resolution_clock::now();
+    std::thread batch([&run_through, from = epoch + 1, to = next] { run_through(from, to); });
+
+    size_t brotli_size = brotlified.size();
+    BrotliEncoderCompress(2, 24, BROTLI_MODE_GENERIC, state.soup.size(), state.soup.data(),
+                          &brotli_size, brotlified.data());
+    const float elapsed_s =
+        std::chrono::duration_cast<std::chrono::microseconds>(stop - start).count() * 1e-6;
+    const float mops_s = insn * 1.0 / elapsed_s * 1e-6;
+    const float sim_elapsed_s =
+        std::chrono::duration_cast<std::chrono::microseconds>(stop - simulation_start).count() *
+        1e-6;
+    std::vector<uint8_t> sorted(256);
+    double h0 = 0;
+    for (size_t i = 0; i < 256; i++) {
+      sorted[i] = (uint8_t)i;
+      const double frac = counts[i] * 1.0 / state.soup.size();
+      h0 -= counts[i] ? frac * std::log2(frac) : 0.0;
+    }
+    std::sort(sorted.begin(), sorted.end(), [&](uint8_t a, uint8_t b) {
+      return std::make_pair(counts[b], b) < std::make_pair(counts[a], a);
+    });
+    const double brotli_bpb = brotli_size * 8.0 / (num_programs * kSingleTapeSize);
+
+    state.elapsed_s = sim_elapsed_s;
+    state.total_ops = total_ops;
+    state.mops_s = mops_s;
+    state.epoch = epoch + 1;
+    state.ops_per_run = insn * 1.0 / num_runs;
+    state.brotli_size = brotli_size;
+    state.brotli_bpb = brotli_bpb;
+    state.bytes_per_prog = brotli_size * 1.0 / num_programs;
+    state.h0 = h0;
+    state.higher_entropy = h0 - brotli_bpb;
+    for (size_t i = 0; i < state.frequent_bytes.size(); i++) {
+      const uint8_t c = sorted[i];
+      char chmem[32];
+      state.frequent_bytes[i].first = p_->MapChar(c, chmem);
+      state.frequent_bytes[i].second = counts[c] * 1.0 / state.soup.size();
+    }
+    for (size_t i = 0; i < state.uncommon_bytes.size(); i++) {
+      const uint8_t c = sorted[256 - state.uncommon_bytes.size() + i];
+      char chmem[32];
+      state.uncommon_bytes[i].first = p_->MapChar(c, chmem);
+      state.uncommon_bytes[i].second = counts[c] * 1.0 / state.soup.size();
+    }
+    if (params.save_to.has_value() && (epoch % params.save_interval == 0)) {
+      std::vector<char> path(params.save_to->size() + 20);
+      snprintf(path.data(), path.size(), "%s/%010zu.dat", params.save_to->c_str(), epoch);
+      FILE *f = CheckFopen(path.data(), "w");
+      const size_t epoch_to_save = epoch + 1;
+      fwrite(&reset_index_here, sizeof(reset_index_here), 1, f);
+      fwrite(&num_programs, sizeof(num_programs), 1, f);
+      fwrite(&epoch_to_save, sizeof(epoch_to_save), 1, f);
+      fwrite(state.soup.data(), 1, state.soup.size(), f);
+      fclose(f);
+    }
+    const bool done = callback(state);
+    batch.join();
+    if (done) break;
+    start = batch_start;
+    epoch = next;
   }
   cubff_ctx_destroy(ctx);
 }

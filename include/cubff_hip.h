@@ -125,6 +125,10 @@ int cubff_set_pipeline(cubff_ctx *ctx, int mode, double wait_limit_s);
  * the runs of queued interactions (up to 64, one wavefront) claimed to run
  * them, and interactions handed to the resolver by CUBFF_FLOW_DEFER runs. */
 int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims, uint64_t *deferred);
+/* Byte histogram of the whole soup (counts[256]): the h0 and frequent /
+ * uncommon byte metrics of the callback block (common_language.h:519-540)
+ * without copying the soup. */
+int cubff_byte_histogram(cubff_ctx *ctx, uint64_t *counts);
 /* Command budget of an interaction in CUBFF_FLOW_DEFER epoch kernels
  * (default 768). */
 int cubff_set_defer_budget(cubff_ctx *ctx, uint32_t budget);

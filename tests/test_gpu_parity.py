@@ -186,3 +186,13 @@ def test_full_size_golden_sha(capi):
         g.run_epochs(capi.PAIR_SHUFFLE, seed, 1, 64, P)
         assert _sha(g.read_soup()) == (
             "72826732daa298a988a3ffc30a6734dbfd406084472df72ea713919365d80c89")
+
+
+@pytest.mark.parametrize("n", [2, 4096, 131072])
+def test_byte_histogram_matches_soup(capi, n):
+    # The callback block's h0 and byte tables come from this histogram.
+    with capi.Soup("bff_noheads", n) as g:
+        g.init_programs(O.epoch_seed(3, 0))
+        g.run_epochs(capi.PAIR_SHUFFLE, 3, 0, 3, 1 << 20)
+        soup = g.read_soup()
+        assert np.array_equal(g.byte_histogram(), np.bincount(soup, minlength=256).astype(np.uint64))
