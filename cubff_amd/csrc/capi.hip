@@ -46,6 +46,7 @@ struct cubff_ctx {
   hipStream_t stream2 = nullptr;
   hipStream_t stream3 = nullptr;  // pairings drawn ahead
   uint32_t *shuf2 = nullptr;      // second pairing buffer
+  uint64_t *num_indices2 = nullptr;  // its pair count
   unsigned long long *hist = nullptr;  // byte histogram, 256 entries
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
@@ -176,9 +177,15 @@ int ensure_flow_scratch(cubff_ctx *c, uint64_t epochs) {
 }
 
 int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint32_t *dst,
-              hipStream_t st = nullptr) {
+              hipStream_t st = nullptr, uint64_t *num_indices = nullptr) {
   if (!st) st = c->stream;
   switch (mode) {
+    case CUBFF_PAIR_2D:
+      if (!c->row_off || !num_indices)
+        return fail(CUBFF_ESTATE, "2-D pairing needs cubff_set_interactions first");
+      HIP_TRY(launch_pair_2d(st, c->fy, c->g2, c->row_off, c->nbr, dst, c->n, params_seed, epoch,
+                             num_indices));
+      return CUBFF_OK;
     case CUBFF_PAIR_SHUFFLE:
       HIP_TRY(launch_pair_shuffle(st, c->fy, dst, c->n, params_seed, epoch));
       return CUBFF_OK;
@@ -194,15 +201,18 @@ int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint
 }
 
 // Pairings drawn one epoch ahead on a third stream into two buffers (the
-// context's shuf and shuf2), so that drawing epoch j+1's pairing overlaps
-// epoch j's kernel: the pairings do not depend on the soup.  Epoch j's
-// kernel waits for its pairing; the pairing of epoch j+2 waits for epoch j's
-// kernel, which reads the same buffer.
+// context's shuf and shuf2, with their pair counts), so that drawing epoch
+// j+1's pairing overlaps epoch j's kernel: no pairing depends on the soup.
+// Epoch j's kernel waits for its pairing; the pairing of epoch j+2 waits for
+// epoch j's kernel, which reads the same buffer.  Epoch j's kernel is queued
+// before pairing j+1 is drawn, since the 2-D pairing blocks the host while
+// its rounds converge.
 struct PairAhead {
   cubff_ctx *c;
   int mode;
   uint64_t seed, first;
   uint32_t *buf[2] = {nullptr, nullptr};
+  uint64_t *nidx[2] = {nullptr, nullptr};  // pairs evaluated, per buffer
   hipEvent_t drawn[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
 
   int init() {
@@ -213,8 +223,11 @@ struct PairAhead {
         return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
     }
     if (!c->shuf2 && (r = dev_alloc(&c->shuf2, c->n))) return r;
+    if (!c->num_indices2 && (r = dev_alloc(&c->num_indices2, 1))) return r;
     buf[0] = c->shuf;
     buf[1] = c->shuf2;
+    nidx[0] = c->num_indices;
+    nidx[1] = c->num_indices2;
     for (int i = 0; i < 2; i++) {
       HIP_TRY(hipEventCreateWithFlags(&drawn[i], hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&used[i], hipEventDisableTiming));
@@ -227,7 +240,8 @@ struct PairAhead {
   int draw(uint32_t j) {
     int r;
     if (j >= 2) HIP_TRY(hipStreamWaitEvent(c->stream3, used[j % 2], 0));
-    if ((r = pair_into(c, mode, seed, first + j, buf[j % 2], c->stream3))) return r;
+    if ((r = pair_into(c, mode, seed, first + j, buf[j % 2], c->stream3, nidx[j % 2]))) return r;
+    if (mode != CUBFF_PAIR_2D) HIP_TRY(launch_set_u64(c->stream3, nidx[j % 2], c->n / 2));
     HIP_TRY(hipEventRecord(drawn[j % 2], c->stream3));
     return CUBFF_OK;
   }
@@ -241,7 +255,10 @@ struct PairAhead {
   }
   // The last epoch's pairing becomes the context's current one.
   void finish(uint32_t epochs) {
-    if (epochs && (epochs - 1) % 2 == 1) std::swap(c->shuf, c->shuf2);
+    if (epochs && (epochs - 1) % 2 == 1) {
+      std::swap(c->shuf, c->shuf2);
+      std::swap(c->num_indices, c->num_indices2);
+    }
   }
   ~PairAhead() {
     for (int i = 0; i < 2; i++) {
@@ -265,7 +282,6 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   hipError_t e = hipSuccess;
   for (auto &x : kev)
     if (e == hipSuccess) e = hipEventCreate(&x);
-  if (e == hipSuccess) e = launch_set_u64(c->stream, c->num_indices, c->n / 2);
   if (e != hipSuccess) {
     cleanup();
     return fail(CUBFF_EHIP, std::string("epoch setup failed: ") + hipGetErrorString(e));
@@ -273,15 +289,15 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   const uint64_t sm = splitmix64(params_seed);
   r = pa.draw(0);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
-    if (j + 1 < epochs && (r = pa.draw(j + 1))) break;
     if ((r = pa.before_kernel(j))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
     if (e == hipSuccess)
-      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % 2], c->n, c->num_indices,
+      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % 2], c->n, pa.nidx[j % 2],
                        epoch_seed_pre(sm, first + j), prob, c->pair_ops, c->pair_steps,
                        c->counters);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
     if (e == hipSuccess) r = pa.after_kernel(j);
+    if (r == CUBFF_OK && e == hipSuccess && j + 1 < epochs) r = pa.draw(j + 1);
   }
   hipError_t e2 = hipStreamSynchronize(c->stream3);
   if (e == hipSuccess) e = e2;
@@ -427,12 +443,13 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   r = pa.init();
   if (r == CUBFF_OK) r = pa.draw(0);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
-    if (j + 1 < epochs && (r = pa.draw(j + 1))) break;
     if ((r = pa.before_kernel(j))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
-    if (e == hipSuccess) e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % 2], j);
+    if (e == hipSuccess)
+      e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % 2], pa.nidx[j % 2], j);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
     if (e == hipSuccess) r = pa.after_kernel(j);
+    if (r == CUBFF_OK && e == hipSuccess && j + 1 < epochs) r = pa.draw(j + 1);
   }
   // Close the list whatever happened, so the resolver leaves.
   hipError_t e2 = launch_defer_close(c->stream, c->dctl);
@@ -460,7 +477,6 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
   pa.finish(epochs);
-  HIP_TRY(launch_set_u64(c->stream, c->num_indices, n / 2));
   c->defer_items += ctl.final_tail;
   // The resolver may leave early when it cannot run beside the epoch
   // kernels; the run is complete as long as every deferred entry was run.
@@ -478,9 +494,11 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
                     uint64_t count, uint32_t prob, float *timing) {
   const bool closed_form = mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
                            mode == CUBFF_PAIR_CYCLIC;
+  // Pairings drawn ahead: every device mode (none depends on the soup).
+  const bool ahead = closed_form || (mode == CUBFF_PAIR_2D && c->row_off);
   const bool defer = (c->pipeline == CUBFF_FLOW_DEFER ||
                       (c->pipeline == CUBFF_FLOW_AUTO && defer_pays(c->lang, c->n))) &&
-                     count >= 2 && closed_form;
+                     count >= 2 && ahead;
   const uint64_t seg = defer ? defer_segment_epochs(c->n) : flow_segment_epochs(c->n);
   const bool flow = !defer &&
                     (c->pipeline == CUBFF_FLOW_ON ||
@@ -524,7 +542,7 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
       e += ep;
       continue;
     }
-    if (!flow && !defer && closed_form && left >= 2) {
+    if (!flow && !defer && ahead && left >= 2) {
       const uint32_t ep = (uint32_t)std::min<uint64_t>(left, kLoopSegmentEpochs);
       r = run_loop_segment(c, mode, params_seed, e, ep, prob, timing ? &defer_kern : nullptr);
       if (timing) untimed[launch] = true;  // timed inside
@@ -683,6 +701,7 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   if (c->stream3) (void)hipStreamSynchronize(c->stream3);
   if (c->stream3) (void)hipStreamDestroy(c->stream3);
   dev_free(c->shuf2);
+  dev_free(c->num_indices2);
   dev_free(c->hist);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -60,9 +60,9 @@ __device__ bool wait_eq(DeferCtl *c, const uint32_t *p, uint32_t want, uint64_t 
 }  // namespace
 
 template <int LANG>
-__global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
-                                                             const uint32_t *__restrict__ shuf,
-                                                             uint32_t e_rel) {
+__global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
+    DeferArgs a, const uint32_t *__restrict__ shuf, const uint64_t *__restrict__ num_indices_p,
+    uint32_t e_rel) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<kEpochLanes>;
   using Aux = typename L::template Aux<kEpochLanes>;
@@ -87,6 +87,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
   const auto t = Tape::at(lds, lane);
   const auto aux = Aux::at(lds + Tape::kRegionBytes, lane);
   uint32_t ops = 0, steps = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone;
+  const bool eval = k < *num_indices_p;  // common_language.h:185
   if (k < a.slots) {
     p1 = shuf[2 * k];
     p2 = shuf[2 * k + 1];
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
       load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
       load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
       mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel), k, a.prob);
-      ops = L::template evaluate<true>(t, aux, kEvalSteps, &steps, nullptr, budget);
+      if (eval) ops = L::template evaluate<true>(t, aux, kEvalSteps, &steps, nullptr, budget);
       if (steps == kOverBudget) {
         defer = true;
         ops = steps = 0;
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
   if (defer && !put) st_agent(&a.ctl->abort, 1);
   if (put) {
     DeferItem *it = a.items + at;
-    it->e = e_rel;
+    it->e = e_rel | (eval ? 0u : kNoEval);
     it->k = (uint32_t)k;
     it->p1 = p1;
     it->p2 = p2;
@@ -156,7 +157,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(DeferArgs a,
   __syncthreads();
   if (lane == 0 && block_ops) atomicAdd(&a.counters[0], block_ops);
   if (lane == 0 && block_steps) atomicAdd(&a.counters[2], block_steps);
-  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.counters[1], (unsigned long long)a.slots);
+  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.counters[1], (unsigned long long)*num_indices_p);
 }
 
 // The resolver: one deferred interaction per wavefront at a time, run by
@@ -222,7 +223,8 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     DeferItem *it = a.items + my;
     if (!wait_eq(c, &it->ready, a.gen, a.spin_ticks)) break;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const uint32_t e = it->e, k = it->k, p1 = it->p1, p2 = it->p2;
+    const uint32_t e = it->e & ~kNoEval, k = it->k, p1 = it->p1, p2 = it->p2;
+    const bool eval = !(it->e & kNoEval);
     const uint32_t d1 = it->dep1, d2 = it->dep2;
     if (d1 != kNone && !wait_eq(c, &a.items[d1].done, a.gen, a.spin_ticks)) break;
     if (d2 != kNone && !wait_eq(c, &a.items[d2].done, a.gen, a.spin_ticks)) break;
@@ -230,8 +232,8 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
     load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
     mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
-    uint32_t steps = 0;
-    const uint32_t ops = L::evaluate(t, aux, kEvalSteps, &steps);
+    uint32_t steps = 0, ops = 0;
+    if (eval) ops = L::evaluate(t, aux, kEvalSteps, &steps);
     store_row(t, 0, a.soup + (uint64_t)p1 * kTape);
     store_row(t, 1, a.soup + (uint64_t)p2 * kTape);
     if (ops) atomicAdd(&a.counters[0], (unsigned long long)ops);
@@ -274,9 +276,9 @@ inline unsigned blocks_of(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
 
 template <int L>
 hipError_t epoch_defer_lang(hipStream_t st, const DeferArgs &a, const uint32_t *shuf,
-                            uint32_t e_rel) {
+                            const uint64_t *num_indices, uint32_t e_rel) {
   hipLaunchKernelGGL(k_epoch_defer<L>, dim3(blocks_of(a.slots, kEpochLanes)),
-                     dim3(kEpochLanes), 0, st, a, shuf, e_rel);
+                     dim3(kEpochLanes), 0, st, a, shuf, num_indices, e_rel);
   return hipGetLastError();
 }
 
@@ -315,9 +317,9 @@ hipError_t launch_defer_close(hipStream_t st, DeferCtl *c) {
 }
 
 hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,
-                              const uint32_t *shuf, uint32_t e_rel) {
+                              const uint32_t *shuf, const uint64_t *num_indices, uint32_t e_rel) {
   if (a.slots == 0) return hipSuccess;
-  CUBFF_DEFER_LANG(lang, epoch_defer_lang, (st, a, shuf, e_rel))
+  CUBFF_DEFER_LANG(lang, epoch_defer_lang, (st, a, shuf, num_indices, e_rel))
 }
 
 hipError_t launch_resolve(hipStream_t st, int lang, const DeferArgs &a, unsigned waves) {

@@ -61,14 +61,17 @@ struct alignas(16) DeferCtl {
   uint32_t head, tail, closed, final_tail, abort, resolved, pad[2];
 };
 
-// One deferred interaction: segment epoch, pair slot, its programs, the list
-// entries that owe them (kNone if none), and the run generation once the
-// entry is complete (ready) and once its result is written (done).
+// One deferred interaction: segment epoch (kNoEval set for a slot past the
+// epoch's evaluated pairs: mutated and written back only), pair slot, its
+// programs, the list entries that owe them (kNone if none), and the run
+// generation once the entry is complete (ready) and once its result is
+// written (done).
 struct DeferItem {
   uint32_t e, k, p1, p2, dep1, dep2, ready, done;
 };
 
 constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
+constexpr uint32_t kNoEval = 0x80000000u;    // DeferItem::e flag
 
 struct DeferArgs {
   uint8_t *soup;
@@ -123,7 +126,7 @@ hipError_t launch_trace_bff(hipStream_t st, int lang, const uint8_t *tape, uint3
 hipError_t launch_defer_start(hipStream_t st, DeferCtl *c);
 hipError_t launch_defer_close(hipStream_t st, DeferCtl *c);
 hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,
-                              const uint32_t *shuf, uint32_t e_rel);
+                              const uint32_t *shuf, const uint64_t *num_indices, uint32_t e_rel);
 hipError_t launch_resolve(hipStream_t st, int lang, const DeferArgs &a, unsigned waves);
 
 }  // namespace cubff
