@@ -199,9 +199,13 @@ def run_other_configs(capi, epochs):
         auto["loop_ms_per_epoch"] = loop["ms_per_epoch"]
         auto["dataflow_ms_per_epoch"] = flow["ms_per_epoch"]
         langs[lang] = auto
-    langs["bff_noheads_grid_512x256"] = run_soup_timed(
-        capi, "bff_noheads", 512 * 256, 1, P, 4, epochs, mode=capi.PAIR_2D,
-        pattern=grid_pattern(512, 256, 2))
+    grid = grid_pattern(512, 256, 2)
+    g_loop = run_soup_timed(capi, "bff_noheads", 512 * 256, 1, P, 4, epochs, mode=capi.PAIR_2D,
+                            pattern=grid, flow=capi.FLOW_OFF)
+    g_auto = run_soup_timed(capi, "bff_noheads", 512 * 256, 1, P, 4, epochs, mode=capi.PAIR_2D,
+                            pattern=grid)
+    g_auto["loop_ms_per_epoch"] = g_loop["ms_per_epoch"]
+    langs["bff_noheads_grid_512x256"] = g_auto
     out["all_interpreters"] = {
         "workload": "2^17 programs, seed 1, Fisher-Yates pairing; plus bff_noheads on a "
                     "512x256 grid with radius-2 neighbour pairing", "langs": langs}

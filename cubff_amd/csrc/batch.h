@@ -20,7 +20,8 @@ struct BatchDev {
   const uint32_t *prob;        // mutation threshold per soup (of 2^30)
   const uint64_t *num_indices; // evaluated slots per soup
   uint8_t *soup;               // total_programs * 64
-  uint32_t *shuf;              // total_programs, local indices per soup
+  uint32_t *shuf[2];           // total_programs each, local indices per soup:
+                               // the pairing of epoch e is in shuf[e & 1]
   uint8_t *stage;              // total_slots * 128
   uint32_t *tgt;               // total_slots * 2, global program indices
   uint32_t *pair_ops, *pair_steps;     // total_slots
@@ -32,8 +33,9 @@ struct BatchDev {
 };
 
 // The epoch is read from d.epoch; batch_end_epoch adds num_runs and
-// advances it, so one captured epoch can be replayed.
-hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d);
+// advances it, so one captured epoch can be replayed.  batch_pair_shuffle
+// draws the pairing of epoch *d.epoch + ahead (ahead 0 or 1).
+hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d, uint32_t ahead);
 hipError_t batch_stage(hipStream_t st, const BatchDev &d);
 hipError_t batch_end_epoch(hipStream_t st, const BatchDev &d);
 hipError_t batch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, uint64_t count);

@@ -33,8 +33,8 @@ __device__ __forceinline__ uint32_t soup_of(const uint64_t *__restrict__ base,
 // ---------------------------------------------------------------------------
 // Pairing.
 
-__global__ void k_b_fy_draw(const BatchDev d) {
-  const uint64_t epoch = *d.epoch;
+__global__ void k_b_fy_draw(const BatchDev d, uint32_t ahead) {
+  const uint64_t epoch = *d.epoch + ahead;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.total_programs) return;
   const uint32_t s = soup_of(d.prog_base, d.num_soups, t);
@@ -114,7 +114,7 @@ __device__ __forceinline__ uint32_t b_fy_in(const BatchDev &d, uint32_t t) {
 
 // Writes the pairing as local (per-soup) program indices, like the
 // reference's shuffle_idx.
-__global__ void k_b_fy_final(const BatchDev d) {
+__global__ void k_b_fy_final(const BatchDev d, uint32_t ahead) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.total_programs) return;
   const uint32_t j = d.fy_j[t];
@@ -126,7 +126,7 @@ __global__ void k_b_fy_final(const BatchDev d) {
     v = c == kNone ? j : b_fy_in(d, c);
   }
   const uint32_t s = soup_of(d.prog_base, d.num_soups, t);
-  d.shuf[t] = (uint32_t)(v - d.prog_base[s]);
+  d.shuf[(*d.epoch + ahead) & 1][t] = (uint32_t)(v - d.prog_base[s]);
 }
 
 __global__ void k_b_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t count) {
@@ -148,7 +148,7 @@ __global__ void k_b_stage(const BatchDev d) {
   const uint64_t k = g - d.slot_base[s];
   const uint64_t pb = d.prog_base[s];
   const uint64_t n = d.prog_base[s + 1] - pb;
-  const uint32_t p = d.shuf[pb + 2 * k + (q >> 2)];
+  const uint32_t p = d.shuf[epoch & 1][pb + 2 * k + (q >> 2)];
   uint4 v = reinterpret_cast<const uint4 *>(d.soup + (pb + p) * kTape)[q & 3];
   uint32_t w[4] = {v.x, v.y, v.z, v.w};
   const uint64_t seed_value = epoch_seed_pre(d.sm_seed[s], epoch);
@@ -263,15 +263,15 @@ hipError_t batch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, uint64_t coun
   return hipGetLastError();
 }
 
-hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d) {
+hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d, uint32_t ahead) {
   const uint64_t n = d.total_programs;
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_b_fill, dim3(nblk(n + 1, 256)), dim3(256), 0, st, d.fy_cnt, 0u, n + 1);
-  hipLaunchKernelGGL(k_b_fy_draw, dim3(g), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_b_fy_draw, dim3(g), dim3(256), 0, st, d, ahead);
   hipLaunchKernelGGL(k_b_fy_alloc, dim3(nblk(n, 1024)), dim3(1024), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_place, dim3(g), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_sort, dim3(g), dim3(256), 0, st, d);
-  hipLaunchKernelGGL(k_b_fy_final, dim3(g), dim3(256), 0, st, d);
+  hipLaunchKernelGGL(k_b_fy_final, dim3(g), dim3(256), 0, st, d, ahead);
   return hipGetLastError();
 }
 

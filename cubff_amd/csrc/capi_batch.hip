@@ -17,6 +17,12 @@ using namespace cubff;
 struct cubff_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  // Pairings are drawn one epoch ahead on stream2, forked from and joined
+  // back into each epoch (also inside the captured graph).
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  uint64_t drawn = UINT64_MAX;  // epoch whose pairing is in d.shuf[drawn & 1]
+  uint64_t last = 0;            // last epoch run
   uint32_t num_soups = 0;
   std::vector<int> langs;
   std::vector<uint64_t> n, prog_base, slot_base;
@@ -86,7 +92,12 @@ void drop_graph(cubff_batch *b) {
 // launches with events.
 int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   const hipStream_t st = b->stream;
-  HIP_TRY(batch_pair_shuffle(st, b->d));
+  // The next epoch's pairing, beside this one (it reads and writes only the
+  // other pairing buffer and the pairing scratch).
+  HIP_TRY(hipEventRecord(b->fork, st));
+  HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
+  HIP_TRY(batch_pair_shuffle(b->stream2, b->d, 1));
+  HIP_TRY(hipEventRecord(b->join, b->stream2));
   HIP_TRY(batch_stage(st, b->d));
   const uint32_t *list = b->plain_list;
   if (sched == CUBFF_SCHED_LEARNED) {
@@ -100,6 +111,7 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   for (const auto &l : b->lists)
     HIP_TRY(batch_interp(st, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
   if (ev1) HIP_TRY(hipEventRecord(ev1, st));
+  HIP_TRY(hipStreamWaitEvent(st, b->join, 0));  // before the epoch counter moves
   HIP_TRY(batch_end_epoch(st, b->d));
   return CUBFF_OK;
 }
@@ -119,9 +131,16 @@ int after_epoch(cubff_batch *b, int sched) {
   return CUBFF_OK;
 }
 
-int set_epoch(cubff_batch *b, uint64_t epoch) {
+// Sets the device epoch counter to the first epoch of a run of `count` and
+// draws that epoch's pairing unless the previous run drew it already.
+int start_run(cubff_batch *b, uint64_t epoch, uint64_t count) {
   HIP_TRY(hipMemcpyAsync(b->d.epoch, &epoch, 8, hipMemcpyHostToDevice, b->stream));
+  if (count && b->drawn != epoch) HIP_TRY(batch_pair_shuffle(b->stream, b->d, 0));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  if (count) {
+    b->drawn = epoch + count;
+    b->last = epoch + count - 1;
+  }
   return CUBFF_OK;
 }
 
@@ -169,7 +188,8 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
   if ((r = alloc(&b->prog_base_d, num_soups + 1)) || (r = alloc(&b->slot_base_d, num_soups + 1)) ||
       (r = alloc(&b->sm_seed_d, num_soups)) || (r = alloc(&b->num_indices_d, num_soups)) ||
       (r = alloc(&b->prob_d, num_soups)) || (r = alloc(&d.soup, P * kTape)) ||
-      (r = alloc(&d.shuf, P)) || (r = alloc(&d.stage, S * kTape2)) ||
+      (r = alloc(&d.shuf[0], P)) || (r = alloc(&d.shuf[1], P)) ||
+      (r = alloc(&d.stage, S * kTape2)) ||
       (r = alloc(&d.tgt, 2 * S)) || (r = alloc(&d.pair_ops, S)) ||
       (r = alloc(&d.pair_steps, S)) || (r = alloc(&d.pair_work, S)) ||
       (r = alloc(&ctr, (uint64_t)num_soups * kNumBatchCounters)) ||
@@ -193,6 +213,9 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
   d.prob = b->prob_d;
   d.num_indices = b->num_indices_d;
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join, hipEventDisableTiming);
   if (e != hipSuccess) {
     cubff_batch_destroy(b);
     return fail(CUBFF_EHIP, "hipStreamCreate failed");
@@ -239,7 +262,8 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->num_indices_d);
   release(b->prob_d);
   release(b->d.soup);
-  release(b->d.shuf);
+  release(b->d.shuf[0]);
+  release(b->d.shuf[1]);
   release(b->d.stage);
   release(b->d.tgt);
   release(b->d.pair_ops);
@@ -256,6 +280,10 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->d.fy_succ);
   release(b->plain_list);
   release(b->order_list);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return CUBFF_OK;
@@ -277,6 +305,7 @@ int cubff_batch_init(cubff_batch *b, const uint64_t *params_seeds,
                          2ull * b->num_soups * kNumBatchCounters));
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->have_seeds = true;
+  b->drawn = UINT64_MAX;  // pairings depend on the seeds
   return CUBFF_OK;
 }
 
@@ -305,7 +334,7 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
   if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
   if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
     return fail(CUBFF_ENOMEM, "scheduler allocation failed");
-  if ((r = set_epoch(b, first_epoch))) return r;
+  if ((r = start_run(b, first_epoch, count))) return r;
   if (!use_graph) {
     for (uint64_t i = 0; i < count; i++) {
       if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
@@ -343,7 +372,7 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t 
   if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
   if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
     return fail(CUBFF_ENOMEM, "scheduler allocation failed");
-  if ((r = set_epoch(b, first_epoch))) return r;
+  if ((r = start_run(b, first_epoch, count))) return r;
   std::vector<hipEvent_t> ev(2 * count + 2);
   for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], b->stream));
@@ -485,7 +514,7 @@ int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s) {
   int r;
   if ((r = check(b))) return r;
   if (soup >= b->num_soups || !s) return fail(CUBFF_EINVAL, "bad soup index or null destination");
-  HIP_TRY(hipMemcpyAsync(s, b->d.shuf + b->prog_base[soup], b->n[soup] * 4,
+  HIP_TRY(hipMemcpyAsync(s, b->d.shuf[b->last & 1] + b->prog_base[soup], b->n[soup] * 4,
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return CUBFF_OK;

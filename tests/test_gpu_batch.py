@@ -84,3 +84,27 @@ def test_batch_autotune_keeps_results(capi):
         b.run_epochs(0, 12)
         for i in range(len(spec)):
             assert np.array_equal(a.read_soup(i), b.read_soup(i))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_pairing_drawn_ahead(capi, graph):
+    # Each epoch draws the next epoch's pairing beside it; runs that continue
+    # where the last one stopped reuse it, runs that start elsewhere draw
+    # their own first.  Soups and pairings must equal the oracle either way.
+    spec = SPEC[:4]
+    E = 9
+    refs = [_oracle_run(l, n, s, p, E) for (l, n, s, p) in spec]
+    with capi.Batch([x[0] for x in spec], [x[1] for x in spec]) as b:
+        b.init([x[2] for x in spec], [x[3] for x in spec])
+        b.run_epochs(0, 4, capi.SCHED_PLAIN, use_graph=graph)
+        b.run_epochs(4, 2, capi.SCHED_PLAIN, use_graph=graph)  # continues: epoch 6 drawn
+        for i, (l, n, s, p) in enumerate(spec):
+            assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, 5)), i
+            assert np.array_equal(b.read_soup(i), refs[i][5][0]), i
+        # Jump: soups of epoch 7 written back, the run starts at epoch 8.
+        for i in range(len(spec)):
+            b.write_soup(i, refs[i][7][0])
+        b.run_epochs(8, 1, capi.SCHED_PLAIN, use_graph=graph)
+        for i, (l, n, s, p) in enumerate(spec):
+            assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, 8)), i
+            assert np.array_equal(b.read_soup(i), refs[i][8][0]), i
