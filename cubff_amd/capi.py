@@ -28,6 +28,7 @@ EXPORTS = [
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
     "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_trace_bff", "cubff_sync",
+    "cubff_trace_steps", "cubff_eval_selfrep_trace",
     "cubff_splitmix64", "cubff_epoch_seed",
     "cubff_batch_last_error", "cubff_batch_create", "cubff_batch_destroy",
     "cubff_batch_init", "cubff_batch_set_launch", "cubff_batch_get_launch",
@@ -98,6 +99,8 @@ def load():
         "cubff_eval_tapes": (_i, [_i, _i, _u8p, _u64, _u64, _u32p, _u32p]),
         "cubff_trace_bff": (_i, [_i, _i, _u8p, _u64, _u8p, _u64, _u64p, _u64p]),
         "cubff_eval_selfrep": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p]),
+        "cubff_eval_selfrep_trace": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p, _u8p]),
+        "cubff_trace_steps": (_i, [_i, _i, _u8p, _u64, _u8p, _u64, _u64p, _u64p, _u64p]),
         "cubff_sync": (_i, [_vp]),
         "cubff_splitmix64": (_u64, [_u64]),
         "cubff_epoch_seed": (_u64, [_u64, _u64]),
@@ -185,6 +188,36 @@ def eval_selfrep(lang: str, prog: bytes | np.ndarray, epoch: int, seed: int,
     _check(load().cubff_eval_selfrep(lang_id(lang), device, _ptr(p, _u8p), p.size,
                                      epoch, seed, ctypes.byref(out)))
     return int(out.value)
+
+
+STEP_RECORD_BYTES = 272
+
+
+def trace_steps(lang: str, tape: bytes | np.ndarray, steps: int, cap: int | None = None,
+                device: int = 0):
+    """Step records of one Evaluate on the GPU: (records uint8[k, 272], final
+    tape, ops, steps run).  Record: u32 pc, r0, r1, r2, tape, stack."""
+    t = np.frombuffer(bytes(tape), dtype=np.uint8).copy()
+    if t.size != 128:
+        raise ValueError("tape must be exactly 128 bytes long")
+    cap = steps if cap is None else cap
+    rec = np.zeros((max(cap, 1), STEP_RECORD_BYTES), dtype=np.uint8)
+    k, ops, run = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+    _check(load().cubff_trace_steps(lang_id(lang), device, _ptr(t, _u8p), steps,
+                                    _ptr(rec, _u8p), cap, ctypes.byref(k), ctypes.byref(ops),
+                                    ctypes.byref(run)))
+    return rec[:k.value].copy(), t, int(ops.value), int(run.value)
+
+
+def eval_selfrep_trace(lang: str, prog: bytes | np.ndarray, epoch: int, seed: int,
+                       device: int = 0):
+    """(score, tapes uint8[13, 5, 2, 128]) of the single-program check."""
+    p = np.frombuffer(bytes(prog), dtype=np.uint8).copy()
+    out = ctypes.c_uint64(0)
+    tapes = np.zeros((13, 5, 2, 128), dtype=np.uint8)
+    _check(load().cubff_eval_selfrep_trace(lang_id(lang), device, _ptr(p, _u8p), p.size,
+                                           epoch, seed, ctypes.byref(out), _ptr(tapes, _u8p)))
+    return int(out.value), tapes
 
 
 def trace_bff(lang: str, tape: bytes | np.ndarray, steps: int = 8192, device: int = 0):

@@ -1001,8 +1001,50 @@ int cubff_trace_bff(int lang, int device, const uint8_t *tape, uint64_t steps, u
   return CUBFF_OK;
 }
 
+static_assert(kStepRecordBytes == CUBFF_STEP_RECORD_BYTES, "step record size");
+
+int cubff_trace_steps(int lang, int device, uint8_t *tape, uint64_t steps, uint8_t *records,
+                      uint64_t cap, uint64_t *num_records, uint64_t *ops, uint64_t *steps_run) {
+  if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
+  if (!tape || (cap && !records)) return fail(CUBFF_EINVAL, "null tape or records");
+  if (steps > 0xffffffffULL) return fail(CUBFF_EINVAL, "steps must fit in 32 bits");
+  int r;
+  if ((r = set_device(device))) return r;
+  uint8_t *d_t = nullptr, *d_o = nullptr;
+  uint64_t *d_m = nullptr;
+  if ((r = dev_alloc(&d_t, kTape2)) ||
+      (r = dev_alloc(&d_o, (uint64_t)kStepRecordBytes * std::max<uint64_t>(cap, 1))) ||
+      (r = dev_alloc(&d_m, 3))) {
+    dev_free(d_t);
+    dev_free(d_o);
+    return r;
+  }
+  uint64_t meta[3] = {0, 0, 0};
+  hipError_t e = hipMemcpy(d_t, tape, kTape2, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_trace_steps(nullptr, lang, d_t, (uint32_t)steps, d_o, cap, d_m);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(meta, d_m, sizeof(meta), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && meta[0] > cap) e = hipErrorUnknown;
+  if (e == hipSuccess && meta[0])
+    e = hipMemcpy(records, d_o, (uint64_t)kStepRecordBytes * meta[0], hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(tape, d_t, kTape2, hipMemcpyDeviceToHost);
+  dev_free(d_t);
+  dev_free(d_o);
+  dev_free(d_m);
+  if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("trace failed: ") + hipGetErrorString(e));
+  if (num_records) *num_records = meta[0];
+  if (ops) *ops = meta[1];
+  if (steps_run) *steps_run = meta[2];
+  return CUBFF_OK;
+}
+
 int cubff_eval_selfrep(int lang, int device, const uint8_t *prog, uint64_t len,
                        uint64_t epoch, uint64_t seed, uint64_t *result) {
+  return cubff_eval_selfrep_trace(lang, device, prog, len, epoch, seed, result, nullptr);
+}
+
+int cubff_eval_selfrep_trace(int lang, int device, const uint8_t *prog, uint64_t len,
+                             uint64_t epoch, uint64_t seed, uint64_t *result, uint8_t *tapes) {
   if (lang < 0 || lang >= kNumLangs) return fail(CUBFF_EINVAL, "unknown language id");
   if (!result) return fail(CUBFF_EINVAL, "null result");
   int r;
@@ -1010,19 +1052,24 @@ int cubff_eval_selfrep(int lang, int device, const uint8_t *prog, uint64_t len,
   uint8_t padded[kTape] = {};
   if (len > kTape) len = kTape;
   if (len) memcpy(padded, prog, len);
-  uint8_t *d_p = nullptr;
+  uint8_t *d_p = nullptr, *d_tr = nullptr;
   uint64_t *d_r = nullptr;
-  if ((r = dev_alloc(&d_p, kTape)) || (r = dev_alloc(&d_r, 1))) {
+  if ((r = dev_alloc(&d_p, kTape)) || (r = dev_alloc(&d_r, 1)) ||
+      (tapes && (r = dev_alloc(&d_tr, CUBFF_SELFREP_TRACE_BYTES)))) {
     dev_free(d_p);
+    dev_free(d_r);
     return r;
   }
   const uint64_t seed_value = epoch_seed_pre(splitmix64(seed), epoch);  // :334
   hipError_t e = hipMemcpy(d_p, padded, kTape, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_selfrep(nullptr, lang, d_p, 1, 1, seed_value, d_r);
+  if (e == hipSuccess) e = launch_selfrep(nullptr, lang, d_p, 1, 1, seed_value, d_r, d_tr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemcpy(result, d_r, 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && tapes)
+    e = hipMemcpy(tapes, d_tr, CUBFF_SELFREP_TRACE_BYTES, hipMemcpyDeviceToHost);
   dev_free(d_p);
   dev_free(d_r);
+  dev_free(d_tr);
   if (e != hipSuccess)
     return fail(CUBFF_EHIP, std::string("eval_selfrep failed: ") + hipGetErrorString(e));
   return CUBFF_OK;

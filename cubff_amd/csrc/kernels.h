@@ -95,9 +95,11 @@ hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
                         const uint64_t *num_indices, uint64_t seed_value,
                         uint32_t prob, uint32_t *pair_ops, uint32_t *pair_steps,
                         unsigned long long *counters);
+// trace (nullable): program 0's tapes before and after each generation of
+// each trial, 13 * 5 * 2 * 128 bytes.
 hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
                           uint64_t n, uint64_t count, uint64_t seed_value,
-                          uint64_t *result);
+                          uint64_t *result, uint8_t *trace = nullptr);
 hipError_t launch_eval_tapes(hipStream_t st, int lang, uint8_t *tapes,
                              uint64_t count, uint32_t steps, uint32_t *ops,
                              uint32_t *steps_out);
@@ -121,6 +123,10 @@ hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,
 hipError_t launch_flow_pairof(hipStream_t st, const uint32_t *shuf, uint32_t *pairof,
                               uint64_t n, uint32_t epochs);
 hipError_t launch_flow(hipStream_t st, int lang, const FlowArgs &a);
+// Step records of any language (trace.hip): kStepRecordBytes per step.
+constexpr int kStepRecordBytes = 16 + 2 * kTape2;
+hipError_t launch_trace_steps(hipStream_t st, int lang, uint8_t *tape, uint32_t steps,
+                              uint8_t *out, uint64_t cap, uint64_t *meta);
 hipError_t launch_trace_bff(hipStream_t st, int lang, const uint8_t *tape, uint32_t steps,
                             uint8_t *out, uint64_t cap, uint64_t *meta);
 hipError_t launch_defer_start(hipStream_t st, DeferCtl *c);

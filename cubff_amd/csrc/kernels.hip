@@ -101,7 +101,7 @@ constexpr int kGens = 4;     // kNumExtraGens, :208
 template <int LANG>
 __global__ __launch_bounds__(kSelfrepLanes) void k_selfrep(
     const uint8_t *__restrict__ soup, uint64_t n, uint64_t count,
-    uint64_t seed_value, uint64_t *__restrict__ result) {
+    uint64_t seed_value, uint64_t *__restrict__ result, uint8_t *__restrict__ trace) {
   using L = typename LangOf<LANG>::T;
   constexpr int kLanes = kSelfrepLanes;  // kSelfrepProgs * 13 (+ padding)
   using Tape = typename L::template Tape<kLanes>;
@@ -135,15 +135,29 @@ __global__ __launch_bounds__(kSelfrepLanes) void k_selfrep(
       t.set_word(w, prog[w]);
       t.set_word(16 + w, noise[w]);
     }
+    // Optional record of program 0's tapes before and after every
+    // generation, [trial][generation][before, after][128] (the states the
+    // reference prints under `debug`, common_language.h:225-255).
+    uint32_t *rec = trace && idx == 0
+                        ? reinterpret_cast<uint32_t *>(trace + (uint64_t)it * (kGens + 1) * 2 * kTape2)
+                        : nullptr;
+    auto keep = [&](int slot) {
+      if (rec)
+        for (int w = 0; w < kTape2 / 4; w++) rec[slot * (kTape2 / 4) + w] = t.word(w);
+    };
     uint32_t steps;
+    keep(0);
     L::evaluate(t, aux, kEvalSteps, &steps);
+    keep(1);
     for (int g = 0; g < kGens; g++) {
 #pragma unroll
       for (int w = 0; w < kTape / 4; w++) {
         t.set_word(w, t.word(16 + w));
         t.set_word(16 + w, noise[w]);
       }
+      keep(2 * g + 2);
       L::evaluate(t, aux, kEvalSteps, &steps);
+      keep(2 * g + 3);
     }
   }
   __syncthreads();
@@ -527,10 +541,10 @@ hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
 
 hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
                           uint64_t n, uint64_t count, uint64_t seed_value,
-                          uint64_t *result) {
+                          uint64_t *result, uint8_t *trace) {
   if (count == 0) return hipSuccess;
   CUBFF_LAUNCH_LANG(lang, k_selfrep, dim3(blocks_for(count, kSelfrepProgs)),
-                    dim3(kSelfrepLanes), st, soup, n, count, seed_value, result)
+                    dim3(kSelfrepLanes), st, soup, n, count, seed_value, result, trace)
 }
 
 hipError_t launch_eval_tapes(hipStream_t st, int lang, uint8_t *tapes,

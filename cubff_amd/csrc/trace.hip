@@ -144,7 +144,192 @@ __global__ __launch_bounds__(64) void k_trace(const uint8_t *__restrict__ tape_i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Step records of every language: the state each language's Evaluate prints
+// before every step under `debug` (bff.inc.h:368-372, forth.inc.h:385-388,
+// subleq.cu:114-118, rsubleq4.cu:122-126).  Record: u32 pc, u32 r0, r1, r2,
+// the 128 tape bytes, 128 stack bytes, with r0/r1 = head0/head1 (BFF), r0 =
+// stack depth (Forth, stack bytes [0, r0) valid), r0/r1 = the two operand
+// addresses (SUBLEQ) or r0/r1/r2 = the three (RSUBLEQ4).  One thread: a
+// debugging aid, not a hot path.  meta = {records, ops, steps}; the final
+// tape is written back.
+
 namespace {
+
+struct PlainTape {  // get/set view for ForthLang::reset
+  uint8_t *p;
+  __device__ uint32_t get(int i) const { return p[i]; }
+  __device__ void set(int i, uint8_t v) const { p[i] = v; }
+};
+
+// EvaluateOne of the Forth family (forth.inc.h:251-373; the reset checks of
+// forthtrivial_reset.cu:166-176 and :221-229).
+template <int LANG>
+__device__ void forth_step(uint8_t *tape, uint8_t *stk, int &sp, bool &ovf, int &pos,
+                           uint32_t &nops) {
+  auto push = [&](uint32_t v) {
+    if (sp != kTape2) stk[sp++] = (uint8_t)v;
+    else ovf = true;
+  };
+  auto pop = [&]() -> uint32_t { return sp == 0 ? 0u : stk[--sp]; };
+  constexpr bool kReset = LANG == kForthtrivialReset;
+  const uint32_t c = tape[pos];
+  switch (forth_kind<LANG>(c)) {
+    case fRead0: { const uint32_t a = pop() % kTape; push(tape[a]); break; }
+    case fRead1: { const uint32_t a = pop() % kTape; push(tape[kTape + a]); break; }
+    case fRead: { const uint32_t a = pop() % kTape2; push(tape[a]); break; }
+    case fWrite: {
+      const uint32_t v = pop(), a = pop() % kTape2;
+      tape[a] = (uint8_t)v;
+      break;
+    }
+    case fWrite0: {
+      const uint32_t v = pop(), a = pop() % kTape;
+      tape[a] = (uint8_t)v;
+      break;
+    }
+    case fWrite1: {
+      const uint32_t v = pop(), a = pop() % kTape;
+      if (kReset && tape[kTape + a] == 0x13) ForthLang<LANG>::reset(PlainTape{tape}, pos);
+      else tape[kTape + a] = (uint8_t)v;
+      break;
+    }
+    case fDup: { const uint32_t v = pop(); push(v); push(v); break; }
+    case fDrop: pop(); break;
+    case fSwap: { const uint32_t x = pop(), y = pop(); push(x); push(y); break; }
+    case fIf0: {
+      const uint32_t v = pop();
+      if (v) pos++;
+      push(v);
+      break;
+    }
+    case fInc: push(pop() + 1); break;
+    case fDec: push(pop() - 1); break;
+    case fAdd: { const uint32_t x = pop(), y = pop(); push(x + y); break; }
+    case fSub: { const uint32_t x = pop(), y = pop(); push(x - y); break; }
+    case fConst: push(c & 63); break;
+    case fCopy0: {
+      const uint32_t a = pop() % kTape;
+      if (kReset && tape[kTape + a] == 0x13) ForthLang<LANG>::reset(PlainTape{tape}, pos);
+      else tape[kTape + a] = tape[a];
+      break;
+    }
+    case fCopy1: { const uint32_t a = pop() % kTape; tape[a] = tape[kTape + a]; break; }
+    case fCopy: {
+      const uint32_t to = pop() % kTape2, from = pop() % kTape2;
+      tape[to] = tape[from];
+      break;
+    }
+    case fXor: push(pop() ^ 64); break;
+    case fJmp: {
+      const int d = (int)(c & 63) + 1;
+      pos += ((c & 64) ? -d : d) - 1;
+      break;
+    }
+    default: nops++; break;
+  }
+  pos++;
+}
+
+template <int LANG>
+__global__ __launch_bounds__(64) void k_steps(uint8_t *__restrict__ tape_io, uint32_t steps,
+                                              uint8_t *__restrict__ out, uint64_t cap,
+                                              uint64_t *__restrict__ meta) {
+  __shared__ uint8_t tape[kTape2], stk[kTape2];
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < kTape2; i++) {
+    tape[i] = tape_io[i];
+    stk[i] = 0;
+  }
+  uint64_t rec = 0;
+  auto record = [&](int pc, uint32_t r0, uint32_t r1, uint32_t r2) {
+    if (rec >= cap) return;
+    uint32_t *o = reinterpret_cast<uint32_t *>(out + rec * kStepRecordBytes);
+    o[0] = (uint32_t)pc;
+    o[1] = r0;
+    o[2] = r1;
+    o[3] = r2;
+    for (int w = 0; w < kTape2 / 4; w++) {
+      o[4 + w] = reinterpret_cast<const uint32_t *>(tape)[w];
+      o[4 + kTape2 / 4 + w] = reinterpret_cast<const uint32_t *>(stk)[w];
+    }
+    rec++;
+  };
+  uint32_t i = 0, nskip = 0;
+  if constexpr (LANG >= kBff && LANG <= kBffSelfmove) {
+    constexpr int MAP = LANG == kBffNoheads4bit ? kMap4bit
+                        : LANG == kBffPerm      ? kMapPerm
+                        : LANG == kBffSelfmove  ? kMapSelfmove
+                                                : kMapAscii;
+    constexpr int LOGIC = (LANG == kBff8 || LANG == kBff8Noheads) ? kLogicBff8
+                          : LANG == kBffSelfmove                  ? kLogicSelfmove
+                                                                  : kLogicStd;
+    constexpr bool HEADS = !(LANG == kBffNoheads || LANG == kBff8Noheads ||
+                             LANG == kBffNoheads4bit);
+    int h0 = HEADS ? tape[0] % kTape2 : kTape2, h1 = HEADS ? tape[1] % kTape2 : kTape2;
+    int pc = HEADS ? 2 : 0;
+    for (; i < steps; i++) {  // bff.inc.h:356-389
+      h0 &= kTape2 - 1;
+      h1 &= kTape2 - 1;
+      record(pc, (uint32_t)h0, (uint32_t)h1, 0);
+      if (!step<MAP, LOGIC>(tape, h0, h1, pc)) nskip++;
+      if (pc < 0 || ++pc >= kTape2) {
+        i++;
+        break;
+      }
+    }
+  } else if constexpr (LANG >= kForth && LANG <= kForthtrivialReset) {
+    int sp = 0, pos = 0;
+    bool ovf = false;
+    for (; i < steps; i++) {  // forth.inc.h:379-397
+      record(pos, (uint32_t)sp, 0, 0);
+      forth_step<LANG>(tape, stk, sp, ovf, pos, nskip);
+      if (pos >= kTape2 || pos < 0 || ovf) {
+        i++;
+        break;
+      }
+    }
+  } else if constexpr (LANG == kSubleq) {
+    int pos = 0;
+    for (; i < steps; i++) {  // subleq.cu:109-135
+      const uint32_t sa = tape[pos] % kTape2, sb = tape[pos + 1] % kTape2;
+      record(pos, sa, sb, 0);
+      tape[sa] = (uint8_t)(tape[sa] - tape[sb]);
+      if ((tape[sa] & 0x80) || tape[sa] == 0) pos = tape[pos + 2];
+      else pos += 3;
+      if (pos + 3 > kTape2 || pos < 0) {
+        i++;
+        break;
+      }
+    }
+  } else {
+    int pos = 0;
+    for (; i < steps; i++) {  // rsubleq4.cu:116-142
+      const uint32_t sa = (pos + tape[pos]) % kTape2, sb = (pos + tape[pos + 1]) % kTape2;
+      const uint32_t sc = (pos + tape[pos + 2]) % kTape2;
+      record(pos, sa, sb, sc);
+      tape[sa] = (uint8_t)(tape[sb] - tape[sc]);
+      if ((tape[sa] & 0x80) || tape[sa] == 0) pos += (int8_t)tape[pos + 3];
+      else pos += 4;
+      if (pos + 4 > kTape2 || pos < 0) {
+        i++;
+        break;
+      }
+    }
+  }
+  for (int q = 0; q < kTape2; q++) tape_io[q] = tape[q];
+  meta[0] = rec;
+  meta[1] = i - nskip;
+  meta[2] = i;
+}
+
+template <int LANG>
+hipError_t steps_one(hipStream_t st, uint8_t *tape, uint32_t steps, uint8_t *out, uint64_t cap,
+                     uint64_t *meta) {
+  hipLaunchKernelGGL(k_steps<LANG>, dim3(1), dim3(64), 0, st, tape, steps, out, cap, meta);
+  return hipGetLastError();
+}
+
 template <int MAP, bool HEADS, int LOGIC>
 hipError_t trace_one(hipStream_t st, const uint8_t *tape, uint32_t steps, uint8_t *out,
                      uint64_t cap, uint64_t *meta) {
@@ -153,6 +338,29 @@ hipError_t trace_one(hipStream_t st, const uint8_t *tape, uint32_t steps, uint8_
   return hipGetLastError();
 }
 }  // namespace
+
+hipError_t launch_trace_steps(hipStream_t st, int lang, uint8_t *tape, uint32_t steps,
+                              uint8_t *out, uint64_t cap, uint64_t *meta) {
+  switch (lang) {
+#define CUBFF_STEPS_CASE(L) \
+  case L: return steps_one<L>(st, tape, steps, out, cap, meta);
+    CUBFF_STEPS_CASE(kBff)
+    CUBFF_STEPS_CASE(kBffNoheads)
+    CUBFF_STEPS_CASE(kBff8)
+    CUBFF_STEPS_CASE(kBff8Noheads)
+    CUBFF_STEPS_CASE(kBffNoheads4bit)
+    CUBFF_STEPS_CASE(kBffPerm)
+    CUBFF_STEPS_CASE(kBffSelfmove)
+    CUBFF_STEPS_CASE(kForth)
+    CUBFF_STEPS_CASE(kForthcopy)
+    CUBFF_STEPS_CASE(kForthtrivial)
+    CUBFF_STEPS_CASE(kForthtrivialReset)
+    CUBFF_STEPS_CASE(kSubleq)
+    CUBFF_STEPS_CASE(kRsubleq4)
+#undef CUBFF_STEPS_CASE
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t launch_trace_bff(hipStream_t st, int lang, const uint8_t *tape, uint32_t steps,
                             uint8_t *out, uint64_t cap, uint64_t *meta) {

@@ -166,6 +166,12 @@ struct BffPresenter : Presenter {
     printf("\n");
   }
 
+  // bff.inc.h:368-372.
+  void PrintStep(const uint8_t *rec) const override {
+    PrintInternal(step_field(rec, 1), step_field(rec, 2), step_field(rec, 0), rec + 16,
+                  2 * kSingleTapeSize, nullptr, 0);
+  }
+
   void PrintProgram(size_t pc_pos, const uint8_t *mem, size_t len, const size_t *seps,
                     size_t nseps) const override {
     size_t h0 = 2 * kSingleTapeSize, h1 = 2 * kSingleTapeSize;
@@ -245,6 +251,17 @@ struct ForthPresenter : Presenter {
 
   void PrintProgram(size_t pc_pos, const uint8_t *mem, size_t len, const size_t *seps,
                     size_t nseps) const override {
+    PrintInternal(pc_pos, mem, len, seps, nseps, nullptr, 0);
+  }
+
+  // forth.inc.h:385-388: the tape, then the stack bottom to top.
+  void PrintStep(const uint8_t *rec) const override {
+    PrintInternal(step_field(rec, 0), rec + 16, 2 * kSingleTapeSize, nullptr, 0,
+                  rec + 16 + 2 * kSingleTapeSize, step_field(rec, 1));
+  }
+
+  void PrintInternal(size_t pc_pos, const uint8_t *mem, size_t len, const size_t *seps,
+                     size_t nseps, const uint8_t *stack, size_t stack_len) const {
     size_t sep = 0;
     for (size_t i = 0; i < len; i++) {
       if (sep < nseps && seps[sep] == i) {
@@ -274,6 +291,13 @@ struct ForthPresenter : Presenter {
         printf("\x1b[38:5:%dm%s", color, hex_char(c, chmem));
       }
       printf("%s", ResetColors());
+    }
+    if (stack) {
+      printf("   ");
+      for (size_t i = 0; i < stack_len; i++) {
+        char chmem[4];
+        printf("%s ", hex_char(stack[i], chmem));
+      }
     }
     printf("\n");
   }
@@ -321,6 +345,20 @@ struct SubleqPresenter : Presenter {
   void PrintProgram(size_t pc, const uint8_t *mem, size_t len, const size_t *seps,
                     size_t nseps) const override {
     const size_t none = 2 * kSingleTapeSize;
+    PrintInternal(none, none, none, pc, mem, len, seps, nseps);
+  }
+
+  // subleq.cu:114-118 (heads a, b), rsubleq4.cu:122-126 (heads a, b, c).
+  void PrintStep(const uint8_t *rec) const override {
+    PrintInternal(step_field(rec, 1), step_field(rec, 2),
+                  rel4 ? step_field(rec, 3) : 2 * kSingleTapeSize, step_field(rec, 0), rec + 16,
+                  2 * kSingleTapeSize, nullptr, 0);
+  }
+
+  // subleq.cu:66-96 and rsubleq4.cu:70-104: the instruction at pc, then the
+  // operand heads; a later colour code overrides an earlier one.
+  void PrintInternal(size_t h0, size_t h1, size_t h2, size_t pc, const uint8_t *mem,
+                     size_t len, const size_t *seps, size_t nseps) const {
     size_t sep = 0;
     for (size_t i = 0; i < len; i++) {
       if (sep < nseps && seps[sep] == i) {
@@ -333,7 +371,14 @@ struct SubleqPresenter : Presenter {
         printf(rel4 ? "\x1b[33;1m" : "\x1b[32;1m");
         color = true;
       }
-      (void)none;
+      if (rel4) {
+        if (i == h2) { printf("\x1b[34;1m"); color = true; }
+        if (i == h1) { printf("\x1b[32;1m"); color = true; }
+        if (i == h0) { printf("\x1b[31;1m"); color = true; }
+      } else {
+        if (i == h0) { printf("\x1b[34;1m"); color = true; }
+        if (i == h1) { printf("\x1b[31;1m"); color = true; }
+      }
       printf("%02X", mem[i]);
       if (color) printf("\x1b[;m");
     }

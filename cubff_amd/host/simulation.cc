@@ -49,6 +49,9 @@ void hip_check(int rc, const char *what) {
   }
 }
 
+// Steps --debug prints at most (272 bytes of record each on the host).
+constexpr uint64_t kMaxDebugSteps = 1ull << 20;
+
 #define CHECK(op)                   \
   if (!(op)) {                      \
     printf("%s is false\n", #op);   \
@@ -80,11 +83,25 @@ class HipLanguage : public LanguageInterface {
     uint8_t tape[2 * kSingleTapeSize] = {};
     memcpy(tape, parsed.data(), std::min<size_t>(parsed.size(), sizeof(tape)));
     p_->PrintProgram(2 * kSingleTapeSize, tape, 2 * kSingleTapeSize, nullptr, 0);
-    if (debug)
-      fprintf(stderr, "note: per-step tracing (--debug) is not available in this build; "
-                      "showing the final state only\n");
-    uint32_t ops = 0, steps = 0;
-    hip_check(cubff_eval_tapes(lang_, 0, tape, 1, stepcount, &ops, &steps), "cubff_eval_tapes");
+    uint64_t ops = 0;
+    if (debug) {
+      // Every step's state, recorded on the GPU, printed in order
+      // (bff.inc.h:368-372 and the other Evaluate loops).
+      const uint64_t cap = std::min<uint64_t>(stepcount, kMaxDebugSteps);
+      std::vector<uint8_t> rec(cap * CUBFF_STEP_RECORD_BYTES);
+      uint64_t nrec = 0, run = 0;
+      hip_check(cubff_trace_steps(lang_, 0, tape, stepcount, rec.data(), cap, &nrec, &ops, &run),
+                "cubff_trace_steps");
+      for (uint64_t i = 0; i < nrec; i++) p_->PrintStep(rec.data() + i * CUBFF_STEP_RECORD_BYTES);
+      if (run > nrec)
+        fprintf(stderr, "note: --debug printed the first %llu of %llu steps\n",
+                (unsigned long long)nrec, (unsigned long long)run);
+    } else {
+      uint32_t ops32 = 0, steps = 0;
+      hip_check(cubff_eval_tapes(lang_, 0, tape, 1, stepcount, &ops32, &steps),
+                "cubff_eval_tapes");
+      ops = ops32;
+    }
     printf("%s", ResetColors());
     printf("ops: %d\n", (int)ops);
     printf("\n");
@@ -99,11 +116,33 @@ class HipLanguage : public LanguageInterface {
   // common_language.h:326-342.
   size_t EvalParsedSelfrep(std::vector<uint8_t> &parsed, size_t epoch, size_t seed,
                            bool debug) override {
-    if (debug)
-      fprintf(stderr, "note: per-generation printing (debug) is not available in this build\n");
     uint64_t res = 0;
-    hip_check(cubff_eval_selfrep(lang_, 0, parsed.data(), parsed.size(), epoch, seed, &res),
-              "cubff_eval_selfrep");
+    if (!debug) {
+      hip_check(cubff_eval_selfrep(lang_, 0, parsed.data(), parsed.size(), epoch, seed, &res),
+                "cubff_eval_selfrep");
+      return res;
+    }
+    // common_language.h:225-255: each trial's tapes before and after each
+    // generation, recorded on the GPU and printed trial by trial.
+    std::vector<uint8_t> tapes(CUBFF_SELFREP_TRACE_BYTES);
+    hip_check(cubff_eval_selfrep_trace(lang_, 0, parsed.data(), parsed.size(), epoch, seed, &res,
+                                       tapes.data()),
+              "cubff_eval_selfrep_trace");
+    const size_t seps[1] = {kSingleTapeSize};
+    for (size_t i = 0; i < 13; i++) {
+      for (size_t g = 0; g < 5; g++) {
+        const uint8_t *t = tapes.data() + (i * 5 + g) * 2 * 2 * kSingleTapeSize;
+        for (int after = 0; after < 2; after++) {
+          if (g == 0)
+            printf("Iteration %lu: %s first step\n", (unsigned long)i, after ? "after" : "before");
+          else
+            printf("Iteration %lu: %s step %lu\n", (unsigned long)i, after ? "after" : "before",
+                   (unsigned long)(g + 1));
+          p_->PrintProgram(2 * kSingleTapeSize, t + after * 2 * kSingleTapeSize,
+                           2 * kSingleTapeSize, seps, 1);
+        }
+      }
+    }
     return res;
   }
 

@@ -179,12 +179,35 @@ int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
 int cubff_trace_bff(int lang, int device, const uint8_t *tape, uint64_t steps, uint8_t *out,
                     uint64_t out_bytes, uint64_t *file_bytes, uint64_t *ops);
 
+/* Step-by-step record of one Evaluate of a 128-byte tape in any language:
+ * before every step, the state the reference prints under `debug`
+ * (bff.inc.h:368-372, forth.inc.h:385-388, subleq.cu:114-118,
+ * rsubleq4.cu:122-126), as CUBFF_STEP_RECORD_BYTES-byte records: u32 pc,
+ * u32 r0, r1, r2 (BFF: head0, head1; Forth: stack depth; SUBLEQ: the two
+ * operand addresses; RSUBLEQ4: the three), then the 128 tape bytes and 128
+ * stack bytes (Forth; zeros otherwise).  Up to `cap` records are kept
+ * (records holds cap * CUBFF_STEP_RECORD_BYTES bytes); the run continues to
+ * its end.  tape is updated in place; *ops receives the Evaluate result,
+ * *steps_run the steps taken, *num_records the records written.  Serves
+ * --debug of RunSingleProgram (common_language.h:196-201, main.cc:179). */
+#define CUBFF_STEP_RECORD_BYTES 272
+int cubff_trace_steps(int lang, int device, uint8_t *tape, uint64_t steps, uint8_t *records,
+                      uint64_t cap, uint64_t *num_records, uint64_t *ops, uint64_t *steps_run);
+
 /* Single-program self-replication check of EvalParsedSelfrep
  * (common_language.h:326-342): prog is zero-padded to 64 bytes and scored
  * with seed_value = S(epoch) computed from `seed`. */
 int cubff_eval_selfrep(int lang, int device, const uint8_t *prog,
                        uint64_t len, uint64_t epoch, uint64_t seed,
                        uint64_t *result);
+
+/* The same check, also returning the tapes CheckSelfRep prints under
+ * `debug` (common_language.h:225-255): for each of the 13 trials and each of
+ * its 5 generations, the 128-byte tape before and after the Evaluate, in
+ * that order (tapes holds CUBFF_SELFREP_TRACE_BYTES bytes). */
+#define CUBFF_SELFREP_TRACE_BYTES (13 * 5 * 2 * 128)
+int cubff_eval_selfrep_trace(int lang, int device, const uint8_t *prog, uint64_t len,
+                             uint64_t epoch, uint64_t seed, uint64_t *result, uint8_t *tapes);
 
 /* Waits for all work queued on the context (Synchronize, :65). */
 int cubff_sync(cubff_ctx *ctx);

@@ -2,6 +2,7 @@
 // code): drives the reference's own RunSimulation (common_language.h:344-597)
 // through its public LanguageInterface and hands every callback's
 // SimulationState to a C callback.  Built only into oracle/_ref/.
+#include <stdio.h>
 #include <cstdint>
 #include <optional>
 #include <string>
@@ -89,6 +90,19 @@ unsigned long long ref_eval_parsed_selfrep(const char *lang,
   std::vector<uint8_t> parsed(prog, prog + len);
   LanguageInterface *l = const_cast<LanguageInterface *>(GetLanguage(lang));
   return l->EvalParsedSelfrep(parsed, epoch, seed, false);
+}
+
+// The same with debug on: CheckSelfRep prints every trial's tapes to stdout.
+unsigned long long ref_eval_parsed_selfrep_debug(const char *lang,
+                                                 const uint8_t *prog,
+                                                 unsigned long long len,
+                                                 unsigned long long epoch,
+                                                 unsigned long long seed) {
+  std::vector<uint8_t> parsed(prog, prog + len);
+  LanguageInterface *l = const_cast<LanguageInterface *>(GetLanguage(lang));
+  unsigned long long r = l->EvalParsedSelfrep(parsed, epoch, seed, true);
+  fflush(stdout);
+  return r;
 }
 
 // LanguageInterface::Parse; returns the parsed length (<= cap bytes copied).

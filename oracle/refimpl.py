@@ -58,6 +58,9 @@ def lib():
             ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_char_p,
             ctypes.c_char_p, ctypes.c_ulonglong, ctypes.c_char_p, _u64p, _u32p,
             CALLBACK, ctypes.c_void_p]
+        L.ref_eval_parsed_selfrep_debug.restype = ctypes.c_ulonglong
+        L.ref_eval_parsed_selfrep_debug.argtypes = [ctypes.c_char_p, _u8p, ctypes.c_ulonglong,
+                                                    ctypes.c_ulonglong, ctypes.c_ulonglong]
         L.ref_eval_parsed_selfrep.restype = ctypes.c_ulonglong
         L.ref_eval_parsed_selfrep.argtypes = [ctypes.c_char_p, _u8p,
                                               ctypes.c_ulonglong,
@@ -153,8 +156,10 @@ def parse(lang: str, program: str) -> np.ndarray:
     return buf[:n].copy()
 
 
-def eval_parsed_selfrep(lang: str, parsed: np.ndarray, epoch: int, seed: int) -> int:
-    """EvalParsedSelfrep (common_language.h:326-342) of the reference build."""
+def eval_parsed_selfrep(lang: str, parsed: np.ndarray, epoch: int, seed: int,
+                        debug: bool = False) -> int:
+    """EvalParsedSelfrep (common_language.h:326-342) of the reference build;
+    with debug, its per-trial printing goes to this process's stdout."""
     p = np.ascontiguousarray(parsed, dtype=np.uint8)
-    return int(lib().ref_eval_parsed_selfrep(lang.encode(), p.ctypes.data_as(_u8p), p.size,
-                                             epoch, seed))
+    f = lib().ref_eval_parsed_selfrep_debug if debug else lib().ref_eval_parsed_selfrep
+    return int(f(lang.encode(), p.ctypes.data_as(_u8p), p.size, epoch, seed))

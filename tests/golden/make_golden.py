@@ -363,6 +363,46 @@ def gen_cli_cases():
         "pattern_2d": "make_2d_pattern.py 32 32 (radius 2)", "cases": cases})
 
 
+def gen_cli_debug():
+    """Reference outputs of the debug printing: bin/main --run ... --debug
+    (every step's state) and EvalParsedSelfrep with debug (every trial's
+    tapes), zlib-compressed."""
+    main = R.MAIN_PATH
+    tmp = tempfile.mkdtemp()
+    runs = {}
+    for lang, texts in run_programs().items():
+        rows = []
+        for k, t in enumerate(texts):
+            for steps in ((300, 1000) if k == 0 else (300,)):
+                p = subprocess.run([main, "--lang", lang, "--run", t, "--run_steps", str(steps),
+                                    "--debug"], cwd=tmp, capture_output=True)
+                assert p.returncode == 0
+                rows.append({"text": t, "steps": steps,
+                             "stdout_zlib_b64": base64.b64encode(zlib.compress(p.stdout, 9)).decode()})
+        runs[lang] = rows
+    selfrep = {}
+    code = ("import sys; sys.path.insert(0, {root!r}); import numpy as np; "
+            "from oracle import refimpl as R; "
+            "r = R.eval_parsed_selfrep({lang!r}, R.parse({lang!r}, {text!r}), {e}, {s}, debug=True); "
+            "sys.stdout.flush(); sys.stderr.write(str(r))")
+    for lang, texts in run_programs().items():
+        rows = []
+        for t in texts:
+            if R.parse(lang, t).size > 64:
+                continue
+            p = subprocess.run([sys.executable, "-c", code.format(root=ROOT, lang=lang, text=t,
+                                                                 e=3, s=11)],
+                               capture_output=True)
+            assert p.returncode == 0, p.stderr
+            rows.append({"text": t, "epoch": 3, "seed": 11, "result": int(p.stderr.decode()),
+                         "stdout_zlib_b64": base64.b64encode(zlib.compress(p.stdout, 9)).decode()})
+            break
+        selfrep[lang] = rows
+    dump("cli_debug.json", {
+        "source": "reference bin/main and EvalParsedSelfrep built in oracle/_ref (debug on)",
+        "runs": runs, "eval_selfrep": selfrep})
+
+
 def _grid_lists(w, h, r=2):
     return [[y * w + x for y in range(i - r, i + r + 1) for x in range(j - r, j + r + 1)
              if 0 <= y < h and 0 <= x < w and (y, x) != (i, j)]
@@ -371,7 +411,7 @@ def _grid_lists(w, h, r=2):
 
 GENS = {"eval": gen_eval, "epochs": gen_epochs, "selfrep": gen_selfrep,
         "2d": gen_2d, "modes": gen_modes, "full": gen_full_size,
-        "cli": gen_cli_logs, "cli_cases": gen_cli_cases}
+        "cli": gen_cli_logs, "cli_cases": gen_cli_cases, "cli_debug": gen_cli_debug}
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()

@@ -115,3 +115,25 @@ def test_cli_argument_errors_match_reference(cases, tmp_path):
             assert p.stderr.splitlines()[2:] == e["stderr"].splitlines()[2:]
         else:
             assert p.stderr == e["stderr"], e["args"]
+
+
+def test_debug_fixtures_agree_with_plain_transcripts():
+    # The reference's --debug transcript of a run = its plain transcript with
+    # one line per step inserted after the initial program (same program,
+    # same step limit).
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    with open(os.path.join(golden, "cli_debug.json")) as f:
+        dbg = json.load(f)["runs"]
+    with open(os.path.join(golden, "cli_cases.json")) as f:
+        plain = json.load(f)["runs"]
+    seen = 0
+    for lang, rows in dbg.items():
+        for r in rows:
+            for p in plain[lang]:
+                if p["text"] == r["text"] and p["steps"] == r["steps"]:
+                    a = zlib.decompress(base64.b64decode(r["stdout_zlib_b64"])).split(b"\n")
+                    b = zlib.decompress(base64.b64decode(p["stdout_zlib_b64"])).split(b"\n")
+                    assert a[0] == b[0] and a[-4:] == b[-4:], lang
+                    assert len(a) > len(b) or r["steps"] == 0
+                    seen += 1
+    assert seen >= 13

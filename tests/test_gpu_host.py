@@ -137,6 +137,40 @@ def test_python_module_full_size_soups():
         assert all(s[3] == 131072 for s in seen)
 
 
+@pytest.mark.parametrize("lang", LANGS)
+def test_cli_debug_transcripts_match_reference(lang, tmp_path):
+    # --run ... --debug: the state before every step, printed the way each
+    # language's Evaluate prints it, from step records made on the GPU.
+    for r in load("cli_debug.json")["runs"][lang]:
+        out = run_main(["--lang", lang, "--run", r["text"], "--run_steps", str(r["steps"]),
+                        "--debug"], tmp_path)
+        ref = zlib.decompress(base64.b64decode(r["stdout_zlib_b64"]))
+        assert out == ref, (lang, r["text"], r["steps"])
+
+
+def test_python_module_eval_selfrep_debug(tmp_path):
+    # EvalSelfrep with debug prints every trial's tapes before and after each
+    # generation, as CheckSelfRep does; the score is unchanged.
+    script = tmp_path / "sr.py"
+    script.write_text(
+        "import sys, json, base64\n"
+        f"sys.path.insert(0, {BIN!r})\n"
+        "import cubff\n"
+        "lang, text, e, s = sys.argv[1], sys.argv[2], int(sys.argv[3]), int(sys.argv[4])\n"
+        "r = cubff.GetLanguage(lang).EvalSelfrep(text, e, s, True)\n"
+        "sys.stdout.flush()\n"
+        "sys.stderr.write('RESULT %d' % r)\n")
+    for lang, rows in load("cli_debug.json")["eval_selfrep"].items():
+        for r in rows:
+            print("selfrep debug", lang, r["text"], flush=True)
+            p = subprocess.run([sys.executable, str(script), lang, r["text"], str(r["epoch"]),
+                                str(r["seed"])], capture_output=True, timeout=120)
+            assert p.returncode == 0, p.stderr.decode()[-2000:]
+            assert p.stderr.decode().endswith("RESULT %d" % r["result"]), lang
+            ref = zlib.decompress(base64.b64decode(r["stdout_zlib_b64"]))
+            assert p.stdout == ref, lang
+
+
 def test_python_module_eval_selfrep():
     cubff = py_module()
     for lang, rows in load("cli_cases.json")["eval_selfrep"].items():

@@ -46,3 +46,30 @@ def test_evaluate_and_save(capi, tmp_path):
         trace.evaluate_and_save(bytearray(64), path, 10)
     with pytest.raises(Exception):
         capi.trace_bff("forth", t.tobytes(), 10)
+
+
+LANGS = BFF_LANGS + ["forth", "forthcopy", "forthtrivial", "forthtrivial_reset", "subleq",
+                     "rsubleq4"]
+
+
+@pytest.mark.parametrize("lang", LANGS)
+def test_step_records_match_oracle(capi, lang):
+    # cubff_trace_steps: the final tape, ops and steps of the run equal the
+    # oracle's Evaluate; record k holds the tape after k steps (the oracle
+    # run with a step limit of k); BFF records equal the oracle's traces.
+    tapes = biased_tapes(lang, 12, seed=5)
+    for i, t in enumerate(tapes):
+        for steps in (0, 1, 50, 8192):
+            rec, final, ops, run = capi.trace_steps(lang, t.tobytes(), steps, cap=min(steps, 600))
+            f_ref, ops_ref, st_ref = O.evaluate(lang, t, steps)
+            assert np.array_equal(final, f_ref) and ops == ops_ref and run == st_ref, (lang, i)
+            assert len(rec) == min(run, 600)
+            for k in sorted({0, 1, 7, len(rec) // 2, len(rec) - 1}):
+                if 0 <= k < len(rec):
+                    assert np.array_equal(rec[k, 16:144], O.evaluate(lang, t, k)[0]), (lang, i, k)
+            if lang in BFF_LANGS and len(rec):
+                states, _, _ = O.trace_bff(lang, t, steps)
+                n = len(rec)
+                assert np.array_equal(rec[:, :12], states[:n, :12]), (lang, i)
+                assert np.array_equal(rec[:, 16:144], states[:n, 12:]), (lang, i)
+                assert not rec[:, 12:16].any() and not rec[:, 144:].any()
