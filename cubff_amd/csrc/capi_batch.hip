@@ -117,7 +117,9 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
 }
 
 // After an epoch under the learned scheduler: train on its recorded ops on
-// the first epoch and every kTrainInterval epochs after that.
+// the first epoch (synchronously: the initial weights are random) and every
+// kTrainInterval epochs after that (beside the next epochs, on the
+// scheduler's training stream); publish finished trainings.
 constexpr uint64_t kTrainInterval = 16;
 constexpr int kTrainSteps = 16;
 constexpr float kTrainLr = 2e-3f;
@@ -125,9 +127,22 @@ constexpr float kTrainLr = 2e-3f;
 int after_epoch(cubff_batch *b, int sched) {
   if (sched != CUBFF_SCHED_LEARNED) return CUBFF_OK;
   if (b->sched.epochs_seen++ % kTrainInterval == 0) {
-    float loss = 0;
-    HIP_TRY(sched_train(b->stream, b->sched, b->d, kTrainSteps, kTrainLr, &loss));
+    if (b->sched.trainings == 0 && !b->sched.pending) {
+      float loss = 0;
+      HIP_TRY(sched_train(b->stream, b->sched, b->d, kTrainSteps, kTrainLr, &loss));
+    } else {
+      HIP_TRY(sched_train_async(b->stream, b->sched, b->d, kTrainSteps, kTrainLr));
+    }
+  } else {
+    HIP_TRY(sched_poll(b->stream, b->sched, false));
   }
+  return CUBFF_OK;
+}
+
+// End of a run: a training still in flight is waited for and published on
+// the batch stream, so the run's work (and any timing of it) includes it.
+int finish_run(cubff_batch *b, int sched) {
+  if (sched == CUBFF_SCHED_LEARNED) HIP_TRY(sched_poll(b->stream, b->sched, true));
   return CUBFF_OK;
 }
 
@@ -340,7 +355,7 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
       if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
       if ((r = after_epoch(b, sched))) return r;
     }
-    return CUBFF_OK;
+    return finish_run(b, sched);
   }
   if (!b->graph || b->graph_sched != sched || b->graph_wg != b->wg ||
       b->graph_per_lane != b->per_lane) {
@@ -362,7 +377,7 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
     HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     if ((r = after_epoch(b, sched))) return r;
   }
-  return CUBFF_OK;
+  return finish_run(b, sched);
 }
 
 int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t count,
@@ -380,6 +395,7 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t 
     if ((r = enqueue_epoch(b, sched, ev[2 + 2 * i], ev[3 + 2 * i]))) return r;
     if ((r = after_epoch(b, sched))) return r;
   }
+  if ((r = finish_run(b, sched))) return r;
   HIP_TRY(hipEventRecord(ev[1], b->stream));
   HIP_TRY(hipEventSynchronize(ev[1]));
   float tot = 0, it = 0;
@@ -432,7 +448,7 @@ int cubff_batch_sched_stats(cubff_batch *b, uint64_t *trainings, float *last_los
   int r;
   if ((r = check(b))) return r;
   if (trainings) *trainings = b->sched.trainings;
-  if (last_loss) *last_loss = b->sched.last_loss;
+  if (last_loss) *last_loss = sched_last_loss(b->sched);
   if (bucket_counts) {
     for (int i = 0; i < kBuckets; i++) bucket_counts[i] = 0;
     if (b->sched.ready) {

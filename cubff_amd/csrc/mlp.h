@@ -26,13 +26,23 @@ constexpr uint64_t kOffW3 = kOffB2 + kMlpHid;
 constexpr uint64_t kOffB3 = kOffW3 + kMlpHid;
 constexpr uint64_t kNumParams = kOffB3 + 1;
 
+// Weights of the fused inference kernel: folded E W1 / 128 and W2, both
+// transposed bf16 [256][256], and vec = b1, b2, w3 (256 each), b3.
+constexpr int kInfVec = 3 * kMlpHid + 1;
+struct MlpInfer {
+  uint16_t *ew1t = nullptr, *w2t = nullptr;
+  float *vec = nullptr;
+};
+
 struct Mlp {
   uint64_t capacity = 0;  // rows the activation buffers hold (multiple of 128)
   float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
   // bf16 weights: W (row-major [in][out]) and WT ([out][in]) per layer.
   uint16_t *e_w = nullptr, *e_wt = nullptr, *w1 = nullptr, *w1t = nullptr;
   uint16_t *w2 = nullptr, *w2t = nullptr;
-  uint16_t *ew1t = nullptr;            // folded E W1 / 128 as [256][256], inference only
+  // Inference weights: `live` is what mlp_predict reads; `next` receives a
+  // fold made on another stream, published into `live` later.
+  MlpInfer live, next;
   bool folded_valid = false;
   uint16_t *x = nullptr;               // [capacity, 256] features
   uint16_t *p = nullptr, *a1 = nullptr, *a2 = nullptr;  // [capacity, 256]
@@ -56,13 +66,19 @@ hipError_t mlp_features_rows(Mlp &m, const uint8_t *stage, const uint32_t *rows,
                              uint64_t count, hipStream_t st);
 // Forward over m.x[0..count) into m.y (keeps p, a1, a2 for backward).
 hipError_t mlp_forward(Mlp &m, uint64_t count, hipStream_t st);
-// Folds the first two linear maps (E W1) for inference; recomputed after
-// every update.
+// Folds the first two linear maps (E W1) and copies W2, the biases and w3
+// for inference, into `live`; recomputed after every update.
 hipError_t mlp_fold(Mlp &m, hipStream_t st);
+// The same into `next` (training on its own stream) and next -> live.
+hipError_t mlp_fold_next(Mlp &m, hipStream_t st);
+hipError_t mlp_publish(Mlp &m, hipStream_t st);
 // Fused inference of `count` tapes (device, 128 bytes apart) straight into
 // y[0..count): features, both layers and the output in one kernel, with the
 // folded first layer.  Needs no activation buffers.
 hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y, hipStream_t st);
+// mlp_predict with the `live` weights as they are (never folds).
+hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
+                            hipStream_t st);
 // Loss 0.5 * mean((y - target)^2) and all gradients into m.grads for
 // m.x / m.target rows [0..count); *m.loss receives the loss.
 hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st);

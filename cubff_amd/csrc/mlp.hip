@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
                                                  uint64_t count,
                                                  const uint16_t *__restrict__ ew1t,
                                                  const uint16_t *__restrict__ w2t,
-                                                 const float *__restrict__ params,
+                                                 const float *__restrict__ vec,
                                                  float *__restrict__ y) {
   __shared__ __align__(16) uint32_t cnt8[kInfRows * kCountStride / 4];
   __shared__ __align__(16) uint16_t hist[kInfRows * kRowStride];
@@ -395,11 +395,11 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
   float b1[2], b2[2], w3[2];
 #pragma unroll
   for (int c = 0; c < 2; c++) {
-    b1[c] = params[kOffB1 + col0 + 16 * c + r16];
-    b2[c] = params[kOffB2 + col0 + 16 * c + r16];
-    w3[c] = params[kOffW3 + col0 + 16 * c + r16];
+    b1[c] = vec[col0 + 16 * c + r16];
+    b2[c] = vec[kMlpHid + col0 + 16 * c + r16];
+    w3[c] = vec[2 * kMlpHid + col0 + 16 * c + r16];
   }
-  const float b3 = params[kOffB3];
+  const float b3 = vec[3 * kMlpHid];
 
   // Thread tid loads 16 bytes (chunk tid % 8) of tape tid / 8 of each pass.
   const int tr = tid >> 3, tpart = tid & 7;
@@ -513,6 +513,12 @@ hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
     hipError_t e = mlp_fold(m, st);
     if (e != hipSuccess) return e;
   }
+  return mlp_predict_live(m, tapes, count, y, st);
+}
+
+hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
+                            hipStream_t st) {
+  if (count == 0) return hipSuccess;
   int dev = 0, cus = 0;
   hipError_t e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
@@ -520,8 +526,8 @@ hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
     return e;
   const uint64_t passes = (count + kInfRows - 1) / kInfRows;
   const unsigned grid = (unsigned)(passes < (uint64_t)cus ? passes : (uint64_t)cus);
-  hipLaunchKernelGGL(k_predict, dim3(grid), dim3(kInfThreads), 0, st, tapes, count, m.ew1t,
-                     m.w2t, m.params, y);
+  hipLaunchKernelGGL(k_predict, dim3(grid), dim3(kInfThreads), 0, st, tapes, count, m.live.ew1t,
+                     m.live.w2t, m.live.vec, y);
   return hipGetLastError();
 }
 
@@ -547,7 +553,12 @@ hipError_t mlp_alloc(Mlp &m, uint64_t capacity) {
       (e = zalloc(&m.w1t, (uint64_t)kMlpHid * kMlpHid)) ||
       (e = zalloc(&m.w2, (uint64_t)kMlpHid * kMlpHid)) ||
       (e = zalloc(&m.w2t, (uint64_t)kMlpHid * kMlpHid)) ||
-      (e = zalloc(&m.ew1t, (uint64_t)kMlpIn * kMlpHid)) ||
+      (e = zalloc(&m.live.ew1t, (uint64_t)kMlpIn * kMlpHid)) ||
+      (e = zalloc(&m.live.w2t, (uint64_t)kMlpHid * kMlpHid)) ||
+      (e = zalloc(&m.live.vec, kInfVec)) ||
+      (e = zalloc(&m.next.ew1t, (uint64_t)kMlpIn * kMlpHid)) ||
+      (e = zalloc(&m.next.w2t, (uint64_t)kMlpHid * kMlpHid)) ||
+      (e = zalloc(&m.next.vec, kInfVec)) ||
       (e = zalloc(&m.x, capacity * kMlpIn)) || (e = zalloc(&m.p, capacity * kMlpHid)) ||
       (e = zalloc(&m.a1, capacity * kMlpHid)) || (e = zalloc(&m.a2, capacity * kMlpHid)) ||
       (e = zalloc(&m.d2, capacity * kMlpHid)) || (e = zalloc(&m.d1, capacity * kMlpHid)) ||
@@ -558,7 +569,8 @@ hipError_t mlp_alloc(Mlp &m, uint64_t capacity) {
 }
 
 void mlp_free(Mlp &m) {
-  void *ps[] = {m.params, m.grads, m.m, m.v, m.e_w, m.e_wt, m.w1, m.w1t, m.w2, m.w2t, m.ew1t,
+  void *ps[] = {m.params, m.grads, m.m, m.v, m.e_w, m.e_wt, m.w1, m.w1t, m.w2, m.w2t,
+                m.live.ew1t, m.live.w2t, m.live.vec, m.next.ew1t, m.next.w2t, m.next.vec,
                 m.x, m.p, m.a1, m.a2, m.d2, m.d1, m.dp, m.y, m.target, m.loss};
   for (void *p : ps)
     if (p) (void)hipFree(p);
@@ -577,10 +589,41 @@ __global__ void k_fold(const float *__restrict__ params, uint16_t *__restrict__ 
   ew1t[j * kMlpIn + i] = f2bf(s * (1.0f / 128));
 }
 
-hipError_t mlp_fold(Mlp &m, hipStream_t st) {
-  hipLaunchKernelGGL(k_fold, dim3(kMlpIn), dim3(kMlpHid), 0, st, m.params, m.ew1t);
+// b1, b2, w3, b3 of the fp32 masters into an inference vector.
+__global__ void k_inf_vec(const float *__restrict__ params, float *__restrict__ vec) {
+  const int j = threadIdx.x;
+  vec[j] = params[kOffB1 + j];
+  vec[kMlpHid + j] = params[kOffB2 + j];
+  vec[2 * kMlpHid + j] = params[kOffW3 + j];
+  if (j == 0) vec[3 * kMlpHid] = params[kOffB3];
+}
+
+static hipError_t fold_into(const Mlp &m, const MlpInfer &dst, hipStream_t st) {
+  hipLaunchKernelGGL(k_fold, dim3(kMlpIn), dim3(kMlpHid), 0, st, m.params, dst.ew1t);
+  hipLaunchKernelGGL(k_inf_vec, dim3(1), dim3(kMlpHid), 0, st, m.params, dst.vec);
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dst.w2t, m.w2t, (uint64_t)kMlpHid * kMlpHid * 2, hipMemcpyDeviceToDevice,
+                       st);
+  return e;
+}
+
+hipError_t mlp_fold(Mlp &m, hipStream_t st) {
+  hipError_t e = fold_into(m, m.live, st);
   if (e == hipSuccess) m.folded_valid = true;
+  return e;
+}
+
+hipError_t mlp_fold_next(Mlp &m, hipStream_t st) { return fold_into(m, m.next, st); }
+
+hipError_t mlp_publish(Mlp &m, hipStream_t st) {
+  hipError_t e = hipMemcpyAsync(m.live.ew1t, m.next.ew1t, (uint64_t)kMlpIn * kMlpHid * 2,
+                                hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m.live.w2t, m.next.w2t, (uint64_t)kMlpHid * kMlpHid * 2,
+                       hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(m.live.vec, m.next.vec, kInfVec * 4, hipMemcpyDeviceToDevice, st);
   return e;
 }
 

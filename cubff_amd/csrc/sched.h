@@ -29,6 +29,14 @@ struct SchedState {
   uint64_t train_rows_n = 0;
   uint64_t epochs_seen = 0, trainings = 0;
   float last_loss = 0.f;
+  // Training after the first runs on its own stream beside the next
+  // epochs: the epoch's features and targets are taken on the batch stream,
+  // the trainer folds the result into the model's `next` weights, and
+  // sched_poll publishes them between epochs once `trained` has fired.
+  hipStream_t train_stream = nullptr;
+  hipEvent_t taken = nullptr, trained = nullptr;
+  bool pending = false;
+  float *loss0 = nullptr;  // device: first-step loss of the last training
 };
 
 constexpr uint64_t kTrainRows = 8192;
@@ -47,8 +55,19 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d,
                        const uint32_t *plain, uint32_t *order, const SchedList *lists,
                        size_t num_lists);
 // Trains on the last epoch's staged tapes and recorded ops (host-driven,
-// outside captured graphs).  Returns the loss of the first step.
+// outside captured graphs) and publishes the weights before returning.
+// Returns the loss of the first step.
 hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int steps,
                        float lr, float *first_loss);
+// The same on the training stream: returns once the work is queued; st only
+// waits for the features and targets to be taken (and for a previous
+// training still pending, which it publishes first).
+hipError_t sched_train_async(hipStream_t st, SchedState &s, const BatchDev &d, int steps,
+                             float lr);
+// Publishes a pending training's weights on st if it has finished, or,
+// with wait, in any case (st then waits for it).
+hipError_t sched_poll(hipStream_t st, SchedState &s, bool wait);
+// Loss of the first step of the last finished training (reads the device).
+float sched_last_loss(SchedState &s);
 
 }  // namespace cubff

@@ -165,6 +165,10 @@ int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
   if (hipMemcpy(s.train_rows, rows.data(), s.train_rows_n * 4, hipMemcpyHostToDevice) !=
       hipSuccess)
     return -1;
+  if (hipStreamCreateWithFlags(&s.train_stream, hipStreamNonBlocking) != hipSuccess) return -1;
+  if (hipEventCreateWithFlags(&s.taken, hipEventDisableTiming) != hipSuccess) return -1;
+  if (hipEventCreateWithFlags(&s.trained, hipEventDisableTiming) != hipSuccess) return -1;
+  if (hipMalloc(&s.loss0, 4) != hipSuccess) return -1;
   hipStream_t st = nullptr;
   if (mlp_init(s.mlp, 0x5eed, st) != hipSuccess) return -1;
   if (mlp_fold(s.mlp, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
@@ -173,7 +177,12 @@ int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
 }
 
 void sched_free(SchedState &s) {
+  if (s.train_stream) (void)hipStreamSynchronize(s.train_stream);
   mlp_free(s.mlp);
+  if (s.loss0) (void)hipFree(s.loss0);
+  if (s.taken) (void)hipEventDestroy(s.taken);
+  if (s.trained) (void)hipEventDestroy(s.trained);
+  if (s.train_stream) (void)hipStreamDestroy(s.train_stream);
   if (s.bucket) (void)hipFree(s.bucket);
   if (s.cnt) (void)hipFree(s.cnt);
   if (s.blk) (void)hipFree(s.blk);
@@ -188,7 +197,7 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const u
                        uint32_t *order, const SchedList *lists, size_t num_lists) {
   if (!s.ready) return hipErrorNotReady;
   hipError_t e;
-  if ((e = mlp_predict(s.mlp, d.stage, d.total_slots, s.pred, st))) return e;
+  if ((e = mlp_predict_live(s.mlp, d.stage, d.total_slots, s.pred, st))) return e;
   hipLaunchKernelGGL(k_sched_fill, dim3(1), dim3(256), 0, st, s.cnt, 0u,
                      (uint64_t)num_lists * 2 * kBuckets);
   hipLaunchKernelGGL(k_sched_fill, dim3(kPredBins / 256), dim3(256), 0, st, s.hist, 0u,
@@ -211,31 +220,92 @@ hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const u
   return hipGetLastError();
 }
 
-hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int steps, float lr,
-                       float *first_loss) {
-  if (!s.ready) return hipErrorNotReady;
+namespace {
+
+// Features and targets of the training sample, from the epoch just run.
+hipError_t take_sample(hipStream_t st, SchedState &s, const BatchDev &d) {
   hipError_t e;
   const uint64_t n = s.train_rows_n;
   if ((e = mlp_features_rows(s.mlp, d.stage, s.train_rows, n, st))) return e;
   // Targets: the commands each slot's interpreter actually ran (the cost
   // the schedule balances), not the reference's op count.
-  if ((e = mlp_targets_from_ops(s.mlp, d.pair_work, s.train_rows, n, st))) return e;
-  // The first round starts from random weights: train it longer.
-  if (s.trainings == 0) steps *= 8;
+  return mlp_targets_from_ops(s.mlp, d.pair_work, s.train_rows, n, st);
+}
+
+// `steps` Adam steps on the taken sample; the first step's loss to loss0.
+hipError_t train_steps(hipStream_t st, SchedState &s, int steps, float lr) {
+  hipError_t e;
+  const uint64_t n = s.train_rows_n;
   for (int i = 0; i < steps; i++) {
     if ((e = mlp_forward(s.mlp, n, st))) return e;
     if ((e = mlp_backward(s.mlp, n, st))) return e;
-    if (i == 0 && first_loss) {
-      if ((e = hipMemcpyAsync(first_loss, s.mlp.loss, 4, hipMemcpyDeviceToHost, st))) return e;
-    }
+    if (i == 0 && (e = hipMemcpyAsync(s.loss0, s.mlp.loss, 4, hipMemcpyDeviceToDevice, st)))
+      return e;
     if ((e = mlp_adam(s.mlp, lr, st))) return e;
   }
-  // Inference (possibly replayed from a captured graph) reads the folded
-  // first layer: refresh it now, outside any capture.
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t sched_poll(hipStream_t st, SchedState &s, bool wait) {
+  if (!s.pending) return hipSuccess;
+  if (!wait) {
+    const hipError_t q = hipEventQuery(s.trained);
+    if (q == hipErrorNotReady) return hipSuccess;
+    if (q != hipSuccess) return q;
+  }
+  hipError_t e;
+  if ((e = hipStreamWaitEvent(st, s.trained, 0))) return e;
+  if ((e = mlp_publish(s.mlp, st))) return e;
+  s.pending = false;
+  s.trainings++;
+  return hipSuccess;
+}
+
+float sched_last_loss(SchedState &s) {
+  if (!s.ready || s.trainings == 0) return s.last_loss;
+  float v = 0.f;
+  if (s.train_stream && hipStreamSynchronize(s.train_stream) == hipSuccess &&
+      hipMemcpy(&v, s.loss0, 4, hipMemcpyDeviceToHost) == hipSuccess)
+    s.last_loss = v;
+  return s.last_loss;
+}
+
+hipError_t sched_train_async(hipStream_t st, SchedState &s, const BatchDev &d, int steps,
+                             float lr) {
+  if (!s.ready) return hipErrorNotReady;
+  hipError_t e;
+  // The sample buffers belong to a pending training until it is done.
+  if ((e = sched_poll(st, s, true))) return e;
+  if ((e = take_sample(st, s, d))) return e;
+  if ((e = hipEventRecord(s.taken, st))) return e;
+  if ((e = hipStreamWaitEvent(s.train_stream, s.taken, 0))) return e;
+  if ((e = train_steps(s.train_stream, s, steps, lr))) return e;
+  if ((e = mlp_fold_next(s.mlp, s.train_stream))) return e;
+  if ((e = hipEventRecord(s.trained, s.train_stream))) return e;
+  s.pending = true;
+  return hipSuccess;
+}
+
+hipError_t sched_train(hipStream_t st, SchedState &s, const BatchDev &d, int steps, float lr,
+                       float *first_loss) {
+  if (!s.ready) return hipErrorNotReady;
+  hipError_t e;
+  if ((e = sched_poll(st, s, true))) return e;
+  if ((e = take_sample(st, s, d))) return e;
+  // The first round starts from random weights: train it longer.
+  if (s.trainings == 0) steps *= 8;
+  if ((e = train_steps(st, s, steps, lr))) return e;
+  // Inference (possibly replayed from a captured graph) reads the live
+  // weights: refresh them now, outside any capture.
   if ((e = mlp_fold(s.mlp, st))) return e;
+  float loss = 0.f;
+  if ((e = hipMemcpyAsync(&loss, s.loss0, 4, hipMemcpyDeviceToHost, st))) return e;
   if ((e = hipStreamSynchronize(st))) return e;
   s.trainings++;
-  if (first_loss) s.last_loss = *first_loss;
+  s.last_loss = loss;
+  if (first_loss) *first_loss = loss;
   return hipSuccess;
 }
 
