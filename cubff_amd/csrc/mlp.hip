@@ -363,6 +363,28 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// x of the lane whose index within its 16-lane row is (row index) ^ 8, or
+// the mirror within its 8-lane half (i -> 7 - i), ^ 2, ^ 1 (DPP moves).
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f32(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+                                       0, __builtin_bit_cast(int, x), kCtrl, 0xf, 0xf, false));
+}
+constexpr int kDppRor8 = 0x128, kDppHalfMirror = 0x141, kDppXor2 = 0x4e, kDppXor1 = 0xb1;
+
+// One step of a butterfly reduction over a 16-lane row: the lane holds 2K
+// partial sums, keeps the half selected by `hi` and adds its partner's copy
+// of that half (the partner keeps the other half).
+template <int K, int kCtrl>
+__device__ __forceinline__ void fold_half(float (&v)[16], bool hi) {
+#pragma unroll
+  for (int j = 0; j < K; j++) {
+    const float keep = hi ? v[K + j] : v[j];
+    const float send = hi ? v[j] : v[K + j];
+    v[j] = keep + dpp_f32<kCtrl>(send);
+  }
+}
+
 // Round to bf16 (nearest even) with the hardware conversion.
 __device__ __forceinline__ uint16_t to_bf16(float x) {
   return __builtin_bit_cast(uint16_t, (__bf16)x);
@@ -405,6 +427,9 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
   const int tr = tid >> 3, tpart = tid & 7;
   const uint64_t passes = (count + kInfRows - 1) / kInfRows;
   uint4 next = make_uint4(0, 0, 0, 0);
+  for (int i = tid; i < kInfRows * kCountStride / 4; i += kInfThreads) cnt8[i] = 0;
+  if (tid < kInfRows) ypart[tid] = 0.f;
+  lds_barrier();
   uint64_t p = blockIdx.x;
   if (p < passes && p * kInfRows + tr < count)
     next = *reinterpret_cast<const uint4 *>(tapes + (p * kInfRows + tr) * kTape2 + tpart * 16);
@@ -412,10 +437,8 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
     const uint64_t row0 = p * kInfRows;
     const bool have = row0 + tr < count;
     const uint4 cur = next;
-    // 1. Counts (the previous pass is done with hist, act and ypart).
-    for (int i = tid; i < kInfRows * kCountStride / 4; i += kInfThreads) cnt8[i] = 0;
-    if (tid < kInfRows) ypart[tid] = 0.f;
-    lds_barrier();
+    // 1. Counts into cnt8 (zeroed by the previous pass's conversion, with
+    // barriers since; hist, act and ypart are next written after barriers).
     const uint64_t pn = p + gridDim.x;
     if (pn < passes && pn * kInfRows + tr < count)
       next = *reinterpret_cast<const uint4 *>(tapes + (pn * kInfRows + tr) * kTape2 + tpart * 16);
@@ -432,11 +455,16 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
     // Counts to bf16, once for all waves: thread tid converts 32 counts
     // (row tid / 8, bins 32 (tid % 8) .. +31).
     {
-      const uint32_t *src = cnt8 + tr * (kCountStride / 4) + 8 * tpart;
+      uint32_t *src = cnt8 + tr * (kCountStride / 4) + 8 * tpart;
       uint32_t *dst = reinterpret_cast<uint32_t *>(hist + tr * kRowStride + 32 * tpart);
+      uint32_t cv[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) cv[q] = src[q];
+#pragma unroll
+      for (int q = 0; q < 8; q++) src[q] = 0;  // ready for the next pass
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        const uint32_t v = src[q];
+        const uint32_t v = cv[q];
         const __bf16 h0 = (__bf16)(float)(v & 255u), h1 = (__bf16)(float)((v >> 8) & 255u);
         const __bf16 h2 = (__bf16)(float)((v >> 16) & 255u), h3 = (__bf16)(float)(v >> 24);
         dst[2 * q] = (uint32_t)__builtin_bit_cast(uint16_t, h0) |
@@ -489,20 +517,30 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
           acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, wb2[c][s], acc[t][c], 0, 0, 0);
       }
     }
+    // Row sums of a2 . w3 over this wave's 32 columns: 16 values per lane
+    // (v[4t + r]: row 16t + 4kg + r), reduced across the 16 lanes of the
+    // column group by a butterfly that leaves value r16 in lane r16.
+    float v[16];
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        float v = 0.f;
+        float x = 0.f;
 #pragma unroll
         for (int c = 0; c < 2; c++)
-          v += (float)(__bf16)fmaxf(acc[t][c][r] + b2[c], 0.f) * w3[c];
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-        if (r16 == 0) atomicAdd(&ypart[16 * t + 4 * kg + r], v);
+          x += (float)(__bf16)fmaxf(acc[t][c][r] + b2[c], 0.f) * w3[c];
+        v[4 * t + r] = x;
       }
+    fold_half<8, kDppRor8>(v, r16 & 8);
+    fold_half<4, kDppHalfMirror>(v, r16 & 4);
+    fold_half<2, kDppXor2>(v, r16 & 2);
+    fold_half<1, kDppXor1>(v, r16 & 1);
+    atomicAdd(&ypart[16 * (r16 >> 2) + 4 * kg + (r16 & 3)], v[0]);
     lds_barrier();
-    if (tid < kInfRows && row0 + tid < count) y[row0 + tid] = ypart[tid] + b3;
+    if (tid < kInfRows) {
+      if (row0 + tid < count) y[row0 + tid] = ypart[tid] + b3;
+      ypart[tid] = 0.f;  // only this thread reads it; added to after barriers
+    }
   }
 }
 
