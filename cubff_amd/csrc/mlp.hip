@@ -333,17 +333,20 @@ __global__ void k_targets(const uint32_t *__restrict__ ops, const uint32_t *__re
 // once per workgroup.
 //
 // Workgroup: 8 waves; wave w owns output columns 32 w .. 32 w + 31 of both
-// layers, held as MFMA B fragments (2 column tiles x 8 K-steps of 32, four
-// VGPRs each: 64 VGPRs per layer).  Fragment layout of
-// v_mfma_f32_16x16x32_bf16: lane l holds A(l % 16, 8 (l / 16) + 0..7),
-// B(8 (l / 16) + 0..7, l % 16) and C(4 (l / 16) + 0..3, l % 16).  A pass:
+// layers, held as the MFMA's A fragments (the transposed product
+// C^T = W^T X^T: 2 column tiles x 8 K-steps of 32, four VGPRs each: 64
+// VGPRs per layer).  Fragment layout of v_mfma_f32_16x16x32_bf16: lane l
+// holds A(l % 16, 8 (l / 16) + 0..7), B(8 (l / 16) + 0..7, l % 16) and
+// C(4 (l / 16) + 0..3, l % 16), so a lane's accumulator is four consecutive
+// columns of one row.  A pass:
 //   1. byte counts of 64 tapes into an LDS histogram (four u8 counters per
 //      dword, LDS integer atomics; counts never exceed 128), from tapes
 //      loaded into registers during the previous pass, then converted once,
-//      cooperatively, to a bf16 copy (exact: integers up to 256);
-//   2. a1 = relu(counts (E W1 / 128) + b1): A fragments are 16-byte LDS
-//      reads of the bf16 counts; a1 written to LDS as bf16
-//      (v_cvt_pk_bf16_f32, round to nearest even);
+//      cooperatively, to a bf16 copy (exact: integers up to 256), zeroing
+//      the counters for the next pass;
+//   2. a1 = relu(counts (E W1 / 128) + b1): B fragments are 16-byte LDS
+//      reads of the bf16 counts; a1 written to LDS as bf16, four columns per
+//      8-byte store (v_cvt_pk_bf16_f32, round to nearest even);
 //   3. a2 = relu(a1 W2 + b2), rounded to bf16 as in the unfused path, and
 //      y = a2 . w3 + b3 reduced across lanes and waves in LDS.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -361,28 +364,6 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// x of the lane whose index within its 16-lane row is (row index) ^ 8, or
-// the mirror within its 8-lane half (i -> 7 - i), ^ 2, ^ 1 (DPP moves).
-template <int kCtrl>
-__device__ __forceinline__ float dpp_f32(float x) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
-                                       0, __builtin_bit_cast(int, x), kCtrl, 0xf, 0xf, false));
-}
-constexpr int kDppRor8 = 0x128, kDppHalfMirror = 0x141, kDppXor2 = 0x4e, kDppXor1 = 0xb1;
-
-// One step of a butterfly reduction over a 16-lane row: the lane holds 2K
-// partial sums, keeps the half selected by `hi` and adds its partner's copy
-// of that half (the partner keeps the other half).
-template <int K, int kCtrl>
-__device__ __forceinline__ void fold_half(float (&v)[16], bool hi) {
-#pragma unroll
-  for (int j = 0; j < K; j++) {
-    const float keep = hi ? v[K + j] : v[j];
-    const float send = hi ? v[j] : v[K + j];
-    v[j] = keep + dpp_f32<kCtrl>(send);
-  }
 }
 
 // Round to bf16 (nearest even) with the hardware conversion.
@@ -414,13 +395,19 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
       wb1[c][s] = *reinterpret_cast<const bf16x8 *>(ew1t + off);
       wb2[c][s] = *reinterpret_cast<const bf16x8 *>(w2t + off);
     }
-  float b1[2], b2[2], w3[2];
+  // Weights are the MFMA's A operand and activations its B operand, so the
+  // accumulator of tile (t, c) holds, in lane l, row 16t + l % 16 at the
+  // four consecutive columns col0 + 16c + 4(l / 16) + r, r = 0..3.
+  float b1[2][4], b2[2][4], w3[2][4];
 #pragma unroll
-  for (int c = 0; c < 2; c++) {
-    b1[c] = vec[col0 + 16 * c + r16];
-    b2[c] = vec[kMlpHid + col0 + 16 * c + r16];
-    w3[c] = vec[2 * kMlpHid + col0 + 16 * c + r16];
-  }
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int col = col0 + 16 * c + 4 * kg + r;
+      b1[c][r] = vec[col];
+      b2[c][r] = vec[kMlpHid + col];
+      w3[c][r] = vec[2 * kMlpHid + col];
+    }
   const float b3 = vec[3 * kMlpHid];
 
   // Thread tid loads 16 bytes (chunk tid % 8) of tape tid / 8 of each pass.
@@ -489,19 +476,21 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
                                                             32 * s + 8 * kg);
 #pragma unroll
         for (int c = 0; c < 2; c++)
-          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, wb1[c][s], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb1[c][s], fa, acc[t][c], 0, 0, 0);
       }
     }
 #pragma unroll
     for (int t = 0; t < 4; t++)
 #pragma unroll
       for (int c = 0; c < 2; c++) {
-        const int col = col0 + 16 * c + r16;
+        uint32_t w[2];
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          act[(16 * t + 4 * kg + r) * kRowStride + col] = to_bf16(fmaxf(acc[t][c][r] + b1[c], 0.f));
-          acc[t][c][r] = 0.f;
-        }
+        for (int h = 0; h < 2; h++)
+          w[h] = (uint32_t)to_bf16(fmaxf(acc[t][c][2 * h] + b1[c][2 * h], 0.f)) |
+                 (uint32_t)to_bf16(fmaxf(acc[t][c][2 * h + 1] + b1[c][2 * h + 1], 0.f)) << 16;
+        *reinterpret_cast<uint2 *>(act + (16 * t + r16) * kRowStride + col0 + 16 * c + 4 * kg) =
+            make_uint2(w[0], w[1]);
+        acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
     lds_barrier();
 
@@ -514,28 +503,38 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
                                                             32 * s + 8 * kg);
 #pragma unroll
         for (int c = 0; c < 2; c++)
-          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, wb2[c][s], acc[t][c], 0, 0, 0);
+          acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb2[c][s], fa, acc[t][c], 0, 0, 0);
       }
     }
-    // Row sums of a2 . w3 over this wave's 32 columns: 16 values per lane
-    // (v[4t + r]: row 16t + 4kg + r), reduced across the 16 lanes of the
-    // column group by a butterfly that leaves value r16 in lane r16.
-    float v[16];
+    // Row sums of a2 . w3 over this wave's 32 columns: lane l holds, for
+    // each t, the sum over its 8 columns of row 16t + l % 16; the four lanes
+    // l % 16 + 16 kg of a row add theirs in two exchanges that leave row
+    // 16 kg + l % 16 in lane l.
+    float v[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < 4; t++) {
+      float x = 0.f;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        float x = 0.f;
+      for (int c = 0; c < 2; c++)
 #pragma unroll
-        for (int c = 0; c < 2; c++)
-          x += (float)(__bf16)fmaxf(acc[t][c][r] + b2[c], 0.f) * w3[c];
-        v[4 * t + r] = x;
+        for (int r = 0; r < 4; r++)
+          x += (float)(__bf16)fmaxf(acc[t][c][r] + b2[c][r], 0.f) * w3[c][r];
+      v[t] = x;
+    }
+    {
+      const bool hi = kg & 2;  // partner: lane ^ 32
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const float keep = hi ? v[2 + j] : v[j];
+        const float send = hi ? v[j] : v[2 + j];
+        v[j] = keep + __shfl_xor(send, 32, 64);
       }
-    fold_half<8, kDppRor8>(v, r16 & 8);
-    fold_half<4, kDppHalfMirror>(v, r16 & 4);
-    fold_half<2, kDppXor2>(v, r16 & 2);
-    fold_half<1, kDppXor1>(v, r16 & 1);
-    atomicAdd(&ypart[16 * (r16 >> 2) + 4 * kg + (r16 & 3)], v[0]);
+      const bool lo = kg & 1;  // partner: lane ^ 16
+      const float keep = lo ? v[1] : v[0];
+      const float send = lo ? v[0] : v[1];
+      v[0] = keep + __shfl_xor(send, 16, 64);
+    }
+    atomicAdd(&ypart[16 * kg + r16], v[0]);
     lds_barrier();
     if (tid < kInfRows) {
       if (row0 + tid < count) y[row0 + tid] = ypart[tid] + b3;
