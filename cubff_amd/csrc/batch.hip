@@ -20,7 +20,7 @@
 namespace cubff {
 
 // Soup of a program / slot index: prefix-sum arrays of length num_soups + 1.
-__device__ __forceinline__ uint32_t soup_of(const uint64_t *__restrict__ base,
+__device__ __forceinline__ uint32_t soup_of(const uint64_t *base,
                                             uint32_t num_soups, uint64_t x) {
   uint32_t lo = 0, hi = num_soups;  // base[lo] <= x < base[hi]
   while (hi - lo > 1) {
@@ -30,16 +30,30 @@ __device__ __forceinline__ uint32_t soup_of(const uint64_t *__restrict__ base,
   return lo;
 }
 
+// A workgroup's copy of a prefix-sum array in LDS when it has at most
+// kBaseTable entries (else the array itself): the per-soup lookups of every
+// thread then hit LDS.  Every thread of the block must call it.
+constexpr uint32_t kBaseTable = 256;
+__device__ __forceinline__ const uint64_t *block_bases(const uint64_t *__restrict__ base,
+                                                       uint32_t num_soups, uint64_t *lds) {
+  if (num_soups + 1 > kBaseTable) return base;
+  for (uint32_t i = threadIdx.x; i <= num_soups; i += blockDim.x) lds[i] = base[i];
+  __syncthreads();
+  return lds;
+}
+
 // ---------------------------------------------------------------------------
 // Pairing.
 
 __global__ void k_b_fy_draw(const BatchDev d, uint32_t ahead) {
+  __shared__ uint64_t pbase[kBaseTable];
+  const uint64_t *pb = block_bases(d.prog_base, d.num_soups, pbase);
   const uint64_t epoch = *d.epoch + ahead;
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.total_programs) return;
-  const uint32_t s = soup_of(d.prog_base, d.num_soups, t);
-  const uint64_t b = d.prog_base[s];
-  const uint64_t n = d.prog_base[s + 1] - b;
+  const uint32_t s = soup_of(pb, d.num_soups, t);
+  const uint64_t b = pb[s];
+  const uint64_t n = pb[s + 1] - b;
   const uint64_t tl = t - b;
   const uint64_t r = splitmix64(epoch_seed_pre(d.sm_seed[s], epoch * n + tl));
   const uint32_t j = (uint32_t)(b + r % (tl + 1));
@@ -48,9 +62,11 @@ __global__ void k_b_fy_draw(const BatchDev d, uint32_t ahead) {
 }
 
 // Contiguous item ranges per bucket: a block-wide scan of the counts and
-// one global atomic per block of 1024.
-__global__ __launch_bounds__(1024) void k_b_fy_alloc(const BatchDev d) {
-  __shared__ uint32_t wtot[16], bbase;
+// one global atomic per block of 256 (small blocks: the pairing of the next
+// epoch runs beside the interpreter and must fit between its waves).
+constexpr int kAllocBlock = 256;
+__global__ __launch_bounds__(kAllocBlock) void k_b_fy_alloc(const BatchDev d) {
+  __shared__ uint32_t wtot[kAllocBlock / 64], bbase;
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = p < d.total_programs;
   const uint32_t c = in ? d.fy_cnt[p] : 0u;
@@ -65,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_b_fy_alloc(const BatchDev d) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t run = 0;
-    for (int i = 0; i < 16; i++) {
+    for (int i = 0; i < kAllocBlock / 64; i++) {
       const uint32_t x = wtot[i];
       wtot[i] = run;
       run += x;
@@ -115,6 +131,8 @@ __device__ __forceinline__ uint32_t b_fy_in(const BatchDev &d, uint32_t t) {
 // Writes the pairing as local (per-soup) program indices, like the
 // reference's shuffle_idx.
 __global__ void k_b_fy_final(const BatchDev d, uint32_t ahead) {
+  __shared__ uint64_t pbase[kBaseTable];
+  const uint64_t *pb = block_bases(d.prog_base, d.num_soups, pbase);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.total_programs) return;
   const uint32_t j = d.fy_j[t];
@@ -125,8 +143,8 @@ __global__ void k_b_fy_final(const BatchDev d, uint32_t ahead) {
     const uint32_t c = d.fy_succ[t];
     v = c == kNone ? j : b_fy_in(d, c);
   }
-  const uint32_t s = soup_of(d.prog_base, d.num_soups, t);
-  d.shuf[(*d.epoch + ahead) & 1][t] = (uint32_t)(v - d.prog_base[s]);
+  const uint32_t s = soup_of(pb, d.num_soups, t);
+  d.shuf[(*d.epoch + ahead) & 1][t] = (uint32_t)(v - pb[s]);
 }
 
 __global__ void k_b_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t count) {
@@ -139,15 +157,18 @@ __global__ void k_b_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t count) {
 // bytes [16q, 16q + 16) of program s[2k] (q < 4) or s[2k+1] (q >= 4) and
 // applies the mutation draws of those positions.
 __global__ void k_b_stage(const BatchDev d) {
+  __shared__ uint64_t pbase[kBaseTable], sbase[kBaseTable];
+  const uint64_t *prog_base = block_bases(d.prog_base, d.num_soups, pbase);
+  const uint64_t *slot_base = block_bases(d.slot_base, d.num_soups, sbase);
   const uint64_t epoch = *d.epoch;
   const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= d.total_slots * 8) return;
   const uint64_t g = u >> 3;
   const uint32_t q = (uint32_t)(u & 7);
-  const uint32_t s = soup_of(d.slot_base, d.num_soups, g);
-  const uint64_t k = g - d.slot_base[s];
-  const uint64_t pb = d.prog_base[s];
-  const uint64_t n = d.prog_base[s + 1] - pb;
+  const uint32_t s = soup_of(slot_base, d.num_soups, g);
+  const uint64_t k = g - slot_base[s];
+  const uint64_t pb = prog_base[s];
+  const uint64_t n = prog_base[s + 1] - pb;
   const uint32_t p = d.shuf[epoch & 1][pb + 2 * k + (q >> 2)];
   uint4 v = reinterpret_cast<const uint4 *>(d.soup + (pb + p) * kTape)[q & 3];
   uint32_t w[4] = {v.x, v.y, v.z, v.w};
@@ -268,7 +289,7 @@ hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d, uint32_t ahead)
   const unsigned g = nblk(n, 256);
   hipLaunchKernelGGL(k_b_fill, dim3(nblk(n + 1, 256)), dim3(256), 0, st, d.fy_cnt, 0u, n + 1);
   hipLaunchKernelGGL(k_b_fy_draw, dim3(g), dim3(256), 0, st, d, ahead);
-  hipLaunchKernelGGL(k_b_fy_alloc, dim3(nblk(n, 1024)), dim3(1024), 0, st, d);
+  hipLaunchKernelGGL(k_b_fy_alloc, dim3(nblk(n, kAllocBlock)), dim3(kAllocBlock), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_place, dim3(g), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_sort, dim3(g), dim3(256), 0, st, d);
   hipLaunchKernelGGL(k_b_fy_final, dim3(g), dim3(256), 0, st, d, ahead);

@@ -92,13 +92,14 @@ void drop_graph(cubff_batch *b) {
 // launches with events.
 int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   const hipStream_t st = b->stream;
-  // The next epoch's pairing, beside this one (it reads and writes only the
-  // other pairing buffer and the pairing scratch).
+  HIP_TRY(batch_stage(st, b->d));
+  // The next epoch's pairing, beside the rest of this one (it reads and
+  // writes only the other pairing buffer and the pairing scratch); forked
+  // after the stage, whose memory traffic it would otherwise slow down.
   HIP_TRY(hipEventRecord(b->fork, st));
   HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
   HIP_TRY(batch_pair_shuffle(b->stream2, b->d, 1));
   HIP_TRY(hipEventRecord(b->join, b->stream2));
-  HIP_TRY(batch_stage(st, b->d));
   const uint32_t *list = b->plain_list;
   if (sched == CUBFF_SCHED_LEARNED) {
     HIP_TRY(sched_order(st, b->sched, b->d, b->plain_list, b->order_list, b->lists.data(),
