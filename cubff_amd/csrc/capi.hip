@@ -123,6 +123,10 @@ constexpr double kFlowPatienceS = 2e-6;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
 constexpr double kResolverIdleS = 0.1;
 constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
+// Deferring segments end by waiting for the resolver to finish the last
+// epochs' long interactions: long segments (bounded by kFlowBytes of list
+// entries, 16 bytes per program and epoch) amortize that wait.
+constexpr uint64_t kDeferMaxEpochs = 1024;
 
 // Which multi-epoch path the default picks (measured on MI355X at 2^17
 // programs, 64 epochs, against one launch per epoch).  Deferring epoch
@@ -151,7 +155,7 @@ bool flow_pays(int lang, uint64_t n) {
 // Deferred-interaction segments: the list holds every pair slot of the
 // segment in the worst case (32 bytes each), within kFlowBytes.
 uint64_t defer_segment_epochs(uint64_t n) {
-  return std::max<uint64_t>(2, std::min(kFlowBytes / (16 * n), kFlowMaxEpochs));
+  return std::max<uint64_t>(2, std::min(kFlowBytes / (16 * n), kDeferMaxEpochs));
 }
 
 uint64_t flow_segment_epochs(uint64_t n) {

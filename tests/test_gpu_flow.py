@@ -169,6 +169,26 @@ def test_defer_equals_epoch_loop_at_2p17(capi):
                pipeline=capi.FLOW_DEFER)
 
 
+def test_defer_segment_boundary_at_2p18(capi):
+    # 2^18 programs: deferring segments of 256 epochs (their entry lists
+    # hold 16 bytes per program and epoch, up to 1 GiB), so 300 epochs cross
+    # a segment boundary.  Same soup, counters and last results as the
+    # epoch-by-epoch loop.
+    n, seed, P = 1 << 18, 3, 1 << 18
+    print("defer over a segment boundary at", n, flush=True)
+    out = []
+    for pipeline in (capi.FLOW_DEFER, capi.FLOW_OFF):
+        with capi.Soup("bff_noheads", n) as g:
+            g.set_pipeline(pipeline)
+            g.init_programs(O.epoch_seed(seed, 0))
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 300, P)
+            out.append((_sha(g.read_soup()), g.read_counters(), _sha(g.read_pair_ops()[0]),
+                        _sha(g.read_pair_ops()[1])))
+            if pipeline == capi.FLOW_DEFER:
+                assert g.flow_stats()[2] > 0
+    assert out[0] == out[1]
+
+
 def test_defer_with_serialized_dispatch(tmp_path):
     # When kernels cannot run side by side (AMD_SERIALIZE_KERNEL=3, as under
     # some profilers), the resolver leaves after a short idle wait and the
