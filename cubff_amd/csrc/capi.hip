@@ -45,8 +45,9 @@ struct cubff_ctx {
   // the list (defer_cap entries), run generation, command budget.
   hipStream_t stream2 = nullptr;
   hipStream_t stream3 = nullptr;  // pairings drawn ahead
-  uint32_t *shuf2 = nullptr;      // second pairing buffer
-  uint64_t *num_indices2 = nullptr;  // its pair count
+  // Pairing buffers drawn ahead of the current one, with their pair counts.
+  uint32_t *shuf_ahead[2] = {nullptr, nullptr};
+  uint64_t *nidx_ahead[2] = {nullptr, nullptr};
   unsigned long long *hist = nullptr;  // byte histogram, 256 entries
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
@@ -204,20 +205,22 @@ int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint
   }
 }
 
-// Pairings drawn one epoch ahead on a third stream into two buffers (the
-// context's shuf and shuf2, with their pair counts), so that drawing epoch
-// j+1's pairing overlaps epoch j's kernel: no pairing depends on the soup.
-// Epoch j's kernel waits for its pairing; the pairing of epoch j+2 waits for
-// epoch j's kernel, which reads the same buffer.  Epoch j's kernel is queued
-// before pairing j+1 is drawn, since the 2-D pairing blocks the host while
-// its rounds converge.
+// Pairings drawn two epochs ahead on a third stream into three buffers (the
+// context's shuf and shuf_ahead, with their pair counts): no pairing depends
+// on the soup, and beside an epoch kernel and the resolver the pairing
+// kernels run several times slower than alone, so one epoch of lead is not
+// always enough.  Epoch j's kernel waits for its pairing; the pairing of
+// epoch j+3 waits for epoch j's kernel, which reads the same buffer.  Epoch
+// j's kernel is queued before pairing j+2 is drawn, since the 2-D pairing
+// blocks the host while its rounds converge.
 struct PairAhead {
+  static constexpr uint32_t kBufs = 3;  // pairings are drawn kBufs - 1 epochs ahead
   cubff_ctx *c;
   int mode;
   uint64_t seed, first;
-  uint32_t *buf[2] = {nullptr, nullptr};
-  uint64_t *nidx[2] = {nullptr, nullptr};  // pairs evaluated, per buffer
-  hipEvent_t drawn[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
+  uint32_t *buf[kBufs] = {};
+  uint64_t *nidx[kBufs] = {};  // pairs evaluated, per buffer
+  hipEvent_t drawn[kBufs] = {}, used[kBufs] = {};
 
   int init() {
     int r;
@@ -226,46 +229,70 @@ struct PairAhead {
       if (e != hipSuccess)
         return fail(CUBFF_EHIP, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
     }
-    if (!c->shuf2 && (r = dev_alloc(&c->shuf2, c->n))) return r;
-    if (!c->num_indices2 && (r = dev_alloc(&c->num_indices2, 1))) return r;
+    for (uint32_t i = 0; i + 1 < kBufs; i++) {
+      if (!c->shuf_ahead[i] && (r = dev_alloc(&c->shuf_ahead[i], c->n))) return r;
+      if (!c->nidx_ahead[i] && (r = dev_alloc(&c->nidx_ahead[i], 1))) return r;
+    }
     buf[0] = c->shuf;
-    buf[1] = c->shuf2;
     nidx[0] = c->num_indices;
-    nidx[1] = c->num_indices2;
-    for (int i = 0; i < 2; i++) {
+    for (uint32_t i = 1; i < kBufs; i++) {
+      buf[i] = c->shuf_ahead[i - 1];
+      nidx[i] = c->nidx_ahead[i - 1];
+    }
+    for (uint32_t i = 0; i < kBufs; i++) {
       HIP_TRY(hipEventCreateWithFlags(&drawn[i], hipEventDisableTiming));
       HIP_TRY(hipEventCreateWithFlags(&used[i], hipEventDisableTiming));
     }
-    // The stream of the first pairing must see everything queued so far.
-    HIP_TRY(hipEventRecord(used[1], c->stream));
-    HIP_TRY(hipStreamWaitEvent(c->stream3, used[1], 0));
+    // The stream of the first pairings must see everything queued so far.
+    HIP_TRY(hipEventRecord(used[0], c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->stream3, used[0], 0));
     return CUBFF_OK;
   }
+  // Pairing of epoch first + j into buffer j % kBufs, after the kernel that
+  // last read that buffer (epoch j - kBufs).
   int draw(uint32_t j) {
     int r;
-    if (j >= 2) HIP_TRY(hipStreamWaitEvent(c->stream3, used[j % 2], 0));
-    if ((r = pair_into(c, mode, seed, first + j, buf[j % 2], c->stream3, nidx[j % 2]))) return r;
-    if (mode != CUBFF_PAIR_2D) HIP_TRY(launch_set_u64(c->stream3, nidx[j % 2], c->n / 2));
-    HIP_TRY(hipEventRecord(drawn[j % 2], c->stream3));
+    const uint32_t b = j % kBufs;
+    if (j >= kBufs) HIP_TRY(hipStreamWaitEvent(c->stream3, used[b], 0));
+    if ((r = pair_into(c, mode, seed, first + j, buf[b], c->stream3, nidx[b]))) return r;
+    if (mode != CUBFF_PAIR_2D) HIP_TRY(launch_set_u64(c->stream3, nidx[b], c->n / 2));
+    HIP_TRY(hipEventRecord(drawn[b], c->stream3));
     return CUBFF_OK;
   }
+  // Draws of the first epochs, before any kernel.
+  int start(uint32_t epochs) {
+    int r;
+    for (uint32_t j = 0; j + 1 < kBufs && j < epochs; j++)
+      if ((r = draw(j))) return r;
+    return CUBFF_OK;
+  }
+  // After epoch j's kernel is queued: the pairing kBufs - 1 epochs on.
+  int next(uint32_t j, uint32_t epochs) {
+    const uint32_t k = j + kBufs - 1;
+    return k < epochs ? draw(k) : CUBFF_OK;
+  }
   int before_kernel(uint32_t j) {
-    HIP_TRY(hipStreamWaitEvent(c->stream, drawn[j % 2], 0));
+    HIP_TRY(hipStreamWaitEvent(c->stream, drawn[j % kBufs], 0));
     return CUBFF_OK;
   }
   int after_kernel(uint32_t j) {
-    HIP_TRY(hipEventRecord(used[j % 2], c->stream));
+    HIP_TRY(hipEventRecord(used[j % kBufs], c->stream));
     return CUBFF_OK;
   }
-  // The last epoch's pairing becomes the context's current one.
+  // The last epoch's pairing becomes the context's current one; the other
+  // buffers stay with the context.
   void finish(uint32_t epochs) {
-    if (epochs && (epochs - 1) % 2 == 1) {
-      std::swap(c->shuf, c->shuf2);
-      std::swap(c->num_indices, c->num_indices2);
+    if (!epochs) return;
+    const uint32_t last = (epochs - 1) % kBufs;
+    c->shuf = buf[last];
+    c->num_indices = nidx[last];
+    for (uint32_t i = 1; i < kBufs; i++) {
+      c->shuf_ahead[i - 1] = buf[(last + i) % kBufs];
+      c->nidx_ahead[i - 1] = nidx[(last + i) % kBufs];
     }
   }
   ~PairAhead() {
-    for (int i = 0; i < 2; i++) {
+    for (uint32_t i = 0; i < kBufs; i++) {
       if (drawn[i]) (void)hipEventDestroy(drawn[i]);
       if (used[i]) (void)hipEventDestroy(used[i]);
     }
@@ -291,17 +318,18 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
     return fail(CUBFF_EHIP, std::string("epoch setup failed: ") + hipGetErrorString(e));
   }
   const uint64_t sm = splitmix64(params_seed);
-  r = pa.draw(0);
+  r = pa.start(epochs);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
     if ((r = pa.before_kernel(j))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
     if (e == hipSuccess)
-      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % 2], c->n, pa.nidx[j % 2],
+      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % PairAhead::kBufs], c->n,
+                       pa.nidx[j % PairAhead::kBufs],
                        epoch_seed_pre(sm, first + j), prob, c->pair_ops, c->pair_steps,
                        c->counters);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
     if (e == hipSuccess) r = pa.after_kernel(j);
-    if (r == CUBFF_OK && e == hipSuccess && j + 1 < epochs) r = pa.draw(j + 1);
+    if (r == CUBFF_OK && e == hipSuccess) r = pa.next(j, epochs);
   }
   hipError_t e2 = hipStreamSynchronize(c->stream3);
   if (e == hipSuccess) e = e2;
@@ -445,15 +473,16 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   }
   PairAhead pa{c, mode, params_seed, first};
   r = pa.init();
-  if (r == CUBFF_OK) r = pa.draw(0);
+  if (r == CUBFF_OK) r = pa.start(epochs);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
     if ((r = pa.before_kernel(j))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
     if (e == hipSuccess)
-      e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % 2], pa.nidx[j % 2], j);
+      e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % PairAhead::kBufs],
+                             pa.nidx[j % PairAhead::kBufs], j);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
     if (e == hipSuccess) r = pa.after_kernel(j);
-    if (r == CUBFF_OK && e == hipSuccess && j + 1 < epochs) r = pa.draw(j + 1);
+    if (r == CUBFF_OK && e == hipSuccess) r = pa.next(j, epochs);
   }
   // Close the list whatever happened, so the resolver leaves.
   hipError_t e2 = launch_defer_close(c->stream, c->dctl);
@@ -704,8 +733,10 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream3) (void)hipStreamSynchronize(c->stream3);
   if (c->stream3) (void)hipStreamDestroy(c->stream3);
-  dev_free(c->shuf2);
-  dev_free(c->num_indices2);
+  for (int i = 0; i < 2; i++) {
+    dev_free(c->shuf_ahead[i]);
+    dev_free(c->nidx_ahead[i]);
+  }
   dev_free(c->hist);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -353,7 +353,7 @@ struct BffLang {
             ((k == kCopy10 ? v1 : v0) + ((kTDelta >> (2 * k)) & 3) - 1) & 0xff;
         const bool changed = (k - kPlus) < 4u && wval != wold;
         t.set(changed ? waddr : T::kDummy, (uint8_t)wval);
-        if constexpr (kAccel) {
+        if constexpr (kAccel && !kBudget) {
           if (changed) cdelta += (wval - wold) * cycle_weight(waddr);
         }
         h0 = (h0 + (int)((kTH0 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
@@ -459,7 +459,11 @@ struct BffLang {
                 aw0 = v0;
                 aw1 = v1;
                 wrote = moved = branched = false;
-                if (i1 >= kCycleStart && cq != -2) {
+                // Not under a command budget: cycles that start after step
+                // kCycleStart have periods of ~1.5K steps and more, and a
+                // budgeted run is abandoned well before it could skip one;
+                // the run that completes it checks instead.
+                if (!kBudget && i1 >= kCycleStart && cq != -2) {
                   bool same = pc == cq && h0 == ch0 && h1 == ch1 && cdelta == 0;
                   for (int w = 0; same && w < kTape2 / 4; w++) same = t.word(w) == aux.word(w);
                   if (same) {
