@@ -241,6 +241,10 @@ struct BffLang {
   static constexpr uint32_t kTDelta = 0x55525;
   // Loop acceleration needs the standard `[` / `]` semantics.
   static constexpr bool kAccel = LOGIC == kLogicStd;
+  // Static-tape cycle skip where it measured faster (bff_selfmove: ~2.6x
+  // per epoch with deferral; bff8's long runs mostly cycle with a changing
+  // tape, and the check only costs there).
+  static constexpr bool kStaticCycle = LOGIC == kLogicSelfmove;
 
   // Evaluate (bff.inc.h:356-389) with EvaluateOne (bff.inc.h:289-349,
   // bff8.cu:23-65, bff_selfmove.cu:44-98).
@@ -326,6 +330,12 @@ struct BffLang {
       int cq = -1, ch0 = 0, ch1 = 0;
       uint32_t ci = 0, cskip = 0, cwin = kCycleFirstWin, cdelta = 0;
       bool cfirst = true;
+      // Static-tape cycle skip at command boundaries (kStaticCycle), as for
+      // SUBLEQ, on the state (pc, heads, tape): anchor, its step and no-op
+      // counts, window, and whether a write changed a byte since.
+      int spc = -1, sh0 = 0, sh1 = 0;
+      uint32_t si = 0, sskip = 0, swin = 1;
+      bool schg = false, sdone = false;
       for (;;) {
         iters++;
         if constexpr (kBudget) {
@@ -500,6 +510,28 @@ struct BffLang {
         nskip += adv;
         if (adv == rem || nextc >= kTape2) break;
         pc = nextc;
+        if constexpr (kStaticCycle) {
+          schg |= changed;
+          if (!sdone) {
+            if (!schg && pc == spc && h0 == sh0 && h1 == sh1) {
+              // The state is the anchor's: skip whole periods.
+              sdone = true;
+              const uint32_t len = i - si;
+              const uint32_t reps = (stepcount - i) / len;
+              i += reps * len;
+              nskip += reps * (nskip - sskip);
+              if (i >= stepcount) break;
+            } else if (schg || i - si >= swin) {
+              swin = schg ? 1 : min(swin * 2, 1u << 30);
+              spc = pc;
+              sh0 = h0;
+              sh1 = h1;
+              si = i;
+              sskip = nskip;
+              schg = false;
+            }
+          }
+        }
       }
     }
     *steps = i;
@@ -801,6 +833,41 @@ struct ForthLang {
 // SUBLEQ (subleq.cu:109-133) and RSUBLEQ4 (rsubleq4.cu:116-141).  Every
 // iteration counts as an op.
 
+// Static-tape cycle skip (exact), used by SUBLEQ.  The state of a run
+// is (pos, tape).  An anchor holds a position and its step count, and a
+// flag says whether a write changed a byte since.  When pos comes back to
+// the anchor's with no byte changed, the state is the anchor's again: the
+// run repeats with period L = i - anchor step, and whole periods up to the
+// step cap are counted without being run (every step of these languages
+// counts as an op).  The anchor moves to the current step after any
+// changing write and after windows of 1, 2, 4, ... steps (Brent), so a
+// cycle on a settled tape is found within about two periods plus its lead
+// in.  Runs that keep changing the tape are not skipped (still exact).
+struct StaticCycle {
+  int apos = -1;
+  uint32_t ai = 0, win = 1;
+  bool chg = false, done = false;
+  // After a step that left the run at `pos` with `i` steps taken; returns
+  // the steps to skip.
+  __device__ __forceinline__ uint32_t after_step(int pos, uint32_t i, uint32_t stepcount,
+                                                 bool changed) {
+    chg |= changed;
+    if (done) return 0;
+    if (!chg && pos == apos) {
+      done = true;
+      const uint32_t len = i - ai;
+      return (stepcount - i) / len * len;
+    }
+    if (chg || i - ai >= win) {
+      win = chg ? 1 : min(win * 2, 1u << 30);
+      apos = pos;
+      ai = i;
+      chg = false;
+    }
+    return 0;
+  }
+};
+
 struct SubleqLang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <int LANES> using Aux = LdsTape<LANES>;  // unused
@@ -809,29 +876,32 @@ struct SubleqLang {
                                       uint32_t *steps, uint32_t *work = nullptr,
                                       uint32_t budget = 0) {
     int pos = 0;
-    uint32_t i = 0;
+    uint32_t i = 0, run = 0;  // steps taken, steps executed
+    StaticCycle cyc;
     while (i < stepcount) {
       i++;
+      run++;
       if constexpr (kBudget) {
-        if (i > budget) {
+        if (run > budget) {
           *steps = kOverBudget;
           return 0;
         }
       }
       uint32_t sa = t.get(pos) % kTape2;
       uint32_t sb = t.get(pos + 1) % kTape2;
-      uint8_t r = (uint8_t)(t.get(sa) - t.get(sb));
+      const uint32_t vb = t.get(sb);
+      uint8_t r = (uint8_t)(t.get(sa) - vb);
       t.set(sa, r);
       if ((r & 0x80) || r == 0) pos = t.get(pos + 2);  // read after the write
       else pos += 3;
       if (pos + 3 > kTape2 || pos < 0) break;
+      i += cyc.after_step(pos, i, stepcount, vb != 0);  // changed iff b != 0
     }
     *steps = i;
-    if (work) *work = i;
+    if (work) *work = run;
     return i;
   }
 };
-
 struct Rsubleq4Lang {
   template <int LANES> using Tape = LdsTape<LANES>;
   template <int LANES> using Aux = LdsTape<LANES>;  // unused
@@ -839,6 +909,8 @@ struct Rsubleq4Lang {
   __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
                                       uint32_t *steps, uint32_t *work = nullptr,
                                       uint32_t budget = 0) {
+    // No cycle skip: its long runs mostly cycle with a changing tape, and
+    // the static-tape check measured slower per epoch.
     int pos = 0;
     uint32_t i = 0;
     while (i < stepcount) {

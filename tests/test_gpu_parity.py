@@ -48,6 +48,30 @@ def test_eval_tapes_step_limits(capi):
         assert np.array_equal(g_t[0], f)
 
 
+@pytest.mark.parametrize("lang", ["bff8", "bff8_noheads", "bff_selfmove", "subleq", "rsubleq4"])
+def test_static_cycle_skip_vs_oracle(capi, lang):
+    # Runs that end in a cycle on a settled tape are skipped whole periods at
+    # a time; the results at any step limit must equal the step-by-step
+    # oracle's.  Tapes whose oracle run reaches 8192 steps, at limits that
+    # cut periods at arbitrary points.
+    from tests.tapegen import biased_tapes
+    long = []
+    for seed in range(17, 27):  # SUBLEQ's capped runs are rare: draw until enough
+        long += [t for t in biased_tapes(lang, 1500, seed=seed)
+                 if O.evaluate(lang, t, 8192)[2] >= 8192]
+        if len(long) >= 12:
+            break
+    long = long[:40]
+    assert len(long) >= 5, len(long)
+    batch = np.stack(long)
+    for steps in (8192, 8191, 5003, 4096, 1000, 37, 30000):
+        g_t, g_ops, g_st = capi.eval_tapes(lang, batch, steps=steps)
+        for i, t in enumerate(long):
+            f, ops, st = O.evaluate(lang, t, steps)
+            assert (g_ops[i], g_st[i]) == (ops, st), (lang, i, steps)
+            assert np.array_equal(g_t[i], f), (lang, i, steps)
+
+
 @pytest.mark.parametrize("lang", LANGS)
 def test_epochs_shuffle_vs_oracle(capi, lang):
     n, seed, P = 2048, 10248, 1 << 18
