@@ -132,22 +132,23 @@ constexpr uint64_t kDeferMaxEpochs = 1024;
 // Which multi-epoch path the default picks (measured on MI355X at 2^17
 // programs, 32-64 epochs, against one launch per epoch).  Deferring epoch
 // kernels: bff 2.0x, bff_perm 1.8x, bff_noheads 1.4-1.6x, subleq 1.3x,
-// bff_selfmove 1.4x -- languages whose interactions rarely exceed the
-// budget, or whose long runs the static-tape cycle skip ends early.
-// Dataflow kernel: the Forth family 1.6-2.2x, bff_noheads_4bit 1.3x,
-// bff8_noheads 1.1x -- long interactions are common there and would mostly
-// be deferred.  Up to 2^18 programs; at 2^22 the epochs are throughput-bound
+// bff_selfmove 1.4x, rsubleq4 2.4x -- languages whose interactions rarely
+// exceed the budget, or whose long runs a cycle skip ends early.
+// Dataflow kernel: the Forth family 1.6-2.2x, bff_noheads_4bit 1.3x --
+// long interactions are common there and would mostly be deferred.  bff8
+// and bff8_noheads (whose capped runs the full-state cycle skip ends early)
+// are fastest with one launch per epoch.  Up to 2^18 programs; at 2^22 the epochs are throughput-bound
 // and one launch per epoch is used.
 bool defer_pays(int lang, uint64_t n) {
   if (n > (1ULL << 18)) return false;
   return lang == kBff || lang == kBffNoheads || lang == kBffPerm || lang == kSubleq ||
-         lang == kBffSelfmove;
+         lang == kBffSelfmove || lang == kRsubleq4;
 }
 
 bool flow_pays(int lang, uint64_t n) {
   if (n > (1ULL << 18)) return false;
   switch (lang) {
-    case kBffNoheads4bit: case kBff8Noheads:
+    case kBffNoheads4bit:
     case kForth: case kForthcopy: case kForthtrivial: case kForthtrivialReset:
       return true;
     default:

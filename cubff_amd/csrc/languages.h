@@ -44,7 +44,8 @@ constexpr uint32_t kOverBudget = 0xffffffffu;
 // stack, and the tape snapshot of the BFF cycle check.
 constexpr bool lang_needs_aux(int lang) {
   return (lang >= kForth && lang <= kForthtrivialReset) || lang == kBff ||
-         lang == kBffNoheads || lang == kBffNoheads4bit || lang == kBffPerm;
+         lang == kBffNoheads || lang == kBffNoheads4bit || lang == kBffPerm ||
+         lang == kBff8 || lang == kBff8Noheads || lang == kRsubleq4;
 }
 
 // ---------------------------------------------------------------------------
@@ -245,6 +246,16 @@ struct BffLang {
   // per epoch with deferral; bff8's long runs mostly cycle with a changing
   // tape, and the check only costs there).
   static constexpr bool kStaticCycle = LOGIC == kLogicSelfmove;
+  // Full-state cycle skip at command boundaries for the bff8 jump logic:
+  // its step-capped runs are all periodic (periods mostly 8 steps, up to
+  // ~1K; lead-ins ~50 steps, up to ~2K), many with a changing tape.  From
+  // kFullStart steps on, the state at a command boundary (pc, heads, tape)
+  // is compared with an anchor whose tape is kept in the aux region, after
+  // a weighted sum of the writes since the anchor (zero when the tape is
+  // back to the anchor's) filters candidates; anchors move after windows of
+  // kFullStart, 2 kFullStart, ... steps (Brent).
+  static constexpr bool kFullCycle = LOGIC == kLogicBff8;
+  static constexpr uint32_t kFullStart = 64;
 
   // Evaluate (bff.inc.h:356-389) with EvaluateOne (bff.inc.h:289-349,
   // bff8.cu:23-65, bff_selfmove.cu:44-98).
@@ -336,6 +347,11 @@ struct BffLang {
       int spc = -1, sh0 = 0, sh1 = 0;
       uint32_t si = 0, sskip = 0, swin = 1;
       bool schg = false, sdone = false;
+      // Full-state cycle skip (kFullCycle): anchor pc (-1 none yet), heads,
+      // step and no-op counts, window, weighted sum of changes since.
+      int fpc = -1, fh0 = 0, fh1 = 0;
+      uint32_t fi = 0, fskip = 0, fwin = kFullStart, fdelta = 0;
+      bool fdone = false;
       for (;;) {
         iters++;
         if constexpr (kBudget) {
@@ -365,6 +381,9 @@ struct BffLang {
         t.set(changed ? waddr : T::kDummy, (uint8_t)wval);
         if constexpr (kAccel && !kBudget) {
           if (changed) cdelta += (wval - wold) * cycle_weight(waddr);
+        }
+        if constexpr (kFullCycle) {
+          if (changed) fdelta += (wval - wold) * cycle_weight(waddr);
         }
         h0 = (h0 + (int)((kTH0 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
         h1 = (h1 + (int)((kTH1 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
@@ -510,6 +529,30 @@ struct BffLang {
         nskip += adv;
         if (adv == rem || nextc >= kTape2) break;
         pc = nextc;
+        if constexpr (kFullCycle) {
+          if (!fdone && i >= kFullStart) {
+            bool same = pc == fpc && h0 == fh0 && h1 == fh1 && fdelta == 0;
+            for (int w = 0; same && w < kTape2 / 4; w++) same = t.word(w) == aux.word(w);
+            if (same) {
+              // The state is the anchor's: skip whole periods.
+              fdone = true;
+              const uint32_t len = i - fi;
+              const uint32_t reps = (stepcount - i) / len;
+              i += reps * len;
+              nskip += reps * (nskip - fskip);
+              if (i >= stepcount) break;
+            } else if (fpc < 0 || i - fi >= fwin) {
+              if (fpc >= 0) fwin = min(fwin * 2, 1u << 30);
+              for (int w = 0; w < kTape2 / 4; w++) aux.set_word(w, t.word(w));
+              fpc = pc;
+              fh0 = h0;
+              fh1 = h1;
+              fi = i;
+              fskip = nskip;
+              fdelta = 0;
+            }
+          }
+        }
         if constexpr (kStaticCycle) {
           schg |= changed;
           if (!sdone) {
@@ -904,19 +947,34 @@ struct SubleqLang {
 };
 struct Rsubleq4Lang {
   template <int LANES> using Tape = LdsTape<LANES>;
-  template <int LANES> using Aux = LdsTape<LANES>;  // unused
+  template <int LANES> using Aux = LdsTape<LANES>;  // cycle-check anchor tape
+  // Full-state cycle skip (exact): the step-capped runs are periodic with
+  // short periods (up to ~20 steps after lead-ins of ~20), usually with a
+  // changing tape.  From kFullStart steps on, the state (pos, tape) is
+  // compared with an anchor whose tape is kept in the aux region, after a
+  // weighted sum of the changes since the anchor (zero when the tape is back
+  // to the anchor's) filters candidates; anchors move after windows of
+  // kFullStart, 2 kFullStart, ... steps (Brent).  A match means the run
+  // repeats with that period: whole periods up to the step cap are counted
+  // without being run.
+  static constexpr uint32_t kFullStart = 8;
+  __device__ __forceinline__ static uint32_t weight(uint32_t p) {
+    return 2654435761u * (2u * p + 1u);
+  }
   template <bool kBudget = false, class T, class A>
-  __device__ static uint32_t evaluate(const T &t, const A &, uint32_t stepcount,
+  __device__ static uint32_t evaluate(const T &t, const A &aux, uint32_t stepcount,
                                       uint32_t *steps, uint32_t *work = nullptr,
                                       uint32_t budget = 0) {
-    // No cycle skip: its long runs mostly cycle with a changing tape, and
-    // the static-tape check measured slower per epoch.
     int pos = 0;
-    uint32_t i = 0;
+    uint32_t i = 0, run = 0;  // steps taken, steps executed
+    int fpos = -1;
+    uint32_t fi = 0, fwin = kFullStart, fdelta = 0;
+    bool fdone = false;
     while (i < stepcount) {
       i++;
+      run++;
       if constexpr (kBudget) {
-        if (i > budget) {
+        if (run > budget) {
           *steps = kOverBudget;
           return 0;
         }
@@ -924,14 +982,31 @@ struct Rsubleq4Lang {
       uint32_t sa = (uint32_t)(pos + t.get(pos)) % kTape2;
       uint32_t sb = (uint32_t)(pos + t.get(pos + 1)) % kTape2;
       uint32_t sc = (uint32_t)(pos + t.get(pos + 2)) % kTape2;
+      const uint32_t old = t.get(sa);
       uint8_t r = (uint8_t)(t.get(sb) - t.get(sc));
       t.set(sa, r);
+      fdelta += (r - old) * weight(sa);
       if ((r & 0x80) || r == 0) pos += (int8_t)t.get(pos + 3);  // after write
       else pos += 4;
       if (pos + 4 > kTape2 || pos < 0) break;
+      if (!fdone && i >= kFullStart) {
+        bool same = pos == fpos && fdelta == 0;
+        for (int w = 0; same && w < kTape2 / 4; w++) same = t.word(w) == aux.word(w);
+        if (same) {
+          fdone = true;
+          const uint32_t len = i - fi;
+          i += (stepcount - i) / len * len;
+        } else if (fpos < 0 || i - fi >= fwin) {
+          if (fpos >= 0) fwin = min(fwin * 2, 1u << 30);
+          for (int w = 0; w < kTape2 / 4; w++) aux.set_word(w, t.word(w));
+          fpos = pos;
+          fi = i;
+          fdelta = 0;
+        }
+      }
     }
     *steps = i;
-    if (work) *work = i;
+    if (work) *work = run;
     return i;
   }
 };
