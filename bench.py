@@ -133,9 +133,9 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
         b.sync()
         e0 = warmup
         if autotune:
-            wg, tpl, table = b.autotune(e0, 1, sched)
+            wg, tpl, table = b.autotune(e0, 3, sched)
             tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
-            e0 += 12
+            e0 += 36
         b.reset_counters()
         tot, it = b.run_epochs_timed(e0, epochs, sched)
         ops = steps = 0

@@ -418,29 +418,31 @@ int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch, uint64_t epochs_p
   int r;
   if ((r = check(b))) return r;
   static const int kWg[3] = {64, 128, 256};
-  float best = 1e30f;
-  int bw = b->wg;
-  uint32_t bt = b->per_lane;
+  constexpr int kShapes = 12;
+  // Rounds of one epoch per shape, so that every shape sees the soups at
+  // about the same stage of their evolution; times summed per shape.
+  float sum[kShapes] = {};
   uint64_t e = first_epoch;
-  for (int wi = 0; wi < 3; wi++) {
-    for (uint32_t tpl = 1; tpl <= 4; tpl++) {
-      b->wg = kWg[wi];
-      b->per_lane = tpl;
+  for (uint64_t round = 0; round < epochs_per_config; round++) {
+    for (int sh = 0; sh < kShapes; sh++) {
+      b->wg = kWg[sh / 4];
+      b->per_lane = (uint32_t)(sh % 4) + 1;
       float tot = 0, it = 0;
-      if ((r = cubff_batch_run_epochs_timed(b, e, epochs_per_config, sched, &tot, &it))) return r;
-      e += epochs_per_config;
-      if (ms_table) ms_table[wi * 4 + (tpl - 1)] = tot / epochs_per_config;
-      if (tot < best) {
-        best = tot;
-        bw = kWg[wi];
-        bt = tpl;
-      }
+      if ((r = cubff_batch_run_epochs_timed(b, e, 1, sched, &tot, &it))) return r;
+      e++;
+      sum[sh] += tot;
     }
   }
-  b->wg = bw;
-  b->per_lane = bt;
-  if (best_wg) *best_wg = bw;
-  if (best_tapes_per_lane) *best_tapes_per_lane = bt;
+  int best = 0;
+  for (int sh = 1; sh < kShapes; sh++)
+    if (sum[sh] < sum[best]) best = sh;
+  b->wg = kWg[best / 4];
+  b->per_lane = (uint32_t)(best % 4) + 1;
+  if (ms_table)
+    for (int sh = 0; sh < kShapes; sh++)
+      ms_table[sh] = epochs_per_config ? sum[sh] / epochs_per_config : 0.f;
+  if (best_wg) *best_wg = b->wg;
+  if (best_tapes_per_lane) *best_tapes_per_lane = b->per_lane;
   return CUBFF_OK;
 }
 

@@ -251,9 +251,11 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch,
                                  uint64_t count, int sched, float *total_ms,
                                  float *interp_ms);
 /* Runs epochs_per_config live epochs under each of the 12 launch shapes
- * (workgroup 64/128/256 x 1-4 tapes per lane), keeps the fastest and reports
- * it; ms_table (nullable, 12 floats) receives ms/epoch per shape.  Results
- * do not depend on the launch shape, so the tuning epochs are real epochs. */
+ * (workgroup 64/128/256 x 1-4 tapes per lane), in rounds of one epoch per
+ * shape so that every shape sees the soups at the same stage, keeps the
+ * fastest by total time and reports it; ms_table (nullable, 12 floats)
+ * receives ms/epoch per shape.  Results do not depend on the launch shape,
+ * so the tuning epochs are real epochs. */
 int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch,
                          uint64_t epochs_per_config, int sched, int *best_wg,
                          uint32_t *best_tapes_per_lane, float *ms_table);
