@@ -7,6 +7,9 @@
 namespace cubff {
 
 constexpr int kNumBatchCounters = 4;  // ops, num_runs, steps, unused
+// Pairing buffers: epoch e's pairing is in shuf[e % kPairBufs]; the pairing
+// of epoch e + 2 is drawn while epoch e runs.
+constexpr uint32_t kPairBufs = 3;
 
 // Everything a batched launch needs, passed by value to every kernel.
 // Program indices are global (soup s owns [prog_base[s], prog_base[s+1]));
@@ -20,10 +23,17 @@ struct BatchDev {
   const uint32_t *prob;        // mutation threshold per soup (of 2^30)
   const uint64_t *num_indices; // evaluated slots per soup
   uint8_t *soup;               // total_programs * 64
-  uint32_t *shuf[2];           // total_programs each, local indices per soup:
-                               // the pairing of epoch e is in shuf[e & 1]
+  uint32_t *shuf[kPairBufs];   // total_programs each, local indices per soup:
+                               // the pairing of epoch e is in shuf[e % 3]
   uint8_t *stage;              // total_slots * 128
   uint32_t *tgt;               // total_slots * 2, global program indices
+  uint32_t *inv[kPairBufs];    // inverse pairings: inv[e % 3][program] = its
+                               // position in shuf[e % 3] (global indices),
+                               // which is also the staged half (2 * slot + h)
+                               // that holds it in epoch e
+  uint32_t *halves_next;       // nullable, total_slots * 2: written by the
+                               // stage of epoch e with the staged halves
+                               // holding the programs of epoch e + 1's slots
   uint32_t *pair_ops, *pair_steps;     // total_slots
   uint32_t *pair_work;                 // total_slots: commands actually interpreted
   unsigned long long *counters;        // num_soups * kNumBatchCounters
@@ -34,9 +44,10 @@ struct BatchDev {
 
 // The epoch is read from d.epoch; batch_end_epoch adds num_runs and
 // advances it, so one captured epoch can be replayed.  batch_pair_shuffle
-// draws the pairing of epoch *d.epoch + ahead (ahead 0 or 1).
+// draws the pairing of epoch *d.epoch + ahead (ahead 0, 1 or 2).
 hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d, uint32_t ahead);
 hipError_t batch_stage(hipStream_t st, const BatchDev &d);
+hipError_t batch_copy_u32(hipStream_t st, uint32_t *dst, const uint32_t *src, uint64_t count);
 hipError_t batch_end_epoch(hipStream_t st, const BatchDev &d);
 hipError_t batch_fill_u32(hipStream_t st, uint32_t *p, uint32_t v, uint64_t count);
 // Evaluates list[0..count) (staged slot indices) with language `lang`,

@@ -144,7 +144,9 @@ __global__ void k_b_fy_final(const BatchDev d, uint32_t ahead) {
     v = c == kNone ? j : b_fy_in(d, c);
   }
   const uint32_t s = soup_of(pb, d.num_soups, t);
-  d.shuf[(*d.epoch + ahead) & 1][t] = (uint32_t)(v - pb[s]);
+  const uint64_t e = *d.epoch + ahead;
+  d.shuf[e % kPairBufs][t] = (uint32_t)(v - pb[s]);
+  d.inv[e % kPairBufs][v] = (uint32_t)t;
 }
 
 __global__ void k_b_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t count) {
@@ -169,7 +171,7 @@ __global__ void k_b_stage(const BatchDev d) {
   const uint64_t k = g - slot_base[s];
   const uint64_t pb = prog_base[s];
   const uint64_t n = prog_base[s + 1] - pb;
-  const uint32_t p = d.shuf[epoch & 1][pb + 2 * k + (q >> 2)];
+  const uint32_t p = d.shuf[epoch % kPairBufs][pb + 2 * k + (q >> 2)];
   uint4 v = reinterpret_cast<const uint4 *>(d.soup + (pb + p) * kTape)[q & 3];
   uint32_t w[4] = {v.x, v.y, v.z, v.w};
   const uint64_t seed_value = epoch_seed_pre(d.sm_seed[s], epoch);
@@ -184,7 +186,20 @@ __global__ void k_b_stage(const BatchDev d) {
     }
   }
   reinterpret_cast<uint4 *>(d.stage + g * kTape2)[q] = make_uint4(w[0], w[1], w[2], w[3]);
-  if ((q & 3) == 0) d.tgt[2 * g + (q >> 2)] = (uint32_t)(pb + p);  // q=0: s[2k], q=4: s[2k+1]
+  if ((q & 3) == 0) {  // q=0: s[2k], q=4: s[2k+1]
+    const uint64_t h = 2 * g + (q >> 2);  // = pb + 2k + (q >> 2)
+    d.tgt[h] = (uint32_t)(pb + p);
+    // The program at the same position of the next pairing is staged now
+    // in half inv_e[program] (the next pairing is drawn two epochs ahead).
+    if (d.halves_next)
+      d.halves_next[h] = d.inv[epoch % kPairBufs][pb + d.shuf[(epoch + 1) % kPairBufs][h]];
+  }
+}
+
+__global__ void k_b_copy(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src,
+                         uint64_t count) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -298,6 +313,12 @@ hipError_t batch_pair_shuffle(hipStream_t st, const BatchDev &d, uint32_t ahead)
 
 hipError_t batch_stage(hipStream_t st, const BatchDev &d) {
   hipLaunchKernelGGL(k_b_stage, dim3(nblk(d.total_slots * 8, 256)), dim3(256), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t batch_copy_u32(hipStream_t st, uint32_t *dst, const uint32_t *src, uint64_t count) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_b_copy, dim3(nblk(count, 256)), dim3(256), 0, st, dst, src, count);
   return hipGetLastError();
 }
 

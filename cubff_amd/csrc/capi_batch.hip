@@ -17,11 +17,12 @@ using namespace cubff;
 struct cubff_batch {
   int device = 0;
   hipStream_t stream = nullptr;
-  // Pairings are drawn one epoch ahead on stream2, forked from and joined
-  // back into each epoch (also inside the captured graph).
+  // Pairings are drawn two epochs ahead on stream2, and after them the
+  // learned order one epoch ahead, forked from and joined back into each
+  // epoch (also inside the captured graph).
   hipStream_t stream2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
-  uint64_t drawn = UINT64_MAX;  // epoch whose pairing is in d.shuf[drawn & 1]
+  uint64_t drawn = UINT64_MAX;  // the pairings of epochs drawn and drawn + 1 are drawn
   uint64_t last = 0;            // last epoch run
   uint32_t num_soups = 0;
   std::vector<int> langs;
@@ -35,6 +36,12 @@ struct cubff_batch {
   std::vector<SchedList> lists;  // offsets into plain_list / order_list
   uint32_t *plain_list = nullptr;   // total_slots
   uint32_t *order_list = nullptr;   // total_slots (scheduled order)
+  // Learned order of the next epoch, built on stream2 beside the
+  // interpreter from the next pairs' programs as this epoch staged them
+  // (allocated on the first learned run).
+  uint32_t *order_next = nullptr;   // total_slots
+  uint32_t *ahead = nullptr;        // total_slots * 2 staged halves
+  uint64_t ordered = UINT64_MAX;    // epoch whose order is in order_list
   SchedState sched{};
   int wg = 256;
   uint32_t per_lane = 1;
@@ -87,33 +94,52 @@ void drop_graph(cubff_batch *b) {
   b->graph_sched = -1;
 }
 
-// One epoch on the batch stream: pairing, stage, [schedule], interpret,
-// end-of-epoch bookkeeping.  interp_ev (nullable) brackets the interpreter
-// launches with events.
+// One epoch on the batch stream: stage, interpret (in order_list's order
+// under the learned scheduler), end-of-epoch bookkeeping.  interp_ev
+// (nullable) brackets the interpreter launches with events.
 int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   const hipStream_t st = b->stream;
+  if (sched != CUBFF_SCHED_PLAIN && sched != CUBFF_SCHED_LEARNED)
+    return fail(CUBFF_EINVAL, "unknown scheduling mode");
+  const bool learned = sched == CUBFF_SCHED_LEARNED;
   HIP_TRY(batch_stage(st, b->d));
-  // The next epoch's pairing, beside the rest of this one (it reads and
-  // writes only the other pairing buffer and the pairing scratch); forked
-  // after the stage, whose memory traffic it would otherwise slow down.
+  // The pairing of epoch e + 2, beside the rest of this one (it writes only
+  // its own pairing buffer and the pairing scratch); forked after the stage,
+  // whose memory traffic it would otherwise slow down.  Under the learned
+  // scheduler the next epoch's order follows it on stream2: inference on its
+  // pairs' tapes as pieces of this epoch's stage (named by the stage in
+  // d.halves_next) and bucketing into order_next.  Started behind the
+  // pairing, the inference meets the interpreter's tail, when most of its
+  // workgroups are done, rather than its start, when they fill every CU
+  // (measured: 1.8-2.0 ms behind the pairing vs 3.0 ms from the fork, 0.8 ms
+  // alone).
   HIP_TRY(hipEventRecord(b->fork, st));
   HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
-  HIP_TRY(batch_pair_shuffle(b->stream2, b->d, 1));
+  HIP_TRY(batch_pair_shuffle(b->stream2, b->d, 2));
+  if (learned)
+    HIP_TRY(sched_order(b->stream2, b->sched, b->d, b->ahead, b->plain_list, b->order_next,
+                        b->lists.data(), b->lists.size()));
   HIP_TRY(hipEventRecord(b->join, b->stream2));
-  const uint32_t *list = b->plain_list;
-  if (sched == CUBFF_SCHED_LEARNED) {
-    HIP_TRY(sched_order(st, b->sched, b->d, b->plain_list, b->order_list, b->lists.data(),
-                        b->lists.size()));
-    list = b->order_list;
-  } else if (sched != CUBFF_SCHED_PLAIN) {
-    return fail(CUBFF_EINVAL, "unknown scheduling mode");
-  }
+  const uint32_t *list = learned ? b->order_list : b->plain_list;
   if (ev0) HIP_TRY(hipEventRecord(ev0, st));
   for (const auto &l : b->lists)
     HIP_TRY(batch_interp(st, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
   if (ev1) HIP_TRY(hipEventRecord(ev1, st));
   HIP_TRY(hipStreamWaitEvent(st, b->join, 0));  // before the epoch counter moves
+  if (learned) HIP_TRY(batch_copy_u32(st, b->order_list, b->order_next, b->d.total_slots));
   HIP_TRY(batch_end_epoch(st, b->d));
+  return CUBFF_OK;
+}
+
+// The learned order of epoch *d.epoch from its own staged tapes, on the batch
+// stream (outside captured graphs): for the first epoch of a run with no
+// order built ahead, and after the first training, whose weights the order
+// built during that epoch did not have yet.  The epoch's stage is repeated
+// by the epoch itself (same inputs, same bytes).
+int order_fresh(cubff_batch *b) {
+  HIP_TRY(batch_stage(b->stream, b->d));
+  HIP_TRY(sched_order(b->stream, b->sched, b->d, nullptr, b->plain_list, b->order_list,
+                      b->lists.data(), b->lists.size()));
   return CUBFF_OK;
 }
 
@@ -131,6 +157,8 @@ int after_epoch(cubff_batch *b, int sched) {
     if (b->sched.trainings == 0 && !b->sched.pending) {
       float loss = 0;
       HIP_TRY(sched_train(b->stream, b->sched, b->d, kTrainSteps, kTrainLr, &loss));
+      int r;
+      if ((r = order_fresh(b))) return r;
     } else {
       HIP_TRY(sched_train_async(b->stream, b->sched, b->d, kTrainSteps, kTrainLr));
     }
@@ -148,14 +176,38 @@ int finish_run(cubff_batch *b, int sched) {
 }
 
 // Sets the device epoch counter to the first epoch of a run of `count` and
-// draws that epoch's pairing unless the previous run drew it already.
-int start_run(cubff_batch *b, uint64_t epoch, uint64_t count) {
+// draws the pairings of that epoch and the next unless the previous run drew
+// them already; under the learned scheduler, orders that epoch unless the
+// previous run did.
+int start_run(cubff_batch *b, uint64_t epoch, uint64_t count, int sched) {
   HIP_TRY(hipMemcpyAsync(b->d.epoch, &epoch, 8, hipMemcpyHostToDevice, b->stream));
-  if (count && b->drawn != epoch) HIP_TRY(batch_pair_shuffle(b->stream, b->d, 0));
+  if (count && b->drawn != epoch) {
+    HIP_TRY(batch_pair_shuffle(b->stream, b->d, 0));
+    HIP_TRY(batch_pair_shuffle(b->stream, b->d, 1));
+  }
+  const bool learned = sched == CUBFF_SCHED_LEARNED;
+  int r;
+  if (count && learned && b->ordered != epoch && (r = order_fresh(b))) return r;
   HIP_TRY(hipStreamSynchronize(b->stream));
   if (count) {
     b->drawn = epoch + count;
     b->last = epoch + count - 1;
+    b->ordered = learned ? epoch + count : UINT64_MAX;
+  }
+  return CUBFF_OK;
+}
+
+// Scheduler state and the buffers of the order built ahead.
+int ensure_learned(cubff_batch *b) {
+  if (sched_ensure(b->sched, b->d, b->lists.size()))
+    return fail(CUBFF_ENOMEM, "scheduler allocation failed");
+  if (!b->ahead) {
+    int r;
+    if ((r = alloc(&b->order_next, b->d.total_slots)) ||
+        (r = alloc(&b->ahead, b->d.total_slots * 2)))
+      return r;
+    b->d.halves_next = b->ahead;
+    drop_graph(b);  // captured with the old device view
   }
   return CUBFF_OK;
 }
@@ -204,9 +256,10 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
   if ((r = alloc(&b->prog_base_d, num_soups + 1)) || (r = alloc(&b->slot_base_d, num_soups + 1)) ||
       (r = alloc(&b->sm_seed_d, num_soups)) || (r = alloc(&b->num_indices_d, num_soups)) ||
       (r = alloc(&b->prob_d, num_soups)) || (r = alloc(&d.soup, P * kTape)) ||
-      (r = alloc(&d.shuf[0], P)) || (r = alloc(&d.shuf[1], P)) ||
+      (r = alloc(&d.shuf[0], P)) || (r = alloc(&d.shuf[1], P)) || (r = alloc(&d.shuf[2], P)) ||
       (r = alloc(&d.stage, S * kTape2)) ||
-      (r = alloc(&d.tgt, 2 * S)) || (r = alloc(&d.pair_ops, S)) ||
+      (r = alloc(&d.tgt, 2 * S)) || (r = alloc(&d.inv[0], P)) || (r = alloc(&d.inv[1], P)) ||
+      (r = alloc(&d.inv[2], P)) || (r = alloc(&d.pair_ops, S)) ||
       (r = alloc(&d.pair_steps, S)) || (r = alloc(&d.pair_work, S)) ||
       (r = alloc(&ctr, (uint64_t)num_soups * kNumBatchCounters)) ||
       (r = alloc(&d.epoch, 1)) || (r = alloc(&d.fy_j, P)) || (r = alloc(&cnt, P + 1)) ||
@@ -280,8 +333,14 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->d.soup);
   release(b->d.shuf[0]);
   release(b->d.shuf[1]);
+  release(b->d.shuf[2]);
   release(b->d.stage);
   release(b->d.tgt);
+  release(b->d.inv[0]);
+  release(b->d.inv[1]);
+  release(b->d.inv[2]);
+  release(b->order_next);
+  release(b->ahead);
   release(b->d.pair_ops);
   release(b->d.pair_steps);
   release(b->d.pair_work);
@@ -322,6 +381,7 @@ int cubff_batch_init(cubff_batch *b, const uint64_t *params_seeds,
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->have_seeds = true;
   b->drawn = UINT64_MAX;  // pairings depend on the seeds
+  b->ordered = UINT64_MAX;
   return CUBFF_OK;
 }
 
@@ -348,9 +408,8 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
   int r;
   if ((r = check(b))) return r;
   if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
-  if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
-    return fail(CUBFF_ENOMEM, "scheduler allocation failed");
-  if ((r = start_run(b, first_epoch, count))) return r;
+  if (sched == CUBFF_SCHED_LEARNED && (r = ensure_learned(b))) return r;
+  if ((r = start_run(b, first_epoch, count, sched))) return r;
   if (!use_graph) {
     for (uint64_t i = 0; i < count; i++) {
       if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
@@ -386,9 +445,8 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t 
   int r;
   if ((r = check(b))) return r;
   if (!b->have_seeds) return fail(CUBFF_ESTATE, "cubff_batch_init first");
-  if (sched == CUBFF_SCHED_LEARNED && (r = sched_ensure(b->sched, b->d, b->lists.size())))
-    return fail(CUBFF_ENOMEM, "scheduler allocation failed");
-  if ((r = start_run(b, first_epoch, count))) return r;
+  if (sched == CUBFF_SCHED_LEARNED && (r = ensure_learned(b))) return r;
+  if ((r = start_run(b, first_epoch, count, sched))) return r;
   std::vector<hipEvent_t> ev(2 * count + 2);
   for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], b->stream));
@@ -490,6 +548,7 @@ int cubff_batch_write_soup(cubff_batch *b, uint32_t soup, const uint8_t *src) {
   HIP_TRY(hipMemcpyAsync(b->d.soup + b->prog_base[soup] * kTape, src, b->n[soup] * kTape,
                          hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  b->ordered = UINT64_MAX;  // an order built ahead saw other programs
   return CUBFF_OK;
 }
 
@@ -533,7 +592,7 @@ int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s) {
   int r;
   if ((r = check(b))) return r;
   if (soup >= b->num_soups || !s) return fail(CUBFF_EINVAL, "bad soup index or null destination");
-  HIP_TRY(hipMemcpyAsync(s, b->d.shuf[b->last & 1] + b->prog_base[soup], b->n[soup] * 4,
+  HIP_TRY(hipMemcpyAsync(s, b->d.shuf[b->last % kPairBufs] + b->prog_base[soup], b->n[soup] * 4,
                          hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return CUBFF_OK;

@@ -49,6 +49,7 @@ struct Mlp {
   uint16_t *d2 = nullptr, *d1 = nullptr, *dp = nullptr; // [capacity, 256]
   float *y = nullptr, *target = nullptr;                 // [capacity]
   float *loss = nullptr;               // device scalar
+  unsigned int *claim = nullptr;       // inference: passes claimed so far
   uint64_t adam_t = 0;
 };
 
@@ -76,9 +77,11 @@ hipError_t mlp_publish(Mlp &m, hipStream_t st);
 // y[0..count): features, both layers and the output in one kernel, with the
 // folded first layer.  Needs no activation buffers.
 hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y, hipStream_t st);
-// mlp_predict with the `live` weights as they are (never folds).
+// mlp_predict with the `live` weights as they are (never folds).  With
+// `halves` (2 * count entries), tape r is the two 64-byte pieces at
+// tapes + 64 * halves[2r] and tapes + 64 * halves[2r + 1] instead.
 hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
-                            hipStream_t st);
+                            hipStream_t st, const uint32_t *halves = nullptr);
 // Loss 0.5 * mean((y - target)^2) and all gradients into m.grads for
 // m.x / m.target rows [0..count); *m.loss receives the loss.
 hipError_t mlp_backward(Mlp &m, uint64_t count, hipStream_t st);

@@ -372,6 +372,8 @@ __device__ __forceinline__ uint16_t to_bf16(float x) {
 }
 
 __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restrict__ tapes,
+                                                 const uint32_t *__restrict__ halves,
+                                                 unsigned int *__restrict__ claim,
                                                  uint64_t count,
                                                  const uint16_t *__restrict__ ew1t,
                                                  const uint16_t *__restrict__ w2t,
@@ -381,6 +383,7 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
   __shared__ __align__(16) uint16_t hist[kInfRows * kRowStride];
   __shared__ __align__(16) uint16_t act[kInfRows * kRowStride];
   __shared__ float ypart[kInfRows];
+  __shared__ uint32_t claimed;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int r16 = lane & 15, kg = lane >> 4;
   const int col0 = 32 * w;
@@ -410,25 +413,38 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
     }
   const float b3 = vec[3 * kMlpHid];
 
-  // Thread tid loads 16 bytes (chunk tid % 8) of tape tid / 8 of each pass.
+  // Thread tid loads 16 bytes (chunk tid % 8) of tape tid / 8 of each pass:
+  // bytes 16 (tid % 4) .. +15 of 64-byte piece 2 row + (tid % 8) / 4, which
+  // is that piece of the contiguous tapes or the one `halves` names.
   const int tr = tid >> 3, tpart = tid & 7;
+  auto piece = [&](uint64_t row) -> const uint4 * {
+    const uint64_t h = 2 * row + (tpart >> 2);
+    return reinterpret_cast<const uint4 *>(tapes + (halves ? (uint64_t)halves[h] : h) * kTape) +
+           (tpart & 3);
+  };
   const uint64_t passes = (count + kInfRows - 1) / kInfRows;
   uint4 next = make_uint4(0, 0, 0, 0);
   for (int i = tid; i < kInfRows * kCountStride / 4; i += kInfThreads) cnt8[i] = 0;
   if (tid < kInfRows) ypart[tid] = 0.f;
   lds_barrier();
+  // Workgroup b starts with pass b.  Its next pass is b + grid, b + 2 grid,
+  // ... or, with `claim` (zero at launch), the next unclaimed one: beside
+  // other kernels, workgroups placed late then take fewer passes.
   uint64_t p = blockIdx.x;
-  if (p < passes && p * kInfRows + tr < count)
-    next = *reinterpret_cast<const uint4 *>(tapes + (p * kInfRows + tr) * kTape2 + tpart * 16);
-  for (; p < passes; p += gridDim.x) {
+  if (p < passes && p * kInfRows + tr < count) next = *piece(p * kInfRows + tr);
+  while (p < passes) {
     const uint64_t row0 = p * kInfRows;
     const bool have = row0 + tr < count;
     const uint4 cur = next;
     // 1. Counts into cnt8 (zeroed by the previous pass's conversion, with
     // barriers since; hist, act and ypart are next written after barriers).
-    const uint64_t pn = p + gridDim.x;
-    if (pn < passes && pn * kInfRows + tr < count)
-      next = *reinterpret_cast<const uint4 *>(tapes + (pn * kInfRows + tr) * kTape2 + tpart * 16);
+    // `claimed` was last read before this pass's earlier barriers.
+    uint64_t pn = p + gridDim.x;
+    if (claim) {
+      if (tid == 0) claimed = gridDim.x + atomicAdd(claim, 1u);
+    } else if (pn < passes && pn * kInfRows + tr < count) {
+      next = *piece(pn * kInfRows + tr);
+    }
     if (have) {
       const uint32_t wd[4] = {cur.x, cur.y, cur.z, cur.w};
       uint32_t *hr = cnt8 + tr * (kCountStride / 4);
@@ -439,6 +455,10 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
       }
     }
     lds_barrier();
+    if (claim) {
+      pn = claimed;
+      if (pn < passes && pn * kInfRows + tr < count) next = *piece(pn * kInfRows + tr);
+    }
     // Counts to bf16, once for all waves: thread tid converts 32 counts
     // (row tid / 8, bins 32 (tid % 8) .. +31).
     {
@@ -540,8 +560,11 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
       if (row0 + tid < count) y[row0 + tid] = ypart[tid] + b3;
       ypart[tid] = 0.f;  // only this thread reads it; added to after barriers
     }
+    p = pn;
   }
 }
+
+__global__ void k_zero_claim(unsigned int *claim) { *claim = 0; }
 
 hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
                        hipStream_t st) {
@@ -554,7 +577,7 @@ hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
 }
 
 hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
-                            hipStream_t st) {
+                            hipStream_t st, const uint32_t *halves) {
   if (count == 0) return hipSuccess;
   int dev = 0, cus = 0;
   hipError_t e;
@@ -563,7 +586,9 @@ hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, 
     return e;
   const uint64_t passes = (count + kInfRows - 1) / kInfRows;
   const unsigned grid = (unsigned)(passes < (uint64_t)cus ? passes : (uint64_t)cus);
-  hipLaunchKernelGGL(k_predict, dim3(grid), dim3(kInfThreads), 0, st, tapes, count, m.live.ew1t,
+  if (m.claim) hipLaunchKernelGGL(k_zero_claim, dim3(1), dim3(1), 0, st, m.claim);
+  hipLaunchKernelGGL(k_predict, dim3(grid), dim3(kInfThreads), 0, st, tapes, halves, m.claim,
+                     count, m.live.ew1t,
                      m.live.w2t, m.live.vec, y);
   return hipGetLastError();
 }
@@ -600,7 +625,8 @@ hipError_t mlp_alloc(Mlp &m, uint64_t capacity) {
       (e = zalloc(&m.a1, capacity * kMlpHid)) || (e = zalloc(&m.a2, capacity * kMlpHid)) ||
       (e = zalloc(&m.d2, capacity * kMlpHid)) || (e = zalloc(&m.d1, capacity * kMlpHid)) ||
       (e = zalloc(&m.dp, capacity * kMlpHid)) || (e = zalloc(&m.y, capacity)) ||
-      (e = zalloc(&m.target, capacity)) || (e = zalloc(&m.loss, 1)))
+      (e = zalloc(&m.target, capacity)) || (e = zalloc(&m.loss, 1)) ||
+      (e = zalloc(&m.claim, 1)))
     return e;
   return hipSuccess;
 }
@@ -608,7 +634,7 @@ hipError_t mlp_alloc(Mlp &m, uint64_t capacity) {
 void mlp_free(Mlp &m) {
   void *ps[] = {m.params, m.grads, m.m, m.v, m.e_w, m.e_wt, m.w1, m.w1t, m.w2, m.w2t,
                 m.live.ew1t, m.live.w2t, m.live.vec, m.next.ew1t, m.next.w2t, m.next.vec,
-                m.x, m.p, m.a1, m.a2, m.d2, m.d1, m.dp, m.y, m.target, m.loss};
+                m.x, m.p, m.a1, m.a2, m.d2, m.d1, m.dp, m.y, m.target, m.loss, m.claim};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   m = Mlp{};

@@ -49,9 +49,11 @@ struct SchedList {
 
 int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists);
 void sched_free(SchedState &s);
-// Predicts every staged slot, then writes each list's slots to `order`
-// grouped by bucket, longest bucket first.
-hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d,
+// Predicts every slot from its tape in d.stage (halves == nullptr) or, for
+// the next epoch, from the two staged halves named in d.halves_next,
+// then writes each list's slots to `order` grouped by bucket, longest
+// bucket first.
+hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const uint32_t *halves,
                        const uint32_t *plain, uint32_t *order, const SchedList *lists,
                        size_t num_lists);
 // Trains on the last epoch's staged tapes and recorded ops (host-driven,

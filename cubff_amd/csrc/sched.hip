@@ -193,11 +193,12 @@ void sched_free(SchedState &s) {
   s = SchedState{};
 }
 
-hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const uint32_t *plain,
-                       uint32_t *order, const SchedList *lists, size_t num_lists) {
+hipError_t sched_order(hipStream_t st, SchedState &s, const BatchDev &d, const uint32_t *halves,
+                       const uint32_t *plain, uint32_t *order, const SchedList *lists,
+                       size_t num_lists) {
   if (!s.ready) return hipErrorNotReady;
   hipError_t e;
-  if ((e = mlp_predict_live(s.mlp, d.stage, d.total_slots, s.pred, st))) return e;
+  if ((e = mlp_predict_live(s.mlp, d.stage, d.total_slots, s.pred, st, halves))) return e;
   hipLaunchKernelGGL(k_sched_fill, dim3(1), dim3(256), 0, st, s.cnt, 0u,
                      (uint64_t)num_lists * 2 * kBuckets);
   hipLaunchKernelGGL(k_sched_fill, dim3(kPredBins / 256), dim3(256), 0, st, s.hist, 0u,
