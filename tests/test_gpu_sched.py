@@ -153,11 +153,13 @@ def test_learned_buckets_cut_interpreter_time(capi):
     # The gain the trained network buys: on the 64-soup batch (seeds 0..63,
     # sizes cycling 2^12..2^17, mutation 1e-5..1e-3) the interpreter launches
     # of the learned schedule take less device time than the plain ones over
-    # the same epochs, with identical soups.
+    # the same epochs, with identical soups, and so do whole epochs: the
+    # next epoch's inference runs beside the interpreter.
     S = 64
     ns = [[4096, 16384, 65536, 131072][k % 4] for k in range(S)]
     probs = [int(round(10 ** (-5 + 2 * k / 63) * (1 << 30))) for k in range(S)]
     interp = {}
+    total = {}
     soups = {}
     for name, sched in (("plain", capi.SCHED_PLAIN), ("learned", capi.SCHED_LEARNED)):
         print("timing", name, flush=True)
@@ -165,9 +167,11 @@ def test_learned_buckets_cut_interpreter_time(capi):
             b.init(list(range(S)), probs)
             b.set_launch(64, 1)
             b.run_epochs(0, 8, sched)
-            _, it = b.run_epochs_timed(8, 12, sched)
+            tot, it = b.run_epochs_timed(8, 12, sched)
             interp[name] = it
+            total[name] = tot
             soups[name] = [b.read_soup(i) for i in (0, 3, 62)]
     for x, y in zip(soups["plain"], soups["learned"]):
         assert np.array_equal(x, y)
     assert interp["learned"] < 0.95 * interp["plain"], interp
+    assert total["learned"] < total["plain"], total
