@@ -22,6 +22,12 @@ struct cubff_batch {
   // epoch (also inside the captured graph).
   hipStream_t stream2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // Batches of several languages: every second language's interpreter
+  // launch runs on stream_i beside the others (one per language otherwise
+  // queue behind each other), forked after the stage and joined before the
+  // epoch ends.
+  hipStream_t stream_i = nullptr;
+  hipEvent_t join_i = nullptr;
   uint64_t drawn = UINT64_MAX;  // the pairings of epochs drawn and drawn + 1 are drawn
   uint64_t last = 0;            // last epoch run
   uint32_t num_soups = 0;
@@ -122,8 +128,21 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   HIP_TRY(hipEventRecord(b->join, b->stream2));
   const uint32_t *list = learned ? b->order_list : b->plain_list;
   if (ev0) HIP_TRY(hipEventRecord(ev0, st));
-  for (const auto &l : b->lists)
-    HIP_TRY(batch_interp(st, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
+  // Languages' slots are disjoint, and so are their soups.
+  bool side = false;
+  for (size_t li = 0; li < b->lists.size(); li++) {
+    const auto &l = b->lists[li];
+    const hipStream_t s = (b->stream_i && (li & 1)) ? b->stream_i : st;
+    if (s != st && !side) {
+      HIP_TRY(hipStreamWaitEvent(s, b->fork, 0));
+      side = true;
+    }
+    HIP_TRY(batch_interp(s, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
+  }
+  if (side) {
+    HIP_TRY(hipEventRecord(b->join_i, b->stream_i));
+    HIP_TRY(hipStreamWaitEvent(st, b->join_i, 0));
+  }
   if (ev1) HIP_TRY(hipEventRecord(ev1, st));
   HIP_TRY(hipStreamWaitEvent(st, b->join, 0));  // before the epoch counter moves
   if (learned) HIP_TRY(batch_copy_u32(st, b->order_list, b->order_next, b->d.total_slots));
@@ -299,6 +318,14 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
         for (uint64_t g = b->slot_base[s]; g < b->slot_base[s + 1]; g++) plain.push_back((uint32_t)g);
     if (plain.size() > begin) b->lists.push_back({lang, begin, plain.size() - begin});
   }
+  if (b->lists.size() > 1) {
+    e = hipStreamCreateWithFlags(&b->stream_i, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join_i, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      cubff_batch_destroy(b);
+      return fail(CUBFF_EHIP, "hipStreamCreate failed");
+    }
+  }
   std::vector<uint64_t> ni(num_soups);
   for (uint32_t s = 0; s < num_soups; s++) ni[s] = b->n[s] / 2;
   const uint64_t zero = 0;
@@ -356,6 +383,9 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->plain_list);
   release(b->order_list);
   if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->stream_i) (void)hipStreamSynchronize(b->stream_i);
+  if (b->join_i) (void)hipEventDestroy(b->join_i);
+  if (b->stream_i) (void)hipStreamDestroy(b->stream_i);
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->join) (void)hipEventDestroy(b->join);
   if (b->stream2) (void)hipStreamDestroy(b->stream2);
