@@ -39,7 +39,8 @@ EXPORTS = [
     "cubff_batch_read_shuffle", "cubff_batch_sched_stats",
     "cubff_mlp_last_error", "cubff_mlp_num_params", "cubff_mlp_create",
     "cubff_mlp_destroy", "cubff_mlp_set_params", "cubff_mlp_get_params",
-    "cubff_mlp_forward", "cubff_mlp_predict", "cubff_mlp_gradients", "cubff_mlp_train",
+    "cubff_mlp_forward", "cubff_mlp_predict", "cubff_mlp_predict_pieces", "cubff_mlp_gradients",
+    "cubff_mlp_train",
 ]
 
 SCHED_PLAIN, SCHED_LEARNED = 0, 1
@@ -133,6 +134,7 @@ def load():
         "cubff_mlp_get_params": (_i, [_vp, _f32p]),
         "cubff_mlp_forward": (_i, [_vp, _u8p, _u64, _f32p]),
         "cubff_mlp_predict": (_i, [_vp, _u8p, _u64, _f32p]),
+        "cubff_mlp_predict_pieces": (_i, [_vp, _u8p, _u64, _u32p, _u64, _f32p]),
         "cubff_mlp_gradients": (_i, [_vp, _u8p, _f32p, _u64, _f32p, _f32p]),
         "cubff_mlp_train": (_i, [_vp, _u8p, _f32p, _u64, ctypes.c_uint32, ctypes.c_float,
                                  _f32p]),
@@ -530,6 +532,18 @@ class Mlp:
         t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128)
         out = np.zeros(t.shape[0], dtype=np.float32)
         _mcheck(load().cubff_mlp_predict(self._h, _ptr(t, _u8p), t.shape[0], _ptr(out, _f32p)))
+        return out
+
+    def predict_pieces(self, pieces: np.ndarray, index: np.ndarray) -> np.ndarray:
+        """The same kernel on tapes given as pairs of 64-byte pieces: tape r
+        is pieces[index[2r]] followed by pieces[index[2r + 1]]."""
+        p = np.ascontiguousarray(pieces, dtype=np.uint8).reshape(-1, 64)
+        ix = np.ascontiguousarray(index, dtype=np.uint32).reshape(-1)
+        if ix.size % 2:
+            raise ValueError("index needs two entries per tape")
+        out = np.zeros(ix.size // 2, dtype=np.float32)
+        _mcheck(load().cubff_mlp_predict_pieces(self._h, _ptr(p, _u8p), p.shape[0],
+                                                _ptr(ix, _u32p), out.size, _ptr(out, _f32p)))
         return out
 
     def gradients(self, tapes: np.ndarray, targets: np.ndarray):

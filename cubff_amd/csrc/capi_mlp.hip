@@ -16,6 +16,7 @@ struct cubff_mlp {
   hipStream_t stream = nullptr;
   Mlp m;
   uint8_t *tapes = nullptr;
+  uint32_t *index = nullptr;  // 2 * cap piece indices
   uint64_t cap = 0;
 };
 
@@ -57,6 +58,7 @@ int cubff_mlp_create(cubff_mlp **out, uint64_t capacity, uint64_t init_seed, int
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = mlp_alloc(h->m, capacity);
   if (e == hipSuccess) e = hipMalloc(&h->tapes, (capacity ? capacity : 1) * kTape2);
+  if (e == hipSuccess) e = hipMalloc(&h->index, (capacity ? capacity : 1) * 2 * 4);
   if (e == hipSuccess) e = mlp_init(h->m, init_seed, h->stream);
   if (e != hipSuccess) {
     cubff_mlp_destroy(h);
@@ -72,6 +74,7 @@ int cubff_mlp_destroy(cubff_mlp *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   mlp_free(h->m);
   if (h->tapes) (void)hipFree(h->tapes);
+  if (h->index) (void)hipFree(h->index);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return CUBFF_OK;
@@ -108,6 +111,23 @@ int cubff_mlp_predict(cubff_mlp *h, const uint8_t *tapes, uint64_t count, float 
   MLP_TRY(hipSetDevice(h->device));
   MLP_TRY(hipMemcpyAsync(h->tapes, tapes, count * kTape2, hipMemcpyHostToDevice, h->stream));
   MLP_TRY(mlp_predict(h->m, h->tapes, count, h->m.y, h->stream));
+  MLP_TRY(hipMemcpyAsync(pred, h->m.y, count * 4, hipMemcpyDeviceToHost, h->stream));
+  MLP_TRY(hipStreamSynchronize(h->stream));
+  return CUBFF_OK;
+}
+
+int cubff_mlp_predict_pieces(cubff_mlp *h, const uint8_t *pieces, uint64_t num_pieces,
+                             const uint32_t *index, uint64_t count, float *pred) {
+  if (!h || !pieces || !index || !pred) return fail(CUBFF_EINVAL, "null argument");
+  if (count == 0 || count > h->cap) return fail(CUBFF_EINVAL, "count must be in [1, capacity]");
+  if (num_pieces == 0 || num_pieces > 2 * h->cap)
+    return fail(CUBFF_EINVAL, "num_pieces must be in [1, 2 * capacity]");
+  for (uint64_t i = 0; i < 2 * count; i++)
+    if (index[i] >= num_pieces) return fail(CUBFF_EINVAL, "piece index out of range");
+  MLP_TRY(hipSetDevice(h->device));
+  MLP_TRY(hipMemcpyAsync(h->tapes, pieces, num_pieces * kTape, hipMemcpyHostToDevice, h->stream));
+  MLP_TRY(hipMemcpyAsync(h->index, index, 2 * count * 4, hipMemcpyHostToDevice, h->stream));
+  MLP_TRY(mlp_predict(h->m, h->tapes, count, h->m.y, h->stream, h->index));
   MLP_TRY(hipMemcpyAsync(pred, h->m.y, count * 4, hipMemcpyDeviceToHost, h->stream));
   MLP_TRY(hipStreamSynchronize(h->stream));
   return CUBFF_OK;

@@ -76,7 +76,8 @@ hipError_t mlp_publish(Mlp &m, hipStream_t st);
 // Fused inference of `count` tapes (device, 128 bytes apart) straight into
 // y[0..count): features, both layers and the output in one kernel, with the
 // folded first layer.  Needs no activation buffers.
-hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y, hipStream_t st);
+hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y, hipStream_t st,
+                       const uint32_t *halves = nullptr);
 // mlp_predict with the `live` weights as they are (never folds).  With
 // `halves` (2 * count entries), tape r is the two 64-byte pieces at
 // tapes + 64 * halves[2r] and tapes + 64 * halves[2r + 1] instead.

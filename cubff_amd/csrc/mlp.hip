@@ -567,13 +567,13 @@ __global__ __launch_bounds__(kInfThreads) void k_predict(const uint8_t *__restri
 __global__ void k_zero_claim(unsigned int *claim) { *claim = 0; }
 
 hipError_t mlp_predict(Mlp &m, const uint8_t *tapes, uint64_t count, float *y,
-                       hipStream_t st) {
+                       hipStream_t st, const uint32_t *halves) {
   if (count == 0) return hipSuccess;
   if (!m.folded_valid) {
     hipError_t e = mlp_fold(m, st);
     if (e != hipSuccess) return e;
   }
-  return mlp_predict_live(m, tapes, count, y, st);
+  return mlp_predict_live(m, tapes, count, y, st, halves);
 }
 
 hipError_t mlp_predict_live(const Mlp &m, const uint8_t *tapes, uint64_t count, float *y,

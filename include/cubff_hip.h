@@ -301,6 +301,14 @@ int cubff_mlp_forward(cubff_mlp *h, const uint8_t *tapes, uint64_t count,
  * rounding of the folded first layer. */
 int cubff_mlp_predict(cubff_mlp *h, const uint8_t *tapes, uint64_t count,
                       float *pred);
+/* The same kernel reading each tape as two 64-byte pieces: tape r is pieces
+ * index[2r] then index[2r + 1] of `pieces` (num_pieces * 64 bytes, at most
+ * 2 * capacity pieces; every index below num_pieces).  This is how the
+ * batched learned scheduler reads the next epoch's tapes out of the current
+ * epoch's stage.  Predictions equal cubff_mlp_predict's on the assembled
+ * tapes. */
+int cubff_mlp_predict_pieces(cubff_mlp *h, const uint8_t *pieces, uint64_t num_pieces,
+                             const uint32_t *index, uint64_t count, float *pred);
 /* Loss 0.5 * mean((pred - target)^2) and its gradient (same layout as the
  * parameters). */
 int cubff_mlp_gradients(cubff_mlp *h, const uint8_t *tapes,

@@ -54,6 +54,29 @@ def test_mlp_fused_predict_matches_cpu(capi):
     assert _rel(y, y_fwd) < 3e-2
 
 
+def test_mlp_predict_pieces_matches_assembled_tapes(capi):
+    # The scheduler reads the next epoch's tapes as two 64-byte pieces of the
+    # current stage: same predictions as on the tapes put together, for
+    # pieces used once, twice or not at all (ragged count of 300 tapes).
+    rng = np.random.default_rng(5)
+    pieces = biased_tapes("bff_noheads", 320, seed=22).reshape(-1, 64)  # 640 pieces
+    index = rng.integers(0, pieces.shape[0], size=600).astype(np.uint32)
+    index[:40] = np.arange(40)  # a few in order
+    tapes = np.concatenate([pieces[index[0::2]], pieces[index[1::2]]], axis=1)
+    with capi.Mlp(512, init_seed=6) as m:
+        params = m.get_params()
+        y = m.predict_pieces(pieces, index)
+        y_tapes = m.predict(tapes)
+        with pytest.raises(RuntimeError, match="out of range"):
+            bad = index.copy()
+            bad[7] = pieces.shape[0]
+            m.predict_pieces(pieces, bad)
+    # The per-row sums of the eight wavefronts' partial products are added
+    # in arrival order: equal up to fp32 reassociation.
+    assert np.allclose(y, y_tapes, rtol=1e-5, atol=1e-5)
+    assert _rel(y, R.predict_folded(params, tapes)) < 5e-3
+
+
 def test_mlp_gradients_match_cpu(capi):
     tapes = biased_tapes("bff_noheads", 256, seed=12)
     targets = np.random.default_rng(0).uniform(0, 13, 256).astype(np.float32)
