@@ -203,8 +203,11 @@ __global__ void k_b_copy(uint32_t *__restrict__ dst, const uint32_t *__restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Interpreter: lane `lane_g` evaluates list entries lane_g, lane_g + L,
-// lane_g + 2L, ... (L = lanes in the grid) of `list`, T entries at most.
+// Interpreter: lane `lane_g` evaluates one list entry in each of the ranges
+// [rL, rL + L) (L = lanes in the grid), T ranges at most: rL + lane_g in
+// even rounds, rL + L - 1 - lane_g in odd ones.  Under the learned order
+// (longest predicted first) the lanes that ran the longest pairs of one
+// round then run the shortest of the next.
 template <int LANG, int LANES>
 __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
                                                     const uint32_t *__restrict__ list,
@@ -220,7 +223,7 @@ __global__ __launch_bounds__(LANES) void k_b_interp(const BatchDev d,
   const uint64_t lanes_total = (uint64_t)gridDim.x * LANES;
   const uint64_t lane_g = (uint64_t)blockIdx.x * LANES + lane;
   for (uint32_t r = 0; r < per_lane; r++) {
-    const uint64_t e = lane_g + r * lanes_total;
+    const uint64_t e = r * lanes_total + ((r & 1) ? lanes_total - 1 - lane_g : lane_g);
     if (e >= count) break;
     const uint32_t g = list[e];
     const uint4 *src = reinterpret_cast<const uint4 *>(d.stage + (uint64_t)g * kTape2);
