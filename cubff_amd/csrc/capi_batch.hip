@@ -220,11 +220,10 @@ int start_run(cubff_batch *b, uint64_t epoch, uint64_t count, int sched) {
 int ensure_learned(cubff_batch *b) {
   if (sched_ensure(b->sched, b->d, b->lists.size()))
     return fail(CUBFF_ENOMEM, "scheduler allocation failed");
+  int r;
+  if (!b->order_next && (r = alloc(&b->order_next, b->d.total_slots))) return r;
   if (!b->ahead) {
-    int r;
-    if ((r = alloc(&b->order_next, b->d.total_slots)) ||
-        (r = alloc(&b->ahead, b->d.total_slots * 2)))
-      return r;
+    if ((r = alloc(&b->ahead, b->d.total_slots * 2))) return r;
     b->d.halves_next = b->ahead;
     drop_graph(b);  // captured with the old device view
   }

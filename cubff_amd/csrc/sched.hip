@@ -144,8 +144,9 @@ __global__ void k_sched_fill(uint32_t *__restrict__ p, uint32_t v, uint64_t n) {
   if (i < n) p[i] = v;
 }
 
-int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
-  if (s.ready) return 0;
+namespace {
+
+int sched_alloc(SchedState &s, const BatchDev &d, size_t num_lists) {
   // Training sample: kTrainRows slots spread evenly over the batch.
   s.train_rows_n = d.total_slots < kTrainRows ? d.total_slots / 64 * 64 : kTrainRows;
   if (s.train_rows_n == 0) s.train_rows_n = d.total_slots;
@@ -174,6 +175,15 @@ int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
   if (mlp_fold(s.mlp, st) != hipSuccess || hipStreamSynchronize(st) != hipSuccess) return -1;
   s.ready = true;
   return 0;
+}
+
+}  // namespace
+
+int sched_ensure(SchedState &s, const BatchDev &d, size_t num_lists) {
+  if (s.ready) return 0;
+  if (sched_alloc(s, d, num_lists) == 0) return 0;
+  sched_free(s);  // whatever the failed attempt allocated
+  return -1;
 }
 
 void sched_free(SchedState &s) {
