@@ -177,7 +177,10 @@ def test_learned_buckets_cut_interpreter_time(capi):
     # sizes cycling 2^12..2^17, mutation 1e-5..1e-3) the interpreter launches
     # of the learned schedule take less device time than the plain ones over
     # the same epochs, with identical soups, and so do whole epochs: the
-    # next epoch's inference runs beside the interpreter.
+    # next epoch's inference runs beside the interpreter.  Two tapes per
+    # lane, the shape the autotuner picks for this batch (with one, the
+    # interpreter's workgroups leave the inference no room and whole epochs
+    # are slower).
     S = 64
     ns = [[4096, 16384, 65536, 131072][k % 4] for k in range(S)]
     probs = [int(round(10 ** (-5 + 2 * k / 63) * (1 << 30))) for k in range(S)]
@@ -188,7 +191,7 @@ def test_learned_buckets_cut_interpreter_time(capi):
         print("timing", name, flush=True)
         with capi.Batch(["bff_noheads"] * S, ns) as b:
             b.init(list(range(S)), probs)
-            b.set_launch(64, 1)
+            b.set_launch(64, 2)
             b.run_epochs(0, 8, sched)
             tot, it = b.run_epochs_timed(8, 12, sched)
             interp[name] = it
