@@ -49,8 +49,8 @@ struct cubff_batch {
   uint32_t *ahead = nullptr;        // total_slots * 2 staged halves
   uint64_t ordered = UINT64_MAX;    // epoch whose order is in order_list
   SchedState sched{};
-  int wg = 256;
-  uint32_t per_lane = 1;
+  int wg = 64;  // launch shape: measured best for plain and learned order
+  uint32_t per_lane = 2;
   bool have_seeds = false;
   hipGraphExec_t graph = nullptr;
   int graph_sched = -1, graph_wg = 0;

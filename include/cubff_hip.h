@@ -239,7 +239,8 @@ int cubff_batch_destroy(cubff_batch *b);
 int cubff_batch_init(cubff_batch *b, const uint64_t *params_seeds,
                      const uint32_t *mutation_probs, int zero_init);
 /* Interpreter launch shape: workgroup size (64, 128 or 256 lanes) and tapes
- * evaluated one after another by each lane. */
+ * evaluated one after another by each lane.  Default 64 lanes x 2 tapes
+ * (what cubff_batch_autotune picks for the 64-soup bff_noheads batch). */
 int cubff_batch_set_launch(cubff_batch *b, int wg, uint32_t tapes_per_lane);
 int cubff_batch_get_launch(cubff_batch *b, int *wg, uint32_t *tapes_per_lane);
 /* `count` epochs from first_epoch; use_graph replays one captured epoch. */
