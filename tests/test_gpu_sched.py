@@ -172,6 +172,34 @@ def test_learned_schedule_is_bit_exact(capi):
             assert np.array_equal(c.read_soup(i), ref.soup)
 
 
+def test_learned_order_across_runs_writes_and_modes(capi):
+    # The order of the next epoch is built during the current one and kept
+    # across calls; a soup write, a plain run or a graph/no-graph switch in
+    # between must not change results (the order only permutes evaluation,
+    # and a stale one is rebuilt).  Same soups and counters as all-plain.
+    spec = [("bff_noheads", 4096, 0, 262144), ("forth", 1024, 1, 262144),
+            ("bff", 2048, 2, 1 << 20)]
+    langs, ns = [x[0] for x in spec], [x[1] for x in spec]
+    seeds, probs = [x[2] for x in spec], [x[3] for x in spec]
+    X = biased_tapes("bff_noheads", 1024, seed=31).reshape(-1)[: 2048 * 64]
+    with capi.Batch(langs, ns) as a, capi.Batch(langs, ns) as b:
+        for x in (a, b):
+            x.init(seeds, probs)
+        a.run_epochs(0, 5, capi.SCHED_LEARNED, use_graph=True)
+        a.run_epochs(5, 3, capi.SCHED_PLAIN)
+        a.run_epochs(8, 4, capi.SCHED_LEARNED, use_graph=False)
+        a.write_soup(2, X)
+        a.run_epochs(12, 1, capi.SCHED_LEARNED)
+        a.run_epochs(13, 6, capi.SCHED_LEARNED, use_graph=True)
+        b.run_epochs(0, 12, capi.SCHED_PLAIN)
+        b.write_soup(2, X)
+        b.run_epochs(12, 7, capi.SCHED_PLAIN)
+        for i in range(len(spec)):
+            assert np.array_equal(a.read_soup(i), b.read_soup(i)), i
+            assert a.read_counters(i) == b.read_counters(i), i
+            assert np.array_equal(a.read_shuffle(i), b.read_shuffle(i)), i
+
+
 def test_learned_buckets_cut_interpreter_time(capi):
     # The gain the trained network buys: on the 64-soup batch (seeds 0..63,
     # sizes cycling 2^12..2^17, mutation 1e-5..1e-3) the interpreter launches
