@@ -56,6 +56,7 @@ struct cubff_ctx {
   uint32_t defer_gen = 0;
   uint32_t defer_budget = kDeferBudget;
   uint64_t defer_items = 0;  // interactions deferred since creation
+  double resolver_idle_s = 0.1;  // the resolver leaves after this long with nothing deferred
 };
 
 namespace {
@@ -122,7 +123,6 @@ constexpr uint64_t kFlowMaxEpochs = 64;
 constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
-constexpr double kResolverIdleS = 0.1;
 constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
 // Deferring segments end by waiting for the resolver to finish the last
 // epochs' long interactions: long segments (bounded by kFlowBytes of list
@@ -451,7 +451,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.pair_steps = c->pair_steps;
   a.counters = c->counters;
   a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
-  a.idle_ticks = (uint64_t)(std::min(kResolverIdleS, c->flow_wait_s) * khz * 1000.0);
+  a.idle_ticks = (uint64_t)(std::min(c->resolver_idle_s, c->flow_wait_s) * khz * 1000.0);
   hipEvent_t started = nullptr, resolved = nullptr;
   std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
   auto cleanup = [&]() {
@@ -513,10 +513,12 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
   pa.finish(epochs);
-  c->defer_items += ctl.final_tail;
   // The resolver may leave early when it cannot run beside the epoch
-  // kernels; the run is complete as long as every deferred entry was run.
-  if (ctl.resolved != ctl.final_tail)
+  // kernels (the list then reads kDeferGone or more and holds nothing);
+  // the run is complete as long as every deferred entry was run.
+  const uint32_t listed = ctl.final_tail >= kDeferGone ? 0u : ctl.final_tail;
+  c->defer_items += listed;
+  if (ctl.resolved != listed)
     return fail(CUBFF_EHIP, "deferred-interaction resolver stopped: a wait exceeded " +
                                 std::to_string(c->flow_wait_s) + " s; soup state is undefined");
   return CUBFF_OK;
@@ -528,8 +530,15 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
 // floats, nullable) receives the total and the epoch-kernel milliseconds.
 int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first_epoch,
                     uint64_t count, uint32_t prob, float *timing) {
-  const bool closed_form = mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
-                           mode == CUBFF_PAIR_CYCLIC;
+  // The fixed shuffle of a soup whose size is not a power of two is not a
+  // permutation (((i * offset) % n) ^ flip repeats programs: n = 6 at epoch
+  // 1 gives 1,2,1,2,1,2).  The reference CLI rejects it (main.cc:233-237);
+  // its library runs it with slots racing on shared programs.  Here it runs
+  // one launch per epoch, whose lanes then race the same way; the dataflow
+  // and deferring kernels, which track one writer per program, never see it.
+  const bool permutation = mode != CUBFF_PAIR_FIXED || (c->n & (c->n - 1)) == 0;
+  const bool closed_form = permutation && (mode == CUBFF_PAIR_SHUFFLE || mode == CUBFF_PAIR_FIXED ||
+                                           mode == CUBFF_PAIR_CYCLIC);
   // Pairings drawn ahead: every device mode (none depends on the soup).
   const bool ahead = closed_form || (mode == CUBFF_PAIR_2D && c->row_off);
   const bool defer = (c->pipeline == CUBFF_FLOW_DEFER ||
@@ -892,6 +901,14 @@ int cubff_flow_stats(cubff_ctx *c, uint64_t *items, uint64_t *claims, uint64_t *
   if (items) *items = c->flow_items;
   if (claims) *claims = c->flow_claims;
   if (deferred) *deferred = c->defer_items;
+  return CUBFF_OK;
+}
+
+int cubff_set_resolver_idle(cubff_ctx *c, double seconds) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (!(seconds >= 0)) return fail(CUBFF_EINVAL, "idle time must be >= 0");
+  c->resolver_idle_s = seconds;
   return CUBFF_OK;
 }
 

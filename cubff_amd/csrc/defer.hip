@@ -135,6 +135,24 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
     }
   }
   const uint32_t at = wave_alloc_u32(&a.ctl->tail, defer ? 1u : 0u);
+  // The resolver left before anything was deferred (tail held kDeferGone
+  // when this allocation landed): no entry of this run exists, so no
+  // program is owed and the interaction was deferred for its budget alone.
+  // Run it here to the end, from the soup's unchanged bytes.
+  const bool gone = defer && at >= kDeferGone;
+  if (gone) {
+    load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
+    load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+    mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel), k, a.prob);
+    if (eval) ops = L::evaluate(t, aux, kEvalSteps, &steps);
+    store_row(t, 0, a.soup + (uint64_t)p1 * kTape);
+    store_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+    if (e_rel + 1 == a.epochs) {
+      a.pair_ops[k] = ops;
+      a.pair_steps[k] = steps;
+    }
+    defer = false;
+  }
   const bool put = defer && at < a.capacity;
   if (defer && !put) st_agent(&a.ctl->abort, 1);
   if (put) {
@@ -186,7 +204,9 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
       for (;;) {
         if (ld_agent(&c->abort)) break;
         const uint32_t h = ld_agent(&c->head);
-        const uint32_t tl = min(ld_agent(&c->tail), a.capacity);
+        const uint32_t traw = ld_agent(&c->tail);
+        if (traw >= kDeferGone) break;  // another wavefront left with the list empty
+        const uint32_t tl = min(traw, a.capacity);
         if (h < tl) {
           uint32_t expect = h;
           if (__hip_atomic_compare_exchange_strong(&c->head, &expect, h + 1, __ATOMIC_RELAXED,
@@ -205,11 +225,18 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
         }
         // Nothing deferred yet and no epoch kernel finished for a while: the
         // epoch kernels cannot start beside this kernel (dispatches are
-        // serialized, as under some profilers).  Leave; the epoch kernels
-        // then run every interaction themselves.
+        // serialized, as under some profilers).  Leave, marking the list
+        // gone in the same atomic step that checks it is still empty; the
+        // epoch kernels then run every interaction themselves (their
+        // allocations return kDeferGone or more).  If an epoch kernel
+        // deferred something meanwhile, stay and resolve it.
         if (tl == 0 && wall_clock64() - t0 > a.idle_ticks) {
-          st_agent(&c->abort, 1);
-          break;
+          uint32_t expect = 0;
+          if (__hip_atomic_compare_exchange_strong(&c->tail, &expect, kDeferGone,
+                                                   __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT))
+            break;
+          continue;
         }
         if (wall_clock64() - t0 > a.spin_ticks) {
           st_agent(&c->abort, 1);

@@ -72,6 +72,11 @@ struct DeferItem {
 
 constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
 constexpr uint32_t kNoEval = 0x80000000u;    // DeferItem::e flag
+// DeferCtl::tail once the resolver has left without anything deferred: it
+// sets tail from 0 to this value in one compare-and-swap, so an allocation
+// either lands before (the resolver stays) or returns at least kDeferGone
+// (the epoch kernel runs that interaction itself).  Above any capacity.
+constexpr uint32_t kDeferGone = 0x80000000u;
 
 struct DeferArgs {
   uint8_t *soup;

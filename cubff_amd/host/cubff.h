@@ -11,6 +11,7 @@
 
 #include <array>
 #include <functional>
+#include <memory>
 #include <optional>
 #include <string>
 #include <utility>
@@ -78,8 +79,15 @@ struct LanguageInterface {
   virtual std::vector<uint8_t> Parse(const std::string &program) = 0;
 };
 
-// Language registry (common.cc:28-40): unknown names print the available
-// languages and exit(1).
+// Language registry (common.cc:22-45, common.h:133-134).  It starts with
+// the 13 device languages of the HIP library; RegisterLanguage adds another
+// LanguageInterface under `lang`, or replaces the one registered there, and
+// the registry owns it.  It may be called from a static constructor of
+// another shared object (the reference's REGISTER does so, common.h:41-45),
+// before or after the built-in languages are set up.  GetLanguage returns a
+// non-owning pointer; unknown names print the available languages and
+// exit(1).
+void RegisterLanguage(const char *lang, std::unique_ptr<LanguageInterface> interface);
 const LanguageInterface *GetLanguage(const std::string &language);
 std::vector<std::string> LanguageNames();
 

@@ -371,6 +371,10 @@ std::map<std::string, std::unique_ptr<LanguageInterface>> &registry() {
 
 }  // namespace
 
+void RegisterLanguage(const char *lang, std::unique_ptr<LanguageInterface> interface) {
+  registry()[lang] = std::move(interface);
+}
+
 const LanguageInterface *GetLanguage(const std::string &language) {
   auto &map = registry();
   auto it = map.find(language);

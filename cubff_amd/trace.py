@@ -20,8 +20,9 @@ _TAPE = 128
 def evaluate_and_save(tape: bytes | bytearray, file_path: str, stepcount: int,
                       debug: bool = False, lang: str = "bff") -> int:
     """Runs the tape for at most `stepcount` steps, writes every state before
-    a step to `file_path`, and returns the number of executed non-no-op
-    steps.  The tape passed in is not modified."""
+    a step to `file_path`, and returns what the reference returns: the
+    executed non-no-op steps, less one when the step cap ended the run (see
+    reference_return).  The tape passed in is not modified."""
     if len(tape) != _TAPE:
         raise ValueError(f"Program must be exactly {_TAPE} bytes long")
     image, ops = capi.trace_bff(lang, bytes(tape), stepcount)
@@ -30,6 +31,26 @@ def evaluate_and_save(tape: bytes | bytearray, file_path: str, stepcount: int,
     if debug:
         for pc, h0, h1, t in _states(image):
             print(f"pc={pc} head0={h0} head1={h1} tape={t.hex()}")
+    steps = (len(image) - 12) // (12 + _TAPE)
+    one_more = steps
+    if stepcount > 0 and steps == stepcount:
+        one_more = int(capi.eval_tapes(lang, bytes(tape), stepcount + 1)[2][0])
+    return reference_return(ops, steps, stepcount, one_more)
+
+
+def reference_return(ops: int, steps: int, stepcount: int, steps_one_more: int) -> int:
+    """evaluate_and_save's return value from Evaluate's results.
+
+    ops and steps are Evaluate's result and loop iterations at `stepcount`
+    (bff.inc.h:356-389), steps_one_more the iterations of the same tape at
+    stepcount + 1.  The reference's loop, `for i in range(stepcount)` with
+    `i += 1; break` where the program ends (bff_interpreter.py:373-391),
+    leaves i = stepcount - 1 when all stepcount steps run and the program is
+    still going, and returns i - nskip: one less than Evaluate, whose i
+    reaches stepcount (-1 when every step was a no-op).  A run that ends
+    before the cap, or at its last step, returns Evaluate's value."""
+    if stepcount > 0 and steps == stepcount and steps_one_more > stepcount:
+        return ops - 1
     return ops
 
 

@@ -132,6 +132,12 @@ int cubff_byte_histogram(cubff_ctx *ctx, uint64_t *counts);
 /* Command budget of an interaction in CUBFF_FLOW_DEFER epoch kernels
  * (default 768). */
 int cubff_set_defer_budget(cubff_ctx *ctx, uint32_t budget);
+/* How long the resolver of CUBFF_FLOW_DEFER runs waits, with nothing
+ * deferred and no epoch kernel finishing, before it leaves (default 0.1 s;
+ * then the epoch kernels run every interaction themselves).  0 makes it
+ * leave at once unless something is already deferred.  No reference
+ * counterpart. */
+int cubff_set_resolver_idle(cubff_ctx *ctx, double seconds);
 
 int cubff_run_epochs_timed(cubff_ctx *ctx, int mode, uint64_t params_seed,
                            uint64_t first_epoch, uint64_t count,

@@ -137,3 +137,31 @@ def test_debug_fixtures_agree_with_plain_transcripts():
                     assert len(a) > len(b) or r["steps"] == 0
                     seen += 1
     assert seen >= 13
+
+
+def test_register_language_from_another_library(tmp_path):
+    # A language registered by its own shared object at load time
+    # (RegisterLanguage, common.h:133-134) is what GetLanguage returns, in
+    # C++ and through the cubff module, beside the 13 built-in languages.
+    src = os.path.join(ROOT, "tests", "plugin", "echo_language.cc")
+    so = str(tmp_path / "libecho_language.so")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared",
+                    "-I", os.path.join(ROOT, "cubff_amd", "host"), src,
+                    os.path.join(BIN, "libcubff_host.so"), f"-Wl,-rpath,{BIN}", "-o", so],
+                   check=True, timeout=120)
+    out = run_py(
+        "import ctypes\n"
+        f"ctypes.CDLL({so!r}, mode=ctypes.RTLD_GLOBAL)\n"
+        "import cubff\n"
+        "L = cubff.GetLanguage('echo')\n"
+        "print(bytes(L.Parse('abc')).decode(), flush=True)\n"
+        "print(L.EvalSelfrep('xy', 3, 4, False), flush=True)\n"
+        "L.RunSingleProgram('+-', 17, True)\n"
+        "p = cubff.SimulationParams(); p.num_programs = 6; p.seed = 40\n"
+        "seen = []\n"
+        "L.RunSimulation(p, None, lambda s: seen.append((s.epoch, s.total_ops, len(s.soup))) "
+        "or s.epoch >= 2)\n"
+        "print(seen)\n"
+        "print(bytes(cubff.GetLanguage('bff').Parse('[]')).hex())\n").decode()
+    assert out.splitlines() == ["cba", "2034", "echo RunSingleProgram +- 17 1",
+                                "[(0, 40, 6), (1, 41, 6), (2, 42, 6)]", "5b5d"]

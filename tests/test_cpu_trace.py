@@ -55,3 +55,41 @@ def test_read_states_checks_header(tmp_path):
         (tmp_path / "b.bff").write_bytes(bad)
         with pytest.raises(ValueError):
             trace.read_states(str(tmp_path / "b.bff"))
+
+
+def _python_return(lang, t, cap):
+    from cubff_amd.trace import reference_return
+    _, ops, steps = O.evaluate(lang, t, cap)
+    _, _, steps1 = O.evaluate(lang, t, cap + 1)
+    return reference_return(ops, steps, cap, steps1)
+
+
+def test_evaluate_and_save_return_at_the_step_cap():
+    # Cases derived by hand from the reference's loop
+    # (python/bff_interpreter.py:373-393: `for i in range(stepcount)`,
+    # `i += 1; break` where the program ends, return i - nskip).
+    # Comments only (no command ever executes): each step is a no-op; at a
+    # cap of 10 the loop leaves i = 9 with nskip = 10, so -1; with the whole
+    # tape run (cap 300) the last step ends it: i = 128, nskip = 128, so 0.
+    comments = np.full(128, ord("a"), dtype=np.uint8)
+    assert _python_return("bff_noheads", comments, 10) == -1
+    assert _python_return("bff_noheads", comments, 300) == 0
+    # 128 '+' commands, no heads: the program ends after its 128th step.
+    plus = np.full(128, ord("+"), dtype=np.uint8)
+    assert _python_return("bff_noheads", plus, 128) == 128  # ends at the last step
+    assert _python_return("bff_noheads", plus, 127) == 126  # cut by the cap: i = 126
+    assert _python_return("bff_noheads", plus, 500) == 128
+    assert _python_return("bff_noheads", plus, 0) == 0      # the loop never runs
+    # bff with heads: bytes 0 and 1 put both heads on byte 0 (behind the
+    # code) and pc starts at 2, so 126 commands end the tape.
+    heads = plus.copy()
+    heads[0] = heads[1] = 0
+    assert _python_return("bff", heads, 126) == 126
+    assert _python_return("bff", heads, 125) == 124
+    # An endless loop "[+]" with a nonzero byte under head 0: always capped.
+    loop = np.full(128, ord("a"), dtype=np.uint8)
+    loop[0], loop[1], loop[2] = ord("+"), ord("["), ord("]")
+    for cap in (5, 64, 8192):
+        _, ops, steps = O.evaluate("bff_noheads", loop, cap)
+        assert steps == cap
+        assert _python_return("bff_noheads", loop, cap) == ops - 1

@@ -245,3 +245,58 @@ def test_2d_pairing_multi_epoch(capi, graph, pipeline):
                    budget=budget, graph=gr)
     _check_run(capi, "forth", n, 6, 1 << 20, capi.PAIR_2D, 0, 5, pipeline=pl,
                budget=None if pipeline == "loop" else 16, graph=gr)
+
+
+def test_fixed_pairing_of_non_power_of_two_soup_stays_on_the_epoch_loop(capi):
+    # The fixed shuffle ((i * offset) % n) ^ flip repeats programs when n is
+    # not a power of two (n = 6 at epoch 1: 1,2,1,2,1,2).  The reference's
+    # library accepts it (its CLI refuses it, main.cc:233-237) and its slots
+    # then race on the shared programs.  Here such runs take one launch per
+    # epoch whatever the pipeline: the dataflow and deferring kernels, which
+    # track one writer per program, must never see them.
+    P = 1 << 22
+    for n in (6, 1000):
+        for lang in ("forth", "bff_noheads"):
+            for pipeline in (capi.FLOW_ON, capi.FLOW_DEFER, capi.FLOW_AUTO):
+                print("fixed non-power-of-two:", lang, n, pipeline, flush=True)
+                with capi.Soup(lang, n) as g:
+                    g.set_pipeline(pipeline)
+                    g.init_programs(O.epoch_seed(9, 0))
+                    g.run_epochs(capi.PAIR_FIXED, 9, 0, 8, P)
+                    s, k = g.read_shuffle()
+                    assert k == n // 2 and np.array_equal(s, O.pairing_fixed(n, 7))
+                    items, _, deferred = g.flow_stats()
+                    assert items == 0 and deferred == 0
+    # Power-of-two soups still take the multi-epoch kernels.
+    with capi.Soup("forth", 1024) as g:
+        g.set_pipeline(capi.FLOW_ON)
+        g.init_programs(O.epoch_seed(9, 0))
+        g.run_epochs(capi.PAIR_FIXED, 9, 0, 8, 1 << 22)
+        assert g.flow_stats()[0] > 0
+
+
+def test_defer_resolver_leaving_at_once(capi):
+    # With a zero idle time every resolver wavefront leaves at its first
+    # look unless something is already deferred: it swaps the empty list
+    # for the gone mark in one atomic step, and an epoch kernel whose
+    # deferral lands on that mark runs the interaction itself.  Either way
+    # the soup, counters and last results are the oracle's.
+    n, seed, P = 131072, 2, 1 << 18
+    for budget in (768, 16):
+        print("resolver idle 0, budget", budget, flush=True)
+        ref = O.Soup("bff_noheads", n, seed, P, nthreads=16)
+        with capi.Soup("bff_noheads", n) as g:
+            g.set_pipeline(capi.FLOW_DEFER)
+            g.set_resolver_idle(0.0)
+            g.set_defer_budget(budget)
+            g.init_programs(O.epoch_seed(seed, 0))
+            g.reset_counters()
+            tot = 0
+            for e in range(6):
+                t, po, ps = ref.step()
+                tot += t
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 6, P)
+            assert _sha(g.read_soup()) == _sha(ref.soup)
+            ops, steps = g.read_pair_ops()
+            assert np.array_equal(ops, po) and np.array_equal(steps, ps)
+            assert g.read_counters() == (tot, 6 * (n // 2))

@@ -38,7 +38,15 @@ def test_evaluate_and_save(capi, tmp_path):
     path = str(tmp_path / "run.bff")
     ops = trace.evaluate_and_save(bytearray(t.tobytes()), path, 500)
     _, ops_ref, steps_ref = O.evaluate("bff", t, 500)
-    assert ops == ops_ref
+    _, _, steps_one_more = O.evaluate("bff", t, 501)
+    assert ops == trace.reference_return(ops_ref, steps_ref, 500, steps_one_more)
+    # At the step cap the reference returns one less than Evaluate.
+    loop = np.full(128, ord("a"), dtype=np.uint8)
+    loop[0], loop[1], loop[2], loop[3] = 5, 5, ord("["), ord("]")
+    _, ops_cap, steps_cap = O.evaluate("bff", loop, 300)
+    assert steps_cap == 300
+    assert trace.evaluate_and_save(bytearray(loop.tobytes()), path, 300) == ops_cap - 1
+    assert len(trace.read_states(path)) == 300
     states = trace.read_states(path)
     assert len(states) == steps_ref
     assert states[0][3] == t.tobytes()
