@@ -626,6 +626,12 @@ int run_epochs_impl(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t first
 
 }  // namespace
 
+namespace cubff {
+// Failures of entry points defined in other files of the library, reported
+// through cubff_last_error().
+int report_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace cubff
+
 extern "C" {
 
 const char *cubff_last_error(void) { return g_last_error.c_str(); }

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 namespace cubff {
 
 constexpr int kEpochLanes = 256;      // lanes (= pair slots) per epoch block
@@ -105,6 +107,8 @@ hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
 hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
                           uint64_t n, uint64_t count, uint64_t seed_value,
                           uint64_t *result, uint8_t *trace = nullptr);
+// Records a failure for cubff_last_error() (capi.hip); returns code.
+int report_error(int code, const std::string &msg);
 hipError_t launch_eval_tapes(hipStream_t st, int lang, uint8_t *tapes,
                              uint64_t count, uint32_t steps, uint32_t *ops,
                              uint32_t *steps_out);

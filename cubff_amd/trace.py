@@ -11,6 +11,8 @@ from __future__ import annotations
 
 import struct
 
+import numpy as np
+
 from cubff_amd import capi
 
 _MAGIC = b"BFF\0"
@@ -34,7 +36,8 @@ def evaluate_and_save(tape: bytes | bytearray, file_path: str, stepcount: int,
     steps = (len(image) - 12) // (12 + _TAPE)
     one_more = steps
     if stepcount > 0 and steps == stepcount:
-        one_more = int(capi.eval_tapes(lang, bytes(tape), stepcount + 1)[2][0])
+        t = np.frombuffer(bytes(tape), dtype=np.uint8)
+        one_more = int(capi.eval_tapes(lang, t, stepcount + 1)[2][0])
     return reference_return(ops, steps, stepcount, one_more)
 
 

@@ -190,6 +190,27 @@ def gen_full_size():
         "source": "reference RunSimulation via oracle/_ref", "seeds": out})
 
 
+def gen_long_run():
+    """bff_noheads, 2^17 programs, seed 0, default mutation, callbacks every
+    64 epochs up to epoch 1025 (the CLI's 1000-epoch run at print_interval
+    64): full soup and pairing hashes and cumulative ops at each callback."""
+    rows = []
+
+    def cb(st):
+        rows.append({"epoch": st.epoch, "soup": sha(R.soup_of(st)),
+                     "shuffle": sha(R.shuffle_of(st, 131072)), "total_ops": st.total_ops,
+                     "mops_s": None})
+        return st.epoch >= 1025
+
+    R.run_simulation("bff_noheads", 131072, 0, cb, callback_interval=64)
+    for r in rows:
+        del r["mops_s"]
+    dump("bff_noheads_2p17_seed0_1025_epochs.json", {
+        "params": {"lang": "bff_noheads", "num_programs": 131072, "seed": 0,
+                   "mutation_prob": 262144, "callback_interval": 64, "epochs": 1025},
+        "source": "reference RunSimulation via oracle/_ref", "rows": rows})
+
+
 def gen_cli_logs():
     """CSV logs of the reference CLI (test.sh:30 settings) with this image's
     libbrotli, plus the reference's own testdata/ files for comparison."""
@@ -410,7 +431,7 @@ def _grid_lists(w, h, r=2):
 
 
 GENS = {"eval": gen_eval, "epochs": gen_epochs, "selfrep": gen_selfrep,
-        "2d": gen_2d, "modes": gen_modes, "full": gen_full_size,
+        "2d": gen_2d, "modes": gen_modes, "full": gen_full_size, "long": gen_long_run,
         "cli": gen_cli_logs, "cli_cases": gen_cli_cases, "cli_debug": gen_cli_debug}
 
 if __name__ == "__main__":

@@ -199,6 +199,19 @@ def test_full_size_first_epoch_vs_reference():
         assert sha(soup) == rows[0]["soup"]
 
 
+def test_long_run_first_callbacks_vs_reference():
+    # The oracle's first 65 epochs of the 1025-epoch bff_noheads run at 2^17
+    # (seed 0) against the reference's recorded soup and pairing hashes.
+    g = load("bff_noheads_2p17_seed0_1025_epochs.json")
+    p = g["params"]
+    rows, soup = simulate(p["lang"], p["num_programs"], p["seed"], 65,
+                          callback_interval=p["callback_interval"])
+    assert [r["epoch"] for r in rows] == [1, 65]
+    for got, want in zip(rows, g["rows"][:2]):
+        assert got["total_ops"] == want["total_ops"]
+    assert sha(soup) == g["rows"][1]["soup"]
+
+
 def test_oracle_vs_live_reference_build():
     """Where oracle/_ref is built (from the reference sources), evaluate and
     one mutate-and-run epoch agree on fresh random inputs."""

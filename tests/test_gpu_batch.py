@@ -108,3 +108,25 @@ def test_batch_pairing_drawn_ahead(capi, graph):
         for i, (l, n, s, p) in enumerate(spec):
             assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, 8)), i
             assert np.array_equal(b.read_soup(i), refs[i][8][0]), i
+
+
+def test_config3_full_size_matches_oracle(capi):
+    # The 64-soup batch (seeds 0..63, sizes cycling 2^12, 2^14, 2^16, 2^17,
+    # mutation log-spaced 1e-5..1e-3), graph-replayed: its first soup of each
+    # size equals a separate oracle run after each of two epochs.
+    S = 64
+    ns = [[4096, 16384, 65536, 131072][k % 4] for k in range(S)]
+    probs = [int(round(10 ** (-5 + 2 * k / 63) * (1 << 30))) for k in range(S)]
+    check = (0, 1, 2, 3)
+    refs = {k: O.Soup("bff_noheads", ns[k], k, probs[k], nthreads=16) for k in check}
+    with capi.Batch(["bff_noheads"] * S, ns) as b:
+        b.init(list(range(S)), probs)
+        for e in range(2):
+            print("config 3 epoch", e, flush=True)
+            b.run_epochs(e, 1, capi.SCHED_PLAIN, use_graph=True)
+            for k, ref in refs.items():
+                tot, po, ps = ref.step()
+                assert np.array_equal(b.read_shuffle(k), O.pairing_shuffle(ns[k], k, e)), (k, e)
+                assert np.array_equal(b.read_soup(k), ref.soup), (k, e)
+                ops, steps = b.read_pair_stats(k)
+                assert np.array_equal(ops, po) and np.array_equal(steps, ps), (k, e)

@@ -137,6 +137,33 @@ def test_python_module_full_size_soups():
         assert all(s[3] == 131072 for s in seen)
 
 
+def test_python_module_long_run():
+    # The 1000-epoch bff_noheads run at 2^17 (seed 0, callbacks every 64
+    # epochs, so 1025 epochs as the reference CLI runs it): soup and pairing
+    # hashes and cumulative ops at every callback equal the reference's.
+    cubff = py_module()
+    g = load("bff_noheads_2p17_seed0_1025_epochs.json")
+    p = g["params"]
+    params = cubff.SimulationParams()
+    params.num_programs = p["num_programs"]
+    params.seed = p["seed"]
+    params.callback_interval = p["callback_interval"]
+    seen = []
+
+    def cb(state):
+        seen.append({"epoch": state.epoch,
+                     "soup": hashlib.sha256(memoryview(state.soup)).hexdigest(),
+                     "shuffle": hashlib.sha256(memoryview(state.shuffle_idx)).hexdigest(),
+                     "total_ops": state.total_ops})
+        if state.epoch % 256 == 1:
+            print("long run at epoch", state.epoch, flush=True)
+        return state.epoch >= p["epochs"]
+
+    print("RunSimulation bff_noheads 2^17 seed 0 to epoch 1025", flush=True)
+    cubff.GetLanguage(p["lang"]).RunSimulation(params, None, cb)
+    assert seen == g["rows"]
+
+
 @pytest.mark.parametrize("lang", LANGS)
 def test_cli_debug_transcripts_match_reference(lang, tmp_path):
     # --run ... --debug: the state before every step, printed the way each

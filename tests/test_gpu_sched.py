@@ -224,8 +224,54 @@ def test_learned_buckets_cut_interpreter_time(capi):
             tot, it = b.run_epochs_timed(8, 12, sched)
             interp[name] = it
             total[name] = tot
-            soups[name] = [b.read_soup(i) for i in (0, 3, 62)]
-    for x, y in zip(soups["plain"], soups["learned"]):
-        assert np.array_equal(x, y)
+            soups[name] = [b.read_soup(i) for i in range(S)]
+    assert len(soups["plain"]) == S
+    for i, (x, y) in enumerate(zip(soups["plain"], soups["learned"])):
+        assert np.array_equal(x, y), i
     assert interp["learned"] < 0.95 * interp["plain"], interp
     assert total["learned"] < total["plain"], total
+
+
+def _config5():
+    """32 soups of 2^16 programs, seeds 100..131, bff on even seeds and forth
+    on odd ones, default mutation."""
+    seeds = list(range(100, 132))
+    return (["bff" if x % 2 == 0 else "forth" for x in seeds], [65536] * 32, seeds,
+            [1 << 18] * 32)
+
+
+def test_config5_learned_with_graph_is_bit_exact(capi):
+    # The learned scheduler's own workload, across the retraining at epoch
+    # 16: graph-replayed learned epochs give every one of the 32 soups the
+    # soup, counters, pairing and last per-pair results of plain epochs, and
+    # the first three epochs of one bff and one forth soup are the oracle's.
+    langs, ns, seeds, probs = _config5()
+    E = 22
+    refs = {i: O.Soup(langs[i], ns[i], seeds[i], probs[i], nthreads=16) for i in (0, 1)}
+    out = {}
+    for name, sched in (("learned", capi.SCHED_LEARNED), ("plain", capi.SCHED_PLAIN)):
+        print("config 5", name, flush=True)
+        with capi.Batch(langs, ns) as b:
+            b.init(seeds, probs)
+            if sched == capi.SCHED_LEARNED:
+                for e in range(3):
+                    b.run_epochs(e, 1, sched, use_graph=True)
+                    for i, ref in refs.items():
+                        tot, po, ps = ref.step()
+                        assert np.array_equal(b.read_soup(i), ref.soup), (langs[i], e)
+                        ops, steps = b.read_pair_stats(i)
+                        assert np.array_equal(ops, po) and np.array_equal(steps, ps), (i, e)
+                b.run_epochs(3, E - 3, sched, use_graph=True)
+                trainings, loss, buckets = b.sched_stats()
+                assert trainings >= 2 and np.isfinite(loss)
+                assert buckets.sum() == sum(ns) // 2
+            else:
+                b.run_epochs(0, E, sched, use_graph=False)
+            out[name] = [(b.read_soup(i), b.read_counters(i), b.read_shuffle(i),
+                          b.read_pair_stats(i)) for i in range(len(ns))]
+    for i in range(len(ns)):
+        (sa, ca, ha, pa), (sb, cb, hb, pb) = out["learned"][i], out["plain"][i]
+        assert np.array_equal(sa, sb), i
+        assert ca == cb, i
+        assert np.array_equal(ha, hb) and np.array_equal(ha, O.pairing_shuffle(ns[i], seeds[i], E - 1))
+        assert np.array_equal(pa[0], pb[0]) and np.array_equal(pa[1], pb[1]), i
