@@ -1,0 +1,406 @@
+// BFF interpreter of the standard jump logic (bff, bff_noheads,
+// bff_noheads_4bit, bff_perm) built for the shortest chain of instructions
+// per executed command.  It serves the budgeted runs of the deferring epoch
+// kernel, where an epoch lasts as long as its longest budgeted interaction,
+// and so costs what one lane's command costs.
+//
+// Per lane it keeps, beside the 128-byte tape, a 128-byte table `nx`: at
+// every command position, the next command position after it (128 if
+// none).  pc is always on a command, so a command reads its byte and
+// nx[pc] in the same LDS round trip as the bytes under the two heads, and
+// the run of comment bytes after it (no-op steps, bff.inc.h:346-347) is
+// skipped without any mask search.  Every command stores one byte (the byte
+// it writes, or for commands that write nothing the byte under head 0 back
+// in place), so no select picks a store address and no branch skips a
+// store.  A 256-byte table of byte classes (comment, head move, write, '[',
+// ']') shared by the workgroup tells whether a write changed a byte's
+// class; only then are the class masks, two nx entries and the jump caches
+// brought up to date (rare).  Bracket scans (bff.inc.h:317-344) walk the
+// masks and are cached per jump source, with the classes of the loop body
+// they enclose, until a class changes.
+//
+// Loop acceleration (exact): a taken backward jump from the same `]` as the
+// last one, with no class changed since, closes one iteration of the body
+// between the brackets.  If that body holds no bracket it runs straight
+// through, and its next iterations repeat the last one exactly when (a) it
+// holds no write -- the tape is constant, the heads advance by fixed
+// deltas, and the loop runs until the byte under head 0 at the `]` is null,
+// found by walking head 0's orbit (at most 128 reads) -- or (b) it holds no
+// head move and left the two bytes under the heads as they were: the whole
+// state repeats.  Whole iterations are then counted (steps and ops) without
+// being run, up to the step cap and the exiting iteration, which runs
+// normally.
+//
+// Steps add the skipped comment bytes to the executed commands, capped at
+// stepcount; ops (the value Evaluate returns, bff.inc.h:356-389) are the
+// executed commands plus those of skipped iterations.
+#pragma once
+
+#include "languages.h"
+
+namespace cubff {
+
+// Class of a byte under a BFF map: 0 comment (or null), 1 head move
+// (< > { }), 2 write (+ - . ,), 3 '[', 4 ']'.
+enum : uint32_t { kClsNone = 0, kClsMove, kClsWrite, kClsOpen, kClsClose };
+template <int MAP>
+__host__ __device__ __forceinline__ uint32_t bff_byte_class(uint32_t v) {
+  using C = BffClass<MAP>;
+  if (!C::cmd(v)) return kClsNone;
+  const uint32_t k = C::kind(v);
+  return k == kLoopStart ? kClsOpen
+                         : (k == kLoopEnd ? kClsClose : ((k - kPlus) < 4u ? kClsWrite : kClsMove));
+}
+
+// Positions strictly between a and b (-1 <= a < b <= 128) as a mask.
+__host__ __device__ __forceinline__ Mask128 mask_between(int a, int b) {
+  const auto upto = [](int n) {  // positions [0, n), 0 <= n <= 128
+    Mask128 m;
+    m.lo = n >= 64 ? ~0ULL : ((1ULL << (n & 63)) - 1);
+    m.hi = n >= 128 ? ~0ULL : (n <= 64 ? 0ULL : ((1ULL << (n & 63)) - 1));
+    return m;
+  };
+  const Mask128 hi = upto(b), lo = upto(a + 1);
+  return Mask128{hi.lo & ~lo.lo, hi.hi & ~lo.hi};
+}
+__host__ __device__ __forceinline__ bool mask_any(const Mask128 &m, const Mask128 &r) {
+  return ((m.lo & r.lo) | (m.hi & r.hi)) != 0;
+}
+
+// Keeps the loads issued before it together (device pass only).
+__host__ __device__ __forceinline__ void keep_loads_together() {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_sched_barrier(0);
+#endif
+}
+
+__host__ __device__ __forceinline__ uint32_t fast_umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ int fast_imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ int fast_imax(int a, int b) { return a > b ? a : b; }
+
+// Class masks of a tape: every command, '[', ']', writes, head moves.
+struct BffMasks {
+  Mask128 cmd, open, close, write, move;
+};
+
+template <int MAP, bool HEADS, bool ACCEL = true>
+struct BffFast {
+  using C = BffClass<MAP>;
+  // Per-kind 2-bit fields indexed by BffKind: head-0 delta + 1, head-1
+  // delta + 1, byte delta + 1 (the same tables as BffLang).
+  static constexpr uint32_t kTH0 = 0x58555;
+  static constexpr uint32_t kTH1 = 0x85555;
+  static constexpr uint32_t kTDelta = 0x55525;
+
+  // Class masks and nx from the tape bytes.
+  template <class T, class N, class L>
+  __host__ __device__ __forceinline__ static void build(const T &t, const N &nx, const L &lut,
+                                                        BffMasks &m) {
+    m = BffMasks{};
+    for (int w = 0; w < kTape2 / 8; w++) {
+      uint64_t bc = 0, bo = 0, bx = 0, bw = 0, bm = 0;
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const uint32_t c = lut(t.get(w * 8 + j));
+        bc |= (uint64_t)(c != kClsNone) << j;
+        bo |= (uint64_t)(c == kClsOpen) << j;
+        bx |= (uint64_t)(c == kClsClose) << j;
+        bw |= (uint64_t)(c == kClsWrite) << j;
+        bm |= (uint64_t)(c == kClsMove) << j;
+      }
+      const int sh = (w * 8) & 63;
+      if (w < 8) {
+        m.cmd.lo |= bc << sh; m.open.lo |= bo << sh; m.close.lo |= bx << sh;
+        m.write.lo |= bw << sh; m.move.lo |= bm << sh;
+      } else {
+        m.cmd.hi |= bc << sh; m.open.hi |= bo << sh; m.close.hi |= bx << sh;
+        m.write.hi |= bw << sh; m.move.hi |= bm << sh;
+      }
+    }
+    // nx[p] = smallest command position > p, from the top down (read at
+    // command positions only).
+    uint32_t next = kTape2;
+#pragma unroll 4
+    for (int p = kTape2 - 1; p >= 0; p--) {
+      nx.set(p, (uint8_t)next);
+      const uint64_t mm = p < 64 ? m.cmd.lo : m.cmd.hi;
+      next = ((mm >> (p & 63)) & 1) ? (uint32_t)p : next;
+    }
+  }
+
+  // Byte w changed class from cold to cnew: masks and nx follow (nx is
+  // kept at command positions: the command before w, and w itself).
+  // Returns the command before w (-1 if none) and in *next_of_prev its new
+  // next command.
+  template <class N>
+  __host__ __device__ static int class_change(const N &nx, BffMasks &m, int w, uint32_t cold,
+                                              uint32_t cnew, int *next_of_prev) {
+    const bool was = cold != kClsNone, is = cnew != kClsNone;
+    m.cmd.flip(w, was != is);
+    m.open.flip(w, (cold == kClsOpen) != (cnew == kClsOpen));
+    m.close.flip(w, (cold == kClsClose) != (cnew == kClsClose));
+    m.write.flip(w, (cold == kClsWrite) != (cnew == kClsWrite));
+    m.move.flip(w, (cold == kClsMove) != (cnew == kClsMove));
+    const int prev = m.cmd.prev_upto(w - 1);
+    const int after = m.cmd.next_from(w + 1);
+    *next_of_prev = is ? w : after;
+    if (was != is) {
+      if (prev >= 0) nx.set(prev, (uint8_t)*next_of_prev);
+      if (is) nx.set(w, (uint8_t)after);
+    }
+    return prev;
+  }
+
+  // Body classes of the loop closed by the `]` at src with its `[` at q.
+  __host__ __device__ __forceinline__ static void body_classes(const BffMasks &m, int q, int src,
+                                                               bool *straight, bool *write,
+                                                               bool *move) {
+    const Mask128 body = mask_between(q, src);
+    *straight = !mask_any(m.open, body) && !mask_any(m.close, body);
+    *write = mask_any(m.write, body);
+    *move = mask_any(m.move, body);
+  }
+
+  // Evaluate (bff.inc.h:356-389) with EvaluateOne (:289-349).  With a
+  // budget (>= 1) below 2^32 - 1, a run that would execute more than
+  // `budget` commands stops and reports *steps = kOverBudget (its tape is
+  // then partly run).  lut(v) is bff_byte_class<MAP>(v).  *work receives
+  // the commands executed.
+  template <class T, class N, class L>
+  __host__ __device__ static uint32_t evaluate(const T &t, const N &nx, const L &lut,
+                                               uint32_t stepcount, uint32_t *steps,
+                                               uint32_t *work, uint32_t budget) {
+    BffMasks m;
+    build(t, nx, lut, m);
+    int h0, h1, pc;
+    if constexpr (HEADS) {  // InitialState, bff.inc.h:269-275
+      h0 = t.get(0) & (kTape2 - 1);
+      h1 = t.get(1) & (kTape2 - 1);
+      pc = 2;
+    } else {  // bff.inc.h:277-282; 128 masks to 0
+      h0 = 0;
+      h1 = 0;
+      pc = 0;
+    }
+    uint32_t i, iters = 0, extra = 0;  // steps; commands executed; ops of skipped iterations
+    bool over = false;
+    {
+      const int first = m.cmd.next_from(pc);
+      i = fast_umin((uint32_t)(first - pc), stepcount);
+      pc = first;
+    }
+    if (i < stepcount && pc < kTape2) {
+      // Jump caches: source pc and matching bracket of the last backward
+      // jump (bq -1: unmatched) and forward jump (fq 128: unmatched), the
+      // next command after the target (nx at the matching bracket, kept up
+      // to date through class changes) and the comment bytes before it.
+      // For the backward jump, whether the body between its brackets is
+      // straight (no bracket), holds a write, holds a head move.
+      int bsrc = -1, bq = -1, bnext = 0, bgap = 0, fsrc = -1, fq = kTape2, fnext = 0, fgap = 0;
+      bool bstraight = false, bwrite = false, bmove = false;
+      // Loop anchor: the last taken backward jump's source (-1 none), the
+      // heads, steps and commands after it, the bytes under the heads.
+      int aq = -1, ah0 = 0, ah1 = 0;
+      uint32_t ai = 0, aiters = 0, aw0 = 0, aw1 = 0;
+      for (;;) {
+        // pc < 128 here; the mask lets the address be one 24-bit multiply-add.
+        const int pcm = pc & (kTape2 - 1);
+        const uint32_t c = t.get(pcm);
+        const uint32_t fall = nx.get(pcm);
+        const uint32_t v0 = t.get(h0);
+        const uint32_t v1 = t.get(h1);
+        keep_loads_together();
+        const uint32_t k = C::kind(c);
+        const uint32_t k2 = 2 * k;
+        const bool c01 = k == kCopy01, c10 = k == kCopy10;
+        const uint32_t wold = c01 ? v1 : v0;
+        const uint32_t vnew = ((c10 ? v1 : v0) + ((kTDelta >> k2) & 3) - 1) & 0xff;
+        const int waddr = c01 ? h1 : h0;
+        t.set(waddr, (uint8_t)vnew);
+        const uint32_t cold = lut(wold), cnew = lut(vnew);
+        h0 = (h0 + (int)((kTH0 >> k2) & 3) - 1) & (kTape2 - 1);
+        h1 = (h1 + (int)((kTH1 >> k2) & 3) - 1) & (kTape2 - 1);
+        const bool z = C::null(v0);
+        const bool tb = k == kLoopEnd && !z;
+        const bool tf = k == kLoopStart && z;
+        int nextc, gap;
+        if ((tb && pc != bsrc) || (tf && pc != fsrc)) {
+          // Matching bracket: backward to the `[` (unmatched: the run
+          // stops), forward to the `]` (unmatched: past the tape).
+          int depth = 1, p = tb ? pc - 1 : pc + 1, q;
+          for (;;) {
+            if (tb) {
+              const int ql = m.open.prev_upto(p), qe = m.close.prev_upto(p);
+              q = fast_imax(ql, qe);
+              if (q < 0) break;
+              depth += (q == qe) ? 1 : -1;
+              p = q - 1;
+            } else {
+              const int ql = m.open.next_from(p), qe = m.close.next_from(p);
+              q = fast_imin(ql, qe);
+              if (q >= kTape2) break;
+              depth += (q == ql) ? 1 : -1;
+              p = q + 1;
+            }
+            if (depth == 0) break;
+          }
+          if (tb) {
+            bsrc = pc;
+            bq = q;
+            bnext = nextc = q < 0 ? -1 : m.cmd.next_from(q + 1);
+            bgap = gap = q < 0 ? 0 : nextc - (q + 1);
+            if constexpr (ACCEL) {
+              body_classes(m, q, pc, &bstraight, &bwrite, &bmove);
+              bstraight = bstraight && q >= 0;
+            }
+          } else {
+            fsrc = pc;
+            fq = fast_imin(q, kTape2);
+            const int after = fast_imin(q + 1, kTape2);
+            fnext = nextc = m.cmd.next_from(after);
+            fgap = gap = nextc - after;
+          }
+        } else {
+          nextc = tb ? bnext : (tf ? fnext : (int)fall);
+          gap = tb ? bgap : (tf ? fgap : (int)fall - pc - 1);
+        }
+        if constexpr (ACCEL) {
+          // Any other bracket command between two visits of the anchor `]`
+          // means the path between them was not one body iteration.
+          if (k <= kLoopEnd && pc != aq) aq = -1;
+          if (tb && bstraight) {
+            const uint32_t ib = i + 1;  // steps after this `]`
+            if (pc == aq &&
+                (!bwrite || (!bmove && h0 == ah0 && h1 == ah1 && v0 == aw0 && v1 == aw1))) {
+              // One more iteration of the loop closed with the state of the
+              // last one advanced by fixed deltas: skip whole iterations.
+              const uint32_t len = ib - ai;
+              uint32_t reps = (stepcount - ib) / len;
+              const int d0 = (h0 - ah0) & (kTape2 - 1);
+              const int d1 = (h1 - ah1) & (kTape2 - 1);
+              if (d0 != 0) {
+                // First later iteration whose `]` sees a null byte (the
+                // tape is constant here: no write in the body).
+                for (uint32_t j = 1; j <= (uint32_t)kTape2 && j <= reps; j++) {
+                  if (C::null(t.get((h0 + j * d0) & (kTape2 - 1)))) {
+                    reps = j - 1;
+                    break;
+                  }
+                }
+              }
+              i += reps * len;
+              extra += reps * (iters + 1 - aiters);
+              h0 = (h0 + (int)(reps * d0)) & (kTape2 - 1);
+              h1 = (h1 + (int)(reps * d1)) & (kTape2 - 1);
+              aq = -1;
+            } else {
+              aq = pc;
+              ah0 = h0;
+              ah1 = h1;
+              ai = ib;
+              aiters = iters + 1;
+              aw0 = v0;
+              aw1 = v1;
+            }
+          }
+        }
+        if (cold != cnew) {
+          // Only a write changes a class, and a write is no jump.  A change
+          // that involves a bracket drops the jump caches; otherwise they
+          // stay (their next commands come from nx), and a change inside
+          // the cached loop's body updates its classes and drops the
+          // anchor.
+          int np;
+          const int prev = class_change(nx, m, waddr, cold, cnew, &np);
+          if (cold >= kClsOpen || cnew >= kClsOpen) {
+            // A bracket appeared or went: a cached match stands unless the
+            // change lies between the jump and its match (or past the jump
+            // where it found none).
+            if (waddr >= bq && waddr <= bsrc) bsrc = -1;
+            if (waddr >= fsrc && waddr <= fq) fsrc = -1;
+          }
+          if constexpr (ACCEL) {
+            if (bsrc >= 0 && waddr > bq && waddr < bsrc) {
+              body_classes(m, bq, bsrc, &bstraight, &bwrite, &bmove);
+              aq = -1;
+            }
+            if (bsrc < 0) aq = -1;
+          }
+          if (prev == pc) {
+            nextc = np;
+            gap = np - pc - 1;
+          }
+          // The next commands after the cached matches are nx there.
+          if (bq >= 0 && prev == bq) {
+            bnext = np;
+            bgap = np - bq - 1;
+          }
+          if (prev == fq) {
+            fnext = np;
+            fgap = np - fq - 1;
+          }
+        }
+        i += 1 + (uint32_t)gap;
+        iters++;
+        const bool fin = (uint32_t)nextc >= (uint32_t)kTape2 || i >= stepcount;
+        if (fin || iters >= budget) {
+          over = !fin;
+          break;
+        }
+        pc = nextc;
+      }
+    }
+    if (over) {
+      *steps = kOverBudget;
+      return 0;
+    }
+    *steps = fast_umin(i, stepcount);
+    if (work) *work = iters;
+    return iters + extra;
+  }
+};
+
+// Device accessors: the workgroup's byte-class table in LDS.
+struct LdsByteLut {
+  const lds_u8 *p;
+  __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return p[v]; }
+};
+
+// Fills a workgroup's byte-class table (every thread of the block calls it
+// before a __syncthreads()).
+template <int MAP>
+__device__ __forceinline__ void fill_byte_lut(uint8_t *lut) {
+  for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x) lut[v] = (uint8_t)bff_byte_class<MAP>(v);
+}
+
+}  // namespace cubff
+
+namespace cubff {
+
+// The languages that BffFast interprets (the standard jump logic), with
+// the LDS it needs beside the tape: nx (a LinTape-shaped region) and the
+// workgroup's byte-class table.
+template <int LANG> struct FastOf { static constexpr bool kHas = false; };
+template <> struct FastOf<kBff> { static constexpr bool kHas = true; using T = BffFast<kMapAscii, true>; static constexpr int kMap = kMapAscii; };
+template <> struct FastOf<kBffNoheads> { static constexpr bool kHas = true; using T = BffFast<kMapAscii, false>; static constexpr int kMap = kMapAscii; };
+template <> struct FastOf<kBffNoheads4bit> { static constexpr bool kHas = true; using T = BffFast<kMap4bit, false>; static constexpr int kMap = kMap4bit; };
+template <> struct FastOf<kBffPerm> { static constexpr bool kHas = true; using T = BffFast<kMapPerm, true>; static constexpr int kMap = kMapPerm; };
+
+// Second per-lane LDS region of a kernel over LANES lanes: the language's
+// aux region, or nx where BffFast runs (whichever is larger).
+template <int LANG, int LANES>
+constexpr int second_region_bytes() {
+  using L = typename LangOf<LANG>::T;
+  constexpr int aux = lang_needs_aux(LANG) ? L::template Aux<LANES>::kRegionBytes : 0;
+  constexpr int nx = FastOf<LANG>::kHas ? LinTape<LANES>::kRegionBytes : 0;
+  return aux > nx ? aux : nx;
+}
+
+// Fills the byte-class table of a workgroup running BffFast for LANG (a
+// no-op for other languages); every thread calls it before a barrier.
+template <int LANG>
+__device__ __forceinline__ void fill_fast_lut(uint8_t *lut) {
+  if constexpr (FastOf<LANG>::kHas) fill_byte_lut<FastOf<LANG>::kMap>(lut);
+}
+
+}  // namespace cubff

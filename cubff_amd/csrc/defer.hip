@@ -27,6 +27,7 @@
 
 #include "device_common.h"
 #include "kernels.h"
+#include "bff_fast.h"
 #include "languages.h"
 #include "tape_io.h"
 
@@ -66,11 +67,17 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<kEpochLanes>;
   using Aux = typename L::template Aux<kEpochLanes>;
-  constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
+  // Budgeted runs of the standard BFF languages go through BffFast, whose
+  // next-command table shares the second region with the aux tape of a
+  // complete run (the two never overlap in time).
+  constexpr bool kFast = FastOf<LANG>::kHas;
+  constexpr int kAuxBytes = second_region_bytes<LANG, kEpochLanes>();
   __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
+  __shared__ uint8_t lut[kFast ? 256 : 1];
   __shared__ unsigned long long block_ops, block_steps;
   __shared__ uint32_t block_budget;
   const int lane = threadIdx.x;
+  fill_fast_lut<LANG>(lut);
   if (lane == 0) {
     block_ops = 0;
     block_steps = 0;
@@ -118,7 +125,14 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
       load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
       load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
       mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel), k, a.prob);
-      if (eval) ops = L::template evaluate<true>(t, aux, kEvalSteps, &steps, nullptr, budget);
+      if (eval) {
+        if constexpr (kFast)
+          ops = FastOf<LANG>::T::evaluate(t, LinTape<kEpochLanes>::at(lds + Tape::kRegionBytes, lane),
+                                          LdsByteLut{(const lds_u8 *)lut}, kEvalSteps, &steps,
+                                          nullptr, budget);
+        else
+          ops = L::template evaluate<true>(t, aux, kEvalSteps, &steps, nullptr, budget);
+      }
       if (steps == kOverBudget) {
         defer = true;
         ops = steps = 0;

@@ -149,19 +149,20 @@ struct LinTape {
 // 128-bit per-lane position masks (two 64-bit halves held in VGPRs).
 struct Mask128 {
   uint64_t lo, hi;
-  __device__ __forceinline__ void flip(int p, bool f) {
+  __host__ __device__ __forceinline__ void flip(int p, bool f) {
     const uint64_t b = (uint64_t)f << (p & 63);
-    if (p < 64) lo ^= b; else hi ^= b;
+    lo ^= p < 64 ? b : 0ULL;
+    hi ^= p < 64 ? 0ULL : b;
   }
   // Smallest set position >= p (0 <= p <= 128), or 128.  Branch-free.
-  __device__ __forceinline__ int next_from(int p) const {
+  __host__ __device__ __forceinline__ int next_from(int p) const {
     const uint64_t l = lo & (p >= 64 ? 0ULL : ~0ULL << (p & 63));
     const uint64_t h = hi & (p <= 64 ? ~0ULL : (p >= 128 ? 0ULL : ~0ULL << (p & 63)));
     const int cl = __builtin_ctzg(l, 64), ch = __builtin_ctzg(h, 64);
     return l ? cl : 64 + ch;
   }
   // Largest set position <= p (-1 <= p <= 127), or -1.
-  __device__ __forceinline__ int prev_upto(int p) const {
+  __host__ __device__ __forceinline__ int prev_upto(int p) const {
     const uint64_t h = p < 64 ? 0 : (p >= 127 ? hi : hi & (~0ULL >> (127 - p)));
     const uint64_t l = p < 0 ? 0 : (p >= 63 ? lo : lo & (~0ULL >> (63 - p)));
     return h ? 127 - __builtin_clzll(h) : (l ? 63 - __builtin_clzll(l) : -1);
@@ -176,7 +177,7 @@ struct Mask128 {
 // > { }; the other maps index by the byte (or its low nibble) directly.
 template <int MAP>
 struct BffClass {
-  __device__ __forceinline__ static bool cmd(uint32_t v) {
+  __host__ __device__ __forceinline__ static bool cmd(uint32_t v) {
     if constexpr (MAP == kMapAscii) {
       // + , - . < > (0x2b-0x2e, 0x3c, 0x3e) and [ ] { } (0x5b, 0x5d, 0x7b, 0x7d).
       constexpr uint64_t kLo = (0xfULL << 0x2b) | (1ULL << 0x3c) | (1ULL << 0x3e);
@@ -191,13 +192,13 @@ struct BffClass {
       return v < 7;
     }
   }
-  __device__ __forceinline__ static bool ls(uint32_t v) {
+  __host__ __device__ __forceinline__ static bool ls(uint32_t v) {
     if constexpr (MAP == kMapAscii) return v == '[';
     else if constexpr (MAP == kMap4bit) return (v & 15) == 6;
     else if constexpr (MAP == kMapPerm) return v == 9;
     else return v == 5;
   }
-  __device__ __forceinline__ static bool le(uint32_t v) {
+  __host__ __device__ __forceinline__ static bool le(uint32_t v) {
     if constexpr (MAP == kMapAscii) return v == ']';
     else if constexpr (MAP == kMap4bit) return (v & 15) == 7;
     else if constexpr (MAP == kMapPerm) return v == 10;
@@ -205,12 +206,12 @@ struct BffClass {
   }
   // Loop-test "byte counts as zero": kind == kNull for the standard logic
   // (bff.inc.h:318, :332), the byte itself for bff8 / selfmove.
-  __device__ __forceinline__ static bool null(uint32_t v) {
+  __host__ __device__ __forceinline__ static bool null(uint32_t v) {
     if constexpr (MAP == kMap4bit) return (v & 15) == 0;
     else return v == 0;
   }
   // BffKind of a command byte (undefined for comments / null).
-  __device__ __forceinline__ static uint32_t kind(uint32_t c) {
+  __host__ __device__ __forceinline__ static uint32_t kind(uint32_t c) {
     uint32_t h;
     uint64_t table;
     if constexpr (MAP == kMapAscii) {

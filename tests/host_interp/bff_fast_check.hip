@@ -1,0 +1,127 @@
+// CPU check of the budgeted BFF interpreter (cubff_amd/csrc/bff_fast.h)
+// against the oracle's Evaluate: the header's template compiled for the
+// host, run on random tapes at random step limits and command budgets.
+// Test infrastructure: built and run by tests/test_cpu_interp.py.
+//
+//   bff_fast_check <tapes per language> <seed>
+// prints one line per language and exits non-zero on the first mismatch.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../cubff_amd/csrc/bff_fast.h"
+#include "../../oracle/cubff_oracle.h"
+
+using namespace cubff;
+
+namespace {
+
+struct HostBytes {
+  uint8_t *p;
+  uint32_t get(uint32_t i) const { return p[i]; }
+  void set(uint32_t i, uint8_t v) const { p[i] = v; }
+};
+
+template <int MAP>
+struct HostLut {
+  uint8_t t[256];
+  HostLut() {
+    for (int v = 0; v < 256; v++) t[v] = (uint8_t)bff_byte_class<MAP>(v);
+  }
+  uint32_t operator()(uint32_t v) const { return t[v & 255]; }
+};
+
+uint64_t sm(uint64_t x) { return orc_splitmix64(x); }
+
+// Tapes with many commands, loops and heads near the code: byte j of tape
+// r is a command character with probability 5/8, else a random byte (for
+// the permuted and 4-bit maps, small values or random low nibbles).
+void make_tape(int map, uint64_t seed, uint8_t *t) {
+  static const char kCmds[] = "[]+-.,<>{}";
+  for (int j = 0; j < 128; j++) {
+    const uint64_t r = sm(seed * 131 + j);
+    uint8_t v;
+    if (map == kMapPerm) {
+      v = (r & 7) < 6 ? (uint8_t)((r >> 8) % 12) : (uint8_t)(r >> 16);
+    } else if (map == kMap4bit) {
+      v = (uint8_t)(r >> 8);
+      if ((r & 7) < 5) v = (uint8_t)((v & 0xf0) | (6 + (r >> 20) % 10));
+    } else {
+      v = (r & 7) < 5 ? (uint8_t)kCmds[(r >> 8) % 10] : (uint8_t)(r >> 16);
+      if ((r & 63) == 7) v = 0;
+    }
+    t[j] = v;
+  }
+}
+
+template <int MAP, bool HEADS, bool ACCEL>
+int check(const char *name, int orc_lang, int count, uint64_t seed) {
+  HostLut<MAP> lut;
+  uint8_t tape[129], ref[128], nxb[129];
+  uint64_t cap_runs = 0, over = 0, ops_sum = 0;
+  for (int r = 0; r < count; r++) {
+    make_tape(MAP, seed * 1000003 + r, tape);
+    const uint64_t x = sm(seed ^ (0x9e37ull * (r + 1)));
+    const uint32_t caps[4] = {8192, (uint32_t)(x % 300), 1 + (uint32_t)(x >> 20) % 3000, 0};
+    const uint32_t stepcount = caps[(x >> 40) & 3];
+    const uint32_t budget = ((x >> 44) & 3) == 0 ? 0xffffffffu : 1 + (uint32_t)((x >> 46) % 400);
+    memcpy(ref, tape, 128);
+    uint64_t ref_steps = 0;
+    const uint64_t ref_ops = orc_evaluate(orc_lang, ref, stepcount, &ref_steps);
+    uint32_t steps = 0, work = 0;
+    const uint32_t ops = BffFast<MAP, HEADS, ACCEL>::evaluate(HostBytes{tape}, HostBytes{nxb}, lut,
+                                                             stepcount, &steps, &work, budget);
+    if (steps == kOverBudget) {
+      over++;
+      // Executed commands never exceed ops (skipped iterations hold ops).
+      if (ref_ops <= budget) {
+        printf("%s tape %d: over budget %u but the run has %llu commands\n", name, r, budget,
+               (unsigned long long)ref_ops);
+        return 1;
+      }
+      continue;
+    }
+    if (!ACCEL && budget != 0xffffffffu && ref_ops > budget) {
+      printf("%s tape %d: %llu commands under budget %u not reported\n", name, r,
+             (unsigned long long)ref_ops, budget);
+      return 1;
+    }
+    if (ops != ref_ops || steps != ref_steps || work > ops || (!ACCEL && work != ops) ||
+        (budget != 0xffffffffu && work > budget) || memcmp(tape, ref, 128) != 0) {
+      printf("%s tape %d cap %u: ops %u/%llu steps %u/%llu tape %s\n", name, r, stepcount, ops,
+             (unsigned long long)ref_ops, steps, (unsigned long long)ref_steps,
+             memcmp(tape, ref, 128) ? "differs" : "same");
+      return 1;
+    }
+    cap_runs += steps == stepcount;
+    ops_sum += ops;
+  }
+  printf("%s: %d tapes ok (%llu at the step limit, %llu over budget, %llu ops)\n", name, count,
+         (unsigned long long)cap_runs, (unsigned long long)over, (unsigned long long)ops_sum);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  const int count = argc > 1 ? atoi(argv[1]) : 20000;
+  const uint64_t seed = argc > 2 ? strtoull(argv[2], nullptr, 10) : 1;
+  int bad = 0;
+  for (int accel = 0; accel < 2; accel++) {
+    printf("loop acceleration %s\n", accel ? "on" : "off");
+    if (accel) {
+      bad |= check<kMapAscii, true, true>("bff", ORC_BFF, count, seed);
+      bad |= check<kMapAscii, false, true>("bff_noheads", ORC_BFF_NOHEADS, count, seed + 1);
+      bad |= check<kMap4bit, false, true>("bff_noheads_4bit", ORC_BFF_NOHEADS_4BIT, count, seed + 2);
+      bad |= check<kMapPerm, true, true>("bff_perm", ORC_BFF_PERM, count, seed + 3);
+    } else {
+      bad |= check<kMapAscii, true, false>("bff", ORC_BFF, count, seed);
+      bad |= check<kMapAscii, false, false>("bff_noheads", ORC_BFF_NOHEADS, count, seed + 1);
+      bad |= check<kMap4bit, false, false>("bff_noheads_4bit", ORC_BFF_NOHEADS_4BIT, count, seed + 2);
+      bad |= check<kMapPerm, true, false>("bff_perm", ORC_BFF_PERM, count, seed + 3);
+    }
+  }
+  return bad;
+}

@@ -40,6 +40,9 @@ def test_evaluate_and_save(capi, tmp_path):
     _, ops_ref, steps_ref = O.evaluate("bff", t, 500)
     _, _, steps_one_more = O.evaluate("bff", t, 501)
     assert ops == trace.reference_return(ops_ref, steps_ref, 500, steps_one_more)
+    states = trace.read_states(path)
+    assert len(states) == steps_ref
+    assert states[0][3] == t.tobytes()
     # At the step cap the reference returns one less than Evaluate.
     loop = np.full(128, ord("a"), dtype=np.uint8)
     loop[0], loop[1], loop[2], loop[3] = 5, 5, ord("["), ord("]")
@@ -47,9 +50,6 @@ def test_evaluate_and_save(capi, tmp_path):
     assert steps_cap == 300
     assert trace.evaluate_and_save(bytearray(loop.tobytes()), path, 300) == ops_cap - 1
     assert len(trace.read_states(path)) == 300
-    states = trace.read_states(path)
-    assert len(states) == steps_ref
-    assert states[0][3] == t.tobytes()
     with pytest.raises(ValueError):
         trace.evaluate_and_save(bytearray(64), path, 10)
     with pytest.raises(Exception):
