@@ -161,15 +161,50 @@ struct BffFast {
     *move = mask_any(m.move, body);
   }
 
-  // Evaluate (bff.inc.h:356-389) with EvaluateOne (:289-349).  With a
-  // budget (>= 1) below 2^32 - 1, a run that would execute more than
-  // `budget` commands stops and reports *steps = kOverBudget (its tape is
-  // then partly run).  lut(v) is bff_byte_class<MAP>(v).  *work receives
-  // the commands executed.
+  // Full-state cycle skip of complete runs (exact): from step kCycleStart
+  // on, the state right after a taken backward jump is compared with an
+  // anchor whose tape is kept in a second per-lane region: pc, heads, a
+  // weighted sum of the tape changes since the anchor (zero when the tape
+  // is back to the anchor's) and, only when those match, every tape word.
+  // A match means the run repeats with that period: whole periods are
+  // counted without being run.  The first anchor stays for kCycleFirstWin
+  // steps (the long runs are mostly cycles that start within ~150 steps
+  // with periods of 1.5K-4K steps, a head sweeping the tape), later anchors
+  // follow Brent's scheme with windows kCycleWin, 2 kCycleWin, ...
+  static constexpr uint32_t kCycleStart = 128;
+  static constexpr uint32_t kCycleFirstWin = 4096;
+  static constexpr uint32_t kCycleWin = 1024;
+
+  // Tape accessor without a second region (runs without the cycle skip).
+  struct NoAux {
+    __host__ __device__ uint32_t word(uint32_t) const { return 0; }
+    __host__ __device__ void set_word(uint32_t, uint32_t) const {}
+  };
+
+  // Evaluate (bff.inc.h:356-389) with EvaluateOne (:289-349), for a
+  // budgeted run.  With a budget (>= 1) below 2^32 - 1, a run whose
+  // executed commands plus orbit-walk reads would exceed `budget` stops and
+  // reports *steps = kOverBudget (its tape is then partly run).  lut(v) is
+  // bff_byte_class<MAP>(v).  *work receives the commands executed.
   template <class T, class N, class L>
   __host__ __device__ static uint32_t evaluate(const T &t, const N &nx, const L &lut,
                                                uint32_t stepcount, uint32_t *steps,
                                                uint32_t *work, uint32_t budget) {
+    return run<false>(t, nx, NoAux{}, lut, stepcount, steps, work, budget);
+  }
+  // A complete run with the full-state cycle skip; aux is a second
+  // 128-byte per-lane region (the anchor tape).
+  template <class T, class N, class A, class L>
+  __host__ __device__ static uint32_t evaluate_full(const T &t, const N &nx, const A &aux,
+                                                    const L &lut, uint32_t stepcount,
+                                                    uint32_t *steps, uint32_t *work) {
+    return run<true>(t, nx, aux, lut, stepcount, steps, work, 0xffffffffu);
+  }
+
+  template <bool CYCLE, class T, class N, class A, class L>
+  __host__ __device__ static uint32_t run(const T &t, const N &nx, const A &aux, const L &lut,
+                                          uint32_t stepcount, uint32_t *steps, uint32_t *work,
+                                          uint32_t budget) {
     BffMasks m;
     build(t, nx, lut, m);
     int h0, h1, pc;
@@ -182,7 +217,9 @@ struct BffFast {
       h1 = 0;
       pc = 0;
     }
-    uint32_t i, iters = 0, extra = 0;  // steps; commands executed; ops of skipped iterations
+    // Steps; commands executed; ops of skipped iterations; budget spent
+    // (commands plus the tape reads of loop acceleration's orbit walks).
+    uint32_t i, iters = 0, extra = 0, spent = 0;
     bool over = false;
     {
       const int first = m.cmd.next_from(pc);
@@ -202,6 +239,11 @@ struct BffFast {
       // heads, steps and commands after it, the bytes under the heads.
       int aq = -1, ah0 = 0, ah1 = 0;
       uint32_t ai = 0, aiters = 0, aw0 = 0, aw1 = 0;
+      // Cycle anchor: jump source (-1 none yet, -2 done), heads, steps and
+      // ops after it, window, weighted sum of the tape changes since.
+      int cq = -1, ch0 = 0, ch1 = 0;
+      uint32_t ci = 0, cops = 0, cwin = kCycleFirstWin, cdelta = 0;
+      bool cfirst = true;
       for (;;) {
         // pc < 128 here; the mask lets the address be one 24-bit multiply-add.
         const int pcm = pc & (kTape2 - 1);
@@ -217,6 +259,7 @@ struct BffFast {
         const uint32_t vnew = ((c10 ? v1 : v0) + ((kTDelta >> k2) & 3) - 1) & 0xff;
         const int waddr = c01 ? h1 : h0;
         t.set(waddr, (uint8_t)vnew);
+        if constexpr (CYCLE) cdelta += (vnew - wold) * (2654435761u * (2u * (uint32_t)waddr + 1u));
         const uint32_t cold = lut(wold), cnew = lut(vnew);
         h0 = (h0 + (int)((kTH0 >> k2) & 3) - 1) & (kTape2 - 1);
         h1 = (h1 + (int)((kTH1 >> k2) & 3) - 1) & (kTape2 - 1);
@@ -268,6 +311,7 @@ struct BffFast {
           // Any other bracket command between two visits of the anchor `]`
           // means the path between them was not one body iteration.
           if (k <= kLoopEnd && pc != aq) aq = -1;
+          bool skipped = false;
           if (tb && bstraight) {
             const uint32_t ib = i + 1;  // steps after this `]`
             if (pc == aq &&
@@ -281,18 +325,21 @@ struct BffFast {
               if (d0 != 0) {
                 // First later iteration whose `]` sees a null byte (the
                 // tape is constant here: no write in the body).
-                for (uint32_t j = 1; j <= (uint32_t)kTape2 && j <= reps; j++) {
+                uint32_t j = 1;
+                for (; j <= (uint32_t)kTape2 && j <= reps; j++) {
                   if (C::null(t.get((h0 + j * d0) & (kTape2 - 1)))) {
                     reps = j - 1;
                     break;
                   }
                 }
+                spent += j;
               }
               i += reps * len;
               extra += reps * (iters + 1 - aiters);
               h0 = (h0 + (int)(reps * d0)) & (kTape2 - 1);
               h1 = (h1 + (int)(reps * d1)) & (kTape2 - 1);
               aq = -1;
+              skipped = true;
             } else {
               aq = pc;
               ah0 = h0;
@@ -301,6 +348,33 @@ struct BffFast {
               aiters = iters + 1;
               aw0 = v0;
               aw1 = v1;
+            }
+          }
+          if constexpr (CYCLE) {
+            const uint32_t ib = i + 1;  // steps after this `]`
+            if (tb && !skipped && nextc >= 0 && ib >= kCycleStart && cq != -2) {
+              bool same = pc == cq && h0 == ch0 && h1 == ch1 && cdelta == 0;
+              for (int w = 0; same && w < kTape2 / 4; w++) same = t.word(w) == aux.word(w);
+              if (same) {
+                const uint32_t len = ib - ci;
+                const uint32_t reps = (stepcount - ib) / len;
+                i += reps * len;
+                extra += reps * (iters + 1 + extra - cops);
+                cq = -2;  // fewer than one period of steps is left
+                aq = -1;  // the loop anchor's counts predate the skip
+              } else if (cq < 0 || ib - ci >= cwin) {
+                if (cq >= 0) {
+                  cwin = cfirst ? kCycleWin : (cwin < (1u << 29) ? cwin * 2 : cwin);
+                  cfirst = false;
+                }
+                for (int w = 0; w < kTape2 / 4; w++) aux.set_word(w, t.word(w));
+                cq = pc;
+                ch0 = h0;
+                ch1 = h1;
+                ci = ib;
+                cops = iters + 1 + extra;
+                cdelta = 0;
+              }
             }
           }
         }
@@ -342,8 +416,9 @@ struct BffFast {
         }
         i += 1 + (uint32_t)gap;
         iters++;
+        spent++;
         const bool fin = (uint32_t)nextc >= (uint32_t)kTape2 || i >= stepcount;
-        if (fin || iters >= budget) {
+        if (fin || spent >= budget) {
           over = !fin;
           break;
         }

@@ -51,6 +51,8 @@ struct cubff_ctx {
   unsigned long long *hist = nullptr;  // byte histogram, 256 entries
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
+  uint32_t *dqueue = nullptr;   // ready queue of the resolver, defer_cap slots
+  uint64_t dqueue_dirty = 0;    // slots the last segment filled (cleared before the next)
   DeferCtl *dctl = nullptr;
   uint64_t defer_cap = 0;
   uint32_t defer_gen = 0;
@@ -124,6 +126,10 @@ constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
 constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
+// How long an epoch kernel's lane waits for the deferred interaction that
+// owes one of its programs before deferring its own (~half an epoch kernel
+// at 2^17 programs).
+constexpr double kOweWaitS = 200e-6;
 // Deferring segments end by waiting for the resolver to finish the last
 // epochs' long interactions: long segments (bounded by kFlowBytes of list
 // entries, 16 bytes per program and epoch) amortize that wait.
@@ -157,9 +163,10 @@ bool flow_pays(int lang, uint64_t n) {
 }
 
 // Deferred-interaction segments: the list holds every pair slot of the
-// segment in the worst case (32 bytes each), within kFlowBytes.
+// segment in the worst case (a 64-byte entry and a 4-byte ready-queue slot
+// each), within kFlowBytes.
 uint64_t defer_segment_epochs(uint64_t n) {
-  return std::max<uint64_t>(2, std::min(kFlowBytes / (16 * n), kDeferMaxEpochs));
+  return std::max<uint64_t>(2, std::min(kFlowBytes / (34 * n), kDeferMaxEpochs));
 }
 
 uint64_t flow_segment_epochs(uint64_t n) {
@@ -412,12 +419,19 @@ int ensure_defer_scratch(cubff_ctx *c, uint64_t epochs) {
   if (!c->dctl && (r = dev_alloc(&c->dctl, 1))) return r;
   if (c->defer_cap < need) {
     dev_free(c->ditems);
+    dev_free(c->dqueue);
     c->defer_cap = 0;
-    if ((r = dev_alloc(&c->ditems, need))) return r;
-    // Generation words start at 0; runs use generations >= 1.
+    if ((r = dev_alloc(&c->ditems, need)) || (r = dev_alloc(&c->dqueue, need))) return r;
+    // Generation words start at 0; runs use generations >= 1.  Queue slots
+    // start empty.
     HIP_TRY(hipMemsetAsync(c->ditems, 0, need * sizeof(DeferItem), c->stream));
+    HIP_TRY(launch_fill_u32(c->stream, c->dqueue, kNone, need));
+    c->dqueue_dirty = 0;
     c->defer_cap = need;
   }
+  // Slots the last segment filled start empty again.
+  if (c->dqueue_dirty) HIP_TRY(launch_fill_u32(c->stream, c->dqueue, kNone, c->dqueue_dirty));
+  c->dqueue_dirty = 0;
   return CUBFF_OK;
 }
 
@@ -435,6 +449,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.soup = c->soup;
   a.owner = c->owner;
   a.items = c->ditems;
+  a.queue = c->dqueue;
   a.ctl = c->dctl;
   a.n = n;
   a.slots = (uint32_t)(n / 2);
@@ -452,6 +467,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.counters = c->counters;
   a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
   a.idle_ticks = (uint64_t)(std::min(c->resolver_idle_s, c->flow_wait_s) * khz * 1000.0);
+  a.owe_ticks = (uint64_t)(std::min(kOweWaitS, c->flow_wait_s) * khz * 1000.0);
   hipEvent_t started = nullptr, resolved = nullptr;
   std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
   auto cleanup = [&]() {
@@ -467,6 +483,8 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   if (e == hipSuccess) e = hipEventCreateWithFlags(&resolved, hipEventDisableTiming);
   if (e == hipSuccess) e = launch_fill_u32(c->stream, c->owner, kNone, n);
   if (e == hipSuccess) e = launch_defer_start(c->stream, c->dctl);
+  // Until this segment ends well, every queue slot counts as used.
+  c->dqueue_dirty = c->defer_cap;
   if (e == hipSuccess) e = hipEventRecord(started, c->stream);
   if (e == hipSuccess) e = hipStreamWaitEvent(c->stream2, started, 0);
   if (e == hipSuccess) e = launch_resolve(c->stream2, c->lang, a, kResolveWaves);
@@ -518,6 +536,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   // the run is complete as long as every deferred entry was run.
   const uint32_t listed = ctl.final_tail >= kDeferGone ? 0u : ctl.final_tail;
   c->defer_items += listed;
+  c->dqueue_dirty = std::min<uint64_t>(std::max(listed, ctl.qtail), c->defer_cap);
   if (ctl.resolved != listed)
     return fail(CUBFF_EHIP, "deferred-interaction resolver stopped: a wait exceeded " +
                                 std::to_string(c->flow_wait_s) + " s; soup state is undefined");
@@ -747,6 +766,7 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   dev_free(c->owner);
   dev_free(c->ditems);
+  dev_free(c->dqueue);
   dev_free(c->dctl);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream3) (void)hipStreamSynchronize(c->stream3);

@@ -13,16 +13,39 @@
 // read by exactly the interaction that follows its last writer, as in the
 // epoch-by-epoch loop, and the results are bit-identical to it.
 //
+// The resolver runs an entry only once the entries it depends on are done:
+// each entry counts its unfinished dependencies and links itself to each
+// of them (one successor per program of the dependency); whoever finishes
+// the last dependency -- the resolver wavefront that ran it, or the
+// creator if it was already done when linking -- puts the entry on a
+// ready queue, from which resolver wavefronts take work.  No wavefront
+// ever holds an entry while it waits, so however long the chains behind a
+// long interaction grow, every wavefront stays free for runnable work.
+//
 // owner[p] names p's pending writer: kNone (the last writer was an epoch
 // kernel, visible to later kernels of the same stream), a list index (a
 // deferred interaction will write p), or kResolved (the resolver wrote p: an
 // epoch kernel reading it takes an acquire fence).  A deferred interaction
 // records the pending writers of its two programs as dependencies.  The
-// resolver publishes a result with a release fence before marking it done
-// and handing the programs back (owner: index -> kResolved).  List entries
-// carry the run's generation number in their ready/done words, so the list
-// is never cleared.  Every wait is bounded by a wall-clock limit; past it
-// the abort flag stops the resolver and the host reports the error.
+// resolver publishes a result before marking it done and handing the
+// programs back (owner: index -> kResolved).  List entries carry the run's
+// generation number in their ready/done words, so the list is never
+// cleared.  Every wait is bounded by a wall-clock limit; past it the abort
+// flag stops the resolver and the host reports the error.
+//
+// No cache-maintaining fence is used between the two kernels.  On gfx950 an
+// agent-scope acquire or release writes back or invalidates the whole L2 of
+// the wavefront's XCD, which slowed every memory-bound kernel running
+// beside them (the pairing kernels of later epochs, 20x).  Instead, every
+// value that crosses between the concurrently running kernels -- list
+// entries, the soup rows the resolver reads and writes, the rows of
+// programs the resolver wrote -- is read and written with agent-scope
+// (device-coherent) atomic accesses, and a writer waits until its stores
+// have completed (s_waitcnt vmcnt(0)) before it raises the flag that
+// publishes them; a reader issues its data loads only after it has read
+// the flag.  Rows that only ever pass from one epoch kernel to a later one
+// on the same stream use plain accesses (the end of a kernel writes back
+// its L2).
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -40,6 +63,64 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t *p) {
 }
 __device__ __forceinline__ void st_agent(uint32_t *p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every store this wavefront has issued has completed: s_waitcnt vmcnt(0)
+// (expcnt and lgkmcnt left at their maxima).  The workgroup-scope fences
+// around it emit no instruction; they keep the compiler from moving memory
+// accesses across this point.
+__device__ __forceinline__ void publish_point() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+}
+// After a flag was read: the compiler keeps later loads after it (the
+// hardware issues them after the flag's value decided the branch).
+__device__ __forceinline__ void after_flag() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+
+// A soup row moved between the soup and a lane's tape with 8-byte
+// device-coherent accesses.
+template <class Tape>
+__device__ __forceinline__ void load_row_coherent(const Tape &t, int half, const uint8_t *src) {
+  uint64_t *s = reinterpret_cast<uint64_t *>(const_cast<uint8_t *>(src));
+  uint64_t v[kTape / 8];
+#pragma unroll
+  for (int q = 0; q < kTape / 8; q++)
+    v[q] = __hip_atomic_load(s + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int q = 0; q < kTape / 8; q++) {
+    t.set_word(half * 16 + 2 * q, (uint32_t)v[q]);
+    t.set_word(half * 16 + 2 * q + 1, (uint32_t)(v[q] >> 32));
+  }
+}
+template <class Tape>
+__device__ __forceinline__ void store_row_coherent(const Tape &t, int half, uint8_t *dst) {
+  uint64_t *d = reinterpret_cast<uint64_t *>(dst);
+#pragma unroll
+  for (int q = 0; q < kTape / 8; q++) {
+    const uint64_t v = (uint64_t)t.word(half * 16 + 2 * q) |
+                       ((uint64_t)t.word(half * 16 + 2 * q + 1) << 32);
+    __hip_atomic_store(d + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Puts list entry x on the ready queue.
+__device__ __forceinline__ void push_ready(const DeferArgs &a, uint32_t x) {
+  const uint32_t q = __hip_atomic_fetch_add(&a.ctl->qtail, 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+  if (q < a.capacity) st_agent(a.queue + q, x);
+}
+
+// Waits until *p == want, or until the wall clock passes `deadline`;
+// false then (nothing is raised: the caller goes on without).
+__device__ bool wait_eq_until(const DeferCtl *c, const uint32_t *p, uint32_t want,
+                              uint64_t deadline) {
+  uint32_t nap = 1;
+  for (;;) {
+    if (ld_agent(p) == want) return true;
+    if (ld_agent(&c->abort) || wall_clock64() > deadline) return false;
+    for (uint32_t q = 0; q < nap; q++) __builtin_amdgcn_s_sleep(1);
+    nap = min(nap * 2, 16u);
+  }
 }
 
 // Waits until *p == want; false (abort raised or seen) past the limit.
@@ -100,30 +181,37 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
     p2 = shuf[2 * k + 1];
     o1 = ld_agent(a.owner + p1);
     o2 = ld_agent(a.owner + p2);
-    // An owing entry the resolver has finished (it hands the program back
-    // right after): the program is resolved.  While the resolver is behind
-    // (no budget), wait for owing entries instead of deferring more: the
-    // resolver never waits on epoch kernels, so the wait ends, and the
-    // list cannot grow without bound.
+    // A program owed by an entry the resolver has not finished is waited
+    // for, at most owe_ticks: a long interaction deferred at the end of one
+    // epoch is mostly resolved early in the next, and every interaction
+    // deferred behind it would make two more programs owed.  While the
+    // resolver is behind (no budget), wait for owing entries for as long as
+    // it takes instead of deferring more: the resolver never waits on epoch
+    // kernels, so the wait ends, and the list cannot grow without bound.
     const bool wait = budget == 0xffffffffu;
+    const uint64_t deadline = wall_clock64() + a.owe_ticks;
     if (o1 < kResolved &&
         (ld_agent(&a.items[o1].done) == a.gen ||
-         (wait && wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks))))
+         (wait ? wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks)
+               : wait_eq_until(a.ctl, &a.items[o1].done, a.gen, deadline))))
       o1 = kResolved;
     if (o2 < kResolved &&
         (ld_agent(&a.items[o2].done) == a.gen ||
-         (wait && wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks))))
+         (wait ? wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks)
+               : wait_eq_until(a.ctl, &a.items[o2].done, a.gen, deadline))))
       o2 = kResolved;
   }
-  // A program last written by the resolver: see its bytes.
-  if (__ballot(o1 == kResolved || o2 == kResolved))
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // Programs last written by the resolver are read with coherent loads,
+  // issued after the owner word or done flag that said so.
+  after_flag();
   bool defer = false;
   if (k < a.slots) {
     defer = o1 < kResolved || o2 < kResolved;  // a program is still owed
     if (!defer) {
-      load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
-      load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+      if (o1 == kResolved) load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+      else load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
+      if (o2 == kResolved) load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
+      else load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
       mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel), k, a.prob);
       if (eval) {
         if constexpr (kFast)
@@ -169,19 +257,48 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
   }
   const bool put = defer && at < a.capacity;
   if (defer && !put) st_agent(&a.ctl->abort, 1);
+  // A slot pairing a program with itself (a 2-D pattern may, main.cc:289-296)
+  // has one program and so at most one dependency.
+  const uint32_t d1 = o1 < kResolved ? o1 : kNone;
+  const uint32_t d2 = (o2 < kResolved && p2 != p1) ? o2 : kNone;
+  DeferItem *it = a.items + (put ? at : 0u);
   if (put) {
-    DeferItem *it = a.items + at;
-    it->e = e_rel | (eval ? 0u : kNoEval);
-    it->k = (uint32_t)k;
-    it->p1 = p1;
-    it->p2 = p2;
-    it->dep1 = o1 < kResolved ? o1 : kNone;
-    it->dep2 = o2 < kResolved ? o2 : kNone;
+    st_agent(&it->e, e_rel | (eval ? 0u : kNoEval));
+    st_agent(&it->k, (uint32_t)k);
+    st_agent(&it->p1, p1);
+    st_agent(&it->p2, p2);
+    st_agent(&it->dep1, d1);
+    st_agent(&it->dep2, d2);
+    st_agent(&it->succ1, kNone);
+    st_agent(&it->succ2, kNone);
+    st_agent(&it->pending, (uint32_t)(d1 != kNone) + (uint32_t)(d2 != kNone) + 1u);
+  }
+  if (__ballot(put)) publish_point();
+  if (put) {
+    st_agent(&it->ready, a.gen);
     st_agent(a.owner + p1, at);
     st_agent(a.owner + p2, at);
+    // Link to each dependency (the successor slot of the program this
+    // entry takes over); one already done counts as satisfied.  Then drop
+    // the creation count with the satisfied ones: at zero, the entry is
+    // runnable.
+    uint32_t sat = 0;
+    if (d1 != kNone) {
+      DeferItem *d = a.items + d1;
+      uint32_t *slot = ld_agent(&d->p1) == p1 ? &d->succ1 : &d->succ2;
+      sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             kLinkDone;
+    }
+    if (d2 != kNone) {
+      DeferItem *d = a.items + d2;
+      uint32_t *slot = ld_agent(&d->p1) == p2 ? &d->succ1 : &d->succ2;
+      sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             kLinkDone;
+    }
+    if (__hip_atomic_fetch_sub(&it->pending, sat + 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == sat + 1u)
+      push_ready(a, at);
   }
-  if (__ballot(put)) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  if (put) st_agent(&a.items[at].ready, a.gen);
   const uint64_t wsum = wave_sum_u64(ops);
   const uint64_t wsteps = wave_sum_u64(steps);
   if ((lane & 63) == 0 && wsum) atomicAdd(&block_ops, (unsigned long long)wsum);
@@ -199,62 +316,77 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<1>;
   using Aux = typename L::template Aux<1>;
-  constexpr int kAuxBytes = lang_needs_aux(LANG) ? Aux::kRegionBytes : 0;
-  __shared__ __align__(16) uint8_t lds[Tape::kRegionBytes + kAuxBytes];
+  constexpr bool kFast = FastOf<LANG>::kHas;
+  // Tape, second region (aux, or nx for BffFast), and BffFast's cycle
+  // anchor, each 16-byte aligned.
+  constexpr int kSecond = (Tape::kRegionBytes + 15) / 16 * 16;
+  constexpr int kThird = kSecond + (second_region_bytes<LANG, 1>() + 15) / 16 * 16;
+  __shared__ __align__(16) uint8_t lds[kThird + (kFast ? LdsTape<1>::kRegionBytes : 0)];
+  __shared__ uint8_t lut[kFast ? 256 : 1];
+  fill_fast_lut<LANG>(lut);
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  // The resolver's one-lane wavefronts share SIMDs with the epoch kernel's
+  // busy ones; issuing first, they finish the long interactions (whose
+  // delay grows the chains of deferred interactions behind them) sooner.
+  __builtin_amdgcn_s_setprio(3);
   const auto t = Tape::at(lds, 0);
-  const auto aux = Aux::at(lds + Tape::kRegionBytes, 0);
+  const auto aux = Aux::at(lds + kSecond, 0);
   DeferCtl *c = a.ctl;
   for (;;) {
-    // Claim the next list entry, or leave once the list is closed and done.
+    // Claim the next ready-queue slot by fetch-and-add, whether or not an
+    // entry has been put there yet, then wait for it on that slot (the
+    // control words all wavefronts share are read only every 8th poll, so
+    // the waiting wavefronts do not crowd one memory channel).  Leave once
+    // the list is closed with fewer entries than this slot, or past its
+    // capacity.
+    const uint32_t qi = __hip_atomic_fetch_add(&c->head, 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+    if (qi >= a.capacity) break;
     uint32_t my = kNone;
     {
       // The epoch kernels bump counters[1] once each: while they progress,
-      // waiting here for the next entry is not a stall.
+      // waiting here is not a stall.
       uint64_t t0 = wall_clock64();
       unsigned long long beat = __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
       uint32_t nap = 1;
-      for (;;) {
-        if (ld_agent(&c->abort)) break;
-        const uint32_t h = ld_agent(&c->head);
-        const uint32_t traw = ld_agent(&c->tail);
-        if (traw >= kDeferGone) break;  // another wavefront left with the list empty
-        const uint32_t tl = min(traw, a.capacity);
-        if (h < tl) {
-          uint32_t expect = h;
-          if (__hip_atomic_compare_exchange_strong(&c->head, &expect, h + 1, __ATOMIC_RELAXED,
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-            my = h;
+      for (uint32_t poll = 0;; poll++) {
+        const uint32_t v = ld_agent(a.queue + qi);
+        if (v != kNone) {
+          my = v;
+          break;
+        }
+        if ((poll & 7) == 0) {
+          if (ld_agent(&c->abort)) break;
+          const uint32_t traw = ld_agent(&c->tail);
+          if (traw >= kDeferGone) break;  // another wavefront left with the list empty
+          if (ld_agent(&c->closed) && qi >= ld_agent(&c->final_tail)) break;
+          const unsigned long long b2 =
+              __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (b2 != beat) {
+            beat = b2;
+            t0 = wall_clock64();
+          }
+          // Nothing deferred yet and no epoch kernel finished for a while:
+          // the epoch kernels cannot start beside this kernel (dispatches
+          // are serialized, as under some profilers).  Leave, marking the
+          // list gone in the same atomic step that checks it is still
+          // empty; the epoch kernels then run every interaction themselves
+          // (their allocations return kDeferGone or more).  If an epoch
+          // kernel deferred something meanwhile, stay and resolve it.
+          if (traw == 0 && wall_clock64() - t0 > a.idle_ticks) {
+            uint32_t expect = 0;
+            if (__hip_atomic_compare_exchange_strong(&c->tail, &expect, kDeferGone,
+                                                     __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT))
+              break;
+            continue;
+          }
+          if (wall_clock64() - t0 > a.spin_ticks) {
+            st_agent(&c->abort, 1);
             break;
           }
-          continue;
-        }
-        if (ld_agent(&c->closed) && h >= ld_agent(&c->final_tail)) break;
-        const unsigned long long b2 =
-            __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (b2 != beat) {
-          beat = b2;
-          t0 = wall_clock64();
-        }
-        // Nothing deferred yet and no epoch kernel finished for a while: the
-        // epoch kernels cannot start beside this kernel (dispatches are
-        // serialized, as under some profilers).  Leave, marking the list
-        // gone in the same atomic step that checks it is still empty; the
-        // epoch kernels then run every interaction themselves (their
-        // allocations return kDeferGone or more).  If an epoch kernel
-        // deferred something meanwhile, stay and resolve it.
-        if (tl == 0 && wall_clock64() - t0 > a.idle_ticks) {
-          uint32_t expect = 0;
-          if (__hip_atomic_compare_exchange_strong(&c->tail, &expect, kDeferGone,
-                                                   __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT))
-            break;
-          continue;
-        }
-        if (wall_clock64() - t0 > a.spin_ticks) {
-          st_agent(&c->abort, 1);
-          break;
         }
         for (uint32_t q = 0; q < nap; q++) __builtin_amdgcn_s_sleep(1);
         nap = min(nap * 2, 64u);
@@ -262,30 +394,46 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     }
     if (my == kNone) break;
     DeferItem *it = a.items + my;
-    if (!wait_eq(c, &it->ready, a.gen, a.spin_ticks)) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const uint32_t e = it->e & ~kNoEval, k = it->k, p1 = it->p1, p2 = it->p2;
-    const bool eval = !(it->e & kNoEval);
-    const uint32_t d1 = it->dep1, d2 = it->dep2;
-    if (d1 != kNone && !wait_eq(c, &a.items[d1].done, a.gen, a.spin_ticks)) break;
-    if (d2 != kNone && !wait_eq(c, &a.items[d2].done, a.gen, a.spin_ticks)) break;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    load_row(t, 0, a.soup + (uint64_t)p1 * kTape);
-    load_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+    // Its fields were published before it could be queued, and every entry
+    // it depends on was done before it was.
+    after_flag();
+    const uint32_t ew = ld_agent(&it->e);
+    const uint32_t e = ew & ~kNoEval, k = ld_agent(&it->k), p1 = ld_agent(&it->p1),
+                   p2 = ld_agent(&it->p2);
+    const bool eval = !(ew & kNoEval);
+    load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+    load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
     mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
     uint32_t steps = 0, ops = 0;
-    if (eval) ops = L::evaluate(t, aux, kEvalSteps, &steps);
-    store_row(t, 0, a.soup + (uint64_t)p1 * kTape);
-    store_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+    if (eval) {
+      if constexpr (kFast)
+        ops = FastOf<LANG>::T::evaluate_full(t, LinTape<1>::at(lds + kSecond, 0),
+                                             LdsTape<1>::at(lds + kThird, 0),
+                                             LdsByteLut{(const lds_u8 *)lut}, kEvalSteps, &steps,
+                                             nullptr);
+      else
+        ops = L::evaluate(t, aux, kEvalSteps, &steps);
+    }
+    store_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+    store_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
     if (ops) atomicAdd(&a.counters[0], (unsigned long long)ops);
     if (steps) atomicAdd(&a.counters[2], (unsigned long long)steps);
     if (e + 1 == a.epochs) {
       a.pair_ops[k] = ops;
       a.pair_steps[k] = steps;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    publish_point();
     st_agent(&it->done, a.gen);
     __hip_atomic_fetch_add(&c->resolved, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // Release the entries waiting on this one: the last dependency done
+    // makes an entry runnable.
+    for (int h = 0; h < 2; h++) {
+      const uint32_t z = __hip_atomic_exchange(h ? &it->succ2 : &it->succ1, kLinkDone,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (z != kNone && __hip_atomic_fetch_sub(&a.items[z].pending, 1u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) == 1u)
+        push_ready(a, z);
+    }
     uint32_t expect = my;
     __hip_atomic_compare_exchange_strong(a.owner + p1, &expect, kResolved, __ATOMIC_RELAXED,
                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -302,12 +450,13 @@ __global__ void k_defer_start(DeferCtl *c) {
   c->abort = 0;
   c->final_tail = 0;
   c->resolved = 0;
+  c->qtail = 0;
 }
 
 // After the segment's last epoch kernel: no entry is appended any more.
 __global__ void k_defer_close(DeferCtl *c) {
   st_agent(&c->final_tail, ld_agent(&c->tail));
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  publish_point();
   st_agent(&c->closed, 1);
 }
 

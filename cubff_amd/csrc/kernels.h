@@ -56,21 +56,27 @@ struct FlowArgs {
   uint32_t fixed;           // epoch-0 items handed to the waves directly (set at launch)
 };
 
-// Deferred-interaction path (defer.hip).  Control block: list entries
-// claimed by the resolver (head), appended (tail), the closed flag and the
-// list length at closing, the abort flag.
+// Deferred-interaction path (defer.hip).  Control block: ready-queue
+// slots claimed by the resolver (head), list entries appended (tail), the
+// closed flag and the list length at closing, the abort flag, entries
+// resolved, ready-queue slots filled (qtail).
 struct alignas(16) DeferCtl {
-  uint32_t head, tail, closed, final_tail, abort, resolved, pad[2];
+  uint32_t head, tail, closed, final_tail, abort, resolved, qtail, pad;
 };
 
 // One deferred interaction: segment epoch (kNoEval set for a slot past the
 // epoch's evaluated pairs: mutated and written back only), pair slot, its
 // programs, the list entries that owe them (kNone if none), and the run
 // generation once the entry is complete (ready) and once its result is
-// written (done).
+// written (done); the entries that wait on this one for its first and its
+// second program (kNone until one links, kLinkDone once this entry is
+// done), and the count of its unfinished dependencies (plus one while its
+// creator links it).
 struct DeferItem {
   uint32_t e, k, p1, p2, dep1, dep2, ready, done;
+  uint32_t succ1, succ2, pending, pad[5];
 };
+constexpr uint32_t kLinkDone = 0xfffffffdu;
 
 constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
 constexpr uint32_t kNoEval = 0x80000000u;    // DeferItem::e flag
@@ -84,6 +90,7 @@ struct DeferArgs {
   uint8_t *soup;
   uint32_t *owner;  // n entries
   DeferItem *items;
+  uint32_t *queue;  // capacity slots: entries whose dependencies are done, kNone when empty
   DeferCtl *ctl;
   uint64_t n;
   uint32_t slots, epochs, capacity, budget, gen, prob;
@@ -93,6 +100,7 @@ struct DeferArgs {
   unsigned long long *counters;
   uint64_t spin_ticks;
   uint64_t idle_ticks;  // resolver: leave if nothing arrives and no epoch ends this long
+  uint64_t owe_ticks;   // epoch kernel: wait this long for an owing entry before deferring
 };
 
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
