@@ -28,7 +28,7 @@ EXPORTS = [
     "cubff_byte_histogram",
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
-    "cubff_eval_tapes", "cubff_eval_selfrep", "cubff_trace_bff", "cubff_sync",
+    "cubff_eval_tapes", "cubff_eval_tapes_noalloc", "cubff_eval_selfrep", "cubff_trace_bff", "cubff_sync",
     "cubff_trace_steps", "cubff_eval_selfrep_trace",
     "cubff_splitmix64", "cubff_epoch_seed",
     "cubff_batch_last_error", "cubff_batch_create", "cubff_batch_destroy",
@@ -100,6 +100,7 @@ def load():
         "cubff_read_shuffle": (_i, [_vp, _u32p, _u64p]),
         "cubff_selfrep": (_i, [_vp, _u64, _u64, _u64p]),
         "cubff_eval_tapes": (_i, [_i, _i, _u8p, _u64, _u64, _u32p, _u32p]),
+        "cubff_eval_tapes_noalloc": (_i, [_i, _i, _u8p, _u64, _u64, _u32p, _u32p]),
         "cubff_trace_bff": (_i, [_i, _i, _u8p, _u64, _u8p, _u64, _u64p, _u64p]),
         "cubff_eval_selfrep": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p]),
         "cubff_eval_selfrep_trace": (_i, [_i, _i, _u8p, _u64, _u64, _u64, _u64p, _u8p]),
@@ -182,6 +183,21 @@ def eval_tapes(lang: str, tapes: np.ndarray, steps: int = 8192, device: int = 0)
     st = np.zeros(n, dtype=np.uint32)
     _check(load().cubff_eval_tapes(lang_id(lang), device, _ptr(t, _u8p), n, steps,
                                    _ptr(ops, _u32p), _ptr(st, _u32p)))
+    return t, ops, st
+
+
+NOALLOC_MAX_TAPES = 16384
+
+
+def eval_tapes_noalloc(lang: str, tapes: np.ndarray, steps: int = 8192, device: int = 0):
+    """eval_tapes through cubff_eval_tapes_noalloc (no device allocation; at
+    most NOALLOC_MAX_TAPES tapes).  Returns (final_tapes, ops, steps)."""
+    t = np.ascontiguousarray(tapes, dtype=np.uint8).reshape(-1, 128).copy()
+    n = t.shape[0]
+    ops = np.zeros(n, dtype=np.uint32)
+    st = np.zeros(n, dtype=np.uint32)
+    _check(load().cubff_eval_tapes_noalloc(lang_id(lang), device, _ptr(t, _u8p), n, steps,
+                                           _ptr(ops, _u32p), _ptr(st, _u32p)))
     return t, ops, st
 
 

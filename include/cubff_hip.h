@@ -176,6 +176,19 @@ int cubff_selfrep(cubff_ctx *ctx, uint64_t seed_value, uint64_t count,
 int cubff_eval_tapes(int lang, int device, uint8_t *tapes, uint64_t count,
                      uint64_t steps, uint32_t *ops, uint32_t *steps_out);
 
+/* cubff_eval_tapes without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The tapes and per-tape results are staged in a fixed device
+ * array that each GPU holds until the process ends
+ * (CUBFF_NOALLOC_MAX_TAPES * 136 bytes, about 2.1 MiB); count above
+ * CUBFF_NOALLOC_MAX_TAPES is rejected with CUBFF_EINVAL.  Every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU race.
+ * Same arguments and results as cubff_eval_tapes. */
+#define CUBFF_NOALLOC_MAX_TAPES 16384
+int cubff_eval_tapes_noalloc(int lang, int device, uint8_t *tapes, uint64_t count,
+                             uint64_t steps, uint32_t *ops, uint32_t *steps_out);
+
 /* Step trace of one BFF-family tape for the reference's visualiser: the
  * file image python/bff_interpreter.py:evaluate_and_save (:327-393) writes
  * -- "BFF\0", u32 version 1, u32 tape size 128, then before every step u32
