@@ -28,6 +28,7 @@ struct cubff_ctx {
   uint32_t *pair_ops = nullptr;
   uint32_t *pair_steps = nullptr;
   uint64_t *selfrep = nullptr;
+  SrScratch sr{};  // levelled self-rep check (allocated on first use)
   FyScratch fy{};
   G2Scratch g2{};
   uint64_t *row_off = nullptr;
@@ -740,6 +741,10 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   dev_free(c->pair_ops);
   dev_free(c->pair_steps);
   dev_free(c->selfrep);
+  dev_free(c->sr.finals);
+  dev_free(c->sr.list[0]);
+  dev_free(c->sr.list[1]);
+  dev_free(c->sr.counts);
   dev_free(c->fy.j);
   dev_free(c->fy.cnt);
   dev_free(c->fy.base);
@@ -1007,7 +1012,20 @@ int cubff_selfrep(cubff_ctx *c, uint64_t seed_value, uint64_t count,
   if (count > c->n) return fail(CUBFF_EINVAL, "count exceeds num_programs");
   if (count && !result) return fail(CUBFF_EINVAL, "null result");
   if (!c->selfrep && (r = dev_alloc(&c->selfrep, c->n))) return r;
-  HIP_TRY(launch_selfrep(c->stream, c->lang, c->soup, c->n, count, seed_value, c->selfrep));
+  if (selfrep_levels_supported(c->lang)) {
+    if (!c->sr.finals) {
+      const uint64_t chunk = std::min<uint64_t>(c->n, kSelfrepChunk);
+      if ((r = dev_alloc(&c->sr.finals, chunk * 13 * 128)) ||
+          (r = dev_alloc(&c->sr.list[0], chunk * 13)) ||
+          (r = dev_alloc(&c->sr.list[1], chunk * 13)) || (r = dev_alloc(&c->sr.counts, 2)))
+        return r;
+      c->sr.chunk = chunk;
+    }
+    HIP_TRY(launch_selfrep_levels(c->stream, c->lang, c->soup, c->n, count, seed_value,
+                                  c->selfrep, c->sr));
+  } else {
+    HIP_TRY(launch_selfrep(c->stream, c->lang, c->soup, c->n, count, seed_value, c->selfrep));
+  }
   if (count)
     HIP_TRY(hipMemcpyAsync(result, c->selfrep, count * 8, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));

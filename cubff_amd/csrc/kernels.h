@@ -112,6 +112,27 @@ hipError_t launch_epoch(hipStream_t st, int lang, uint8_t *soup,
                         unsigned long long *counters);
 // trace (nullable): program 0's tapes before and after each generation of
 // each trial, 13 * 5 * 2 * 128 bytes.
+// Scratch of the levelled self-rep check (selfrep.hip) for `chunk` programs
+// at a time: a 128-byte final tape per chain (13 per program), two chain
+// lists of chunk * 13 entries, and their two counters.
+struct SrScratch {
+  uint8_t *finals;
+  uint32_t *list[2];
+  uint32_t *counts;
+  uint64_t chunk;
+};
+// Programs per chunk: 2^22 (7 GiB of final tapes).  The last levels are
+// bound by the latency of the few longest chains, paid once per chunk.
+constexpr uint64_t kSelfrepChunk = 1ull << 22;
+
+// True for the languages the levelled check runs (BffFast's).
+bool selfrep_levels_supported(int lang);
+// CheckSelfRep of programs [0, count) of the soup into result[0, count),
+// chunk by chunk; synchronises `st` between levels.
+hipError_t launch_selfrep_levels(hipStream_t st, int lang, const uint8_t *soup, uint64_t n,
+                                 uint64_t count, uint64_t seed_value, uint64_t *result,
+                                 const SrScratch &w);
+
 hipError_t launch_selfrep(hipStream_t st, int lang, const uint8_t *soup,
                           uint64_t n, uint64_t count, uint64_t seed_value,
                           uint64_t *result, uint8_t *trace = nullptr);

@@ -187,6 +187,33 @@ def test_selfrep_vs_oracle(capi, lang):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("lang", ["bff_noheads", "bff_perm"])
+def test_selfrep_two_chunks_vs_oracle(capi, lang):
+    # More programs than one chunk of the levelled check (2^18), with copy
+    # loops planted around the chunk boundary and elsewhere so that chains
+    # reach every level (long runs, cycles, programs that copy themselves).
+    n, count = 1 << 19, (1 << 18) + 3000
+    soup = O.init_programs(n, O.epoch_seed(11, 0))
+    plants = [b"[[{.>]-]                ]", b"[<,]", b"[{.>>}]", b"[[[-]+]", b"{[.>]}" * 6,
+              b"[>,<]" * 4]
+    rng = np.random.default_rng(5)
+    spots = [(1 << 18) - 2, (1 << 18) - 1, 1 << 18, (1 << 18) + 1, 0, 77] + \
+        [int(x) for x in rng.integers(0, count, 400)]
+    for k, q in enumerate(spots):
+        prog = plants[k % len(plants)]
+        off = q * 64 + int(rng.integers(0, 64 - len(prog) + 1))
+        soup[off:off + len(prog)] = list(prog)
+    sv = O.epoch_seed(11, 9)
+    print("selfrep two chunks", lang, flush=True)
+    with capi.Soup(lang, n) as g:
+        g.write_soup(soup)
+        got = g.selfrep(sv, count)
+        again = g.selfrep(sv, 4096)
+    want = O.check_selfrep(lang, soup, sv, count)
+    assert np.array_equal(got, want)
+    assert np.array_equal(again, want[:4096])
+
+
 def test_eval_selfrep_single(capi):
     prog = bytes(b"[[{.>]-]                ]") + bytes(20)
     for lang in ["bff", "bff_noheads"]:
