@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -56,41 +57,91 @@ def c_step(lang: str) -> int:
     return C_STEP[lang]
 
 
-def cpu_baseline(lang: str, n: int, seed: int, prob: int, budget_s: float):
-    """Times the reference CPU path (or the oracle port) on the host."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(lang: str, n: int, seed: int, prob: int, first: int, count: int,
+                 budget_s: float):
+    """Times the reference CPU path (or the oracle port) on the host over the
+    same epochs as the GPU value, first..first+count-1.
+
+    The reference's RunSimulation reports, at every callback, its rate over
+    the epochs since the previous callback with the callback's own work
+    (soup copy, brotli, byte tables) excluded (common_language.h:496-512,
+    :583-585).  A callback interval that divides both first-1 and count
+    makes those windows tile the GPU's epochs exactly; the run stops after
+    the last of them, and is repeated from epoch 0 while the wall-clock
+    budget lasts (or cut short inside the window if one run overruns it)."""
     threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    last = first + count - 1
     from oracle import refimpl
     if refimpl.available():
-        interval = 32
-        marks = []
+        interval = math.gcd(first - 1, count) if first >= 1 else max(count - 1, 1)
+        ops = 0
+        secs = 0.0
+        reps = 0
+        covered = last
+        t_start = time.perf_counter()
+        while True:
+            prev = {"ops": 0, "epoch": 0}
+            got = {"ops": 0, "secs": 0.0, "end": first - 1}
+            t_run = time.perf_counter()
 
-        def cb(st):
-            marks.append((time.perf_counter(), int(st.total_ops), int(st.epoch)))
-            return (marks[-1][0] - marks[0][0]) >= budget_s
+            def cb(st):
+                # This callback closes the window of epochs prev.epoch..st.epoch-1.
+                lo, hi = prev["epoch"], int(st.epoch) - 1
+                d_ops = int(st.total_ops) - prev["ops"]
+                prev["ops"], prev["epoch"] = int(st.total_ops), int(st.epoch)
+                if lo >= first and hi <= last:
+                    got["ops"] += d_ops
+                    got["secs"] += d_ops / (float(st.mops_s) * 1e6)
+                    got["end"] = hi
+                    if hi >= last:
+                        return True
+                    # One run longer than the whole budget is cut short.
+                    return reps == 0 and time.perf_counter() - t_run > budget_s
+                return False
 
-        refimpl.run_simulation(lang, n, seed, cb, mutation_prob=prob,
-                               callback_interval=interval)
-        (t0, ops0, e0), (t1, ops1, e1) = marks[0], marks[-1]
-        return {"value": (ops1 - ops0) / (t1 - t0), "unit": "instr/s",
-                "cores": threads, "kind": "reference",
-                "epochs_per_s": (e1 - e0) / (t1 - t0),
-                "sample": (f"reference RunSimulation (oracle/_ref, g++ -O3 -fopenmp) on "
-                           f"the same workload, epochs {e0}..{e1 - 1} from seed {seed}, "
-                           f"callbacks every {interval} epochs inside the timed span")}
+            refimpl.run_simulation(lang, n, seed, cb, mutation_prob=prob,
+                                   callback_interval=interval)
+            ops += got["ops"]
+            secs += got["secs"]
+            reps += 1
+            covered = min(covered, got["end"])
+            if got["end"] < last or time.perf_counter() - t_start + (
+                    time.perf_counter() - t_run) > budget_s:
+                break
+        return {"value": ops / secs, "unit": "instr/s", "cores": threads, "kind": "reference",
+                "epochs_per_s": reps * (covered - first + 1) / secs,
+                "cpu": cpu_model(),
+                "sample": (f"reference RunSimulation (oracle/_ref, g++ -O3 -fopenmp, "
+                           f"{threads} OpenMP threads) on the same workload and the same "
+                           f"epochs {first}..{covered} from seed {seed}, run {reps} time(s); "
+                           f"time from the reference's own per-callback rate (callbacks "
+                           f"every {interval} epochs, their work excluded)")}
     from oracle import oracle as O
     S = O.Soup(lang, n, seed, prob, nthreads=threads)
-    S.step()
+    for _ in range(first):
+        S.step()
     t0 = time.perf_counter()
     ops = 0
     e = 0
-    while time.perf_counter() - t0 < budget_s:
+    while e < count and time.perf_counter() - t0 < budget_s:
         ops += S.step()[0]
         e += 1
     dt = time.perf_counter() - t0
     return {"value": ops / dt, "unit": "instr/s", "cores": threads, "kind": "port",
-            "epochs_per_s": e / dt,
-            "sample": f"oracle OpenMP port, epochs 1..{e} of the same workload"}
+            "epochs_per_s": e / dt, "cpu": cpu_model(),
+            "sample": f"oracle OpenMP port, epochs {first}..{first + e - 1} of the same workload"}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_defer_bff_noheads_2p17.md")
@@ -145,6 +196,7 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
             steps += st
         wg, tpl = b.get_launch()
         out = {"instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / epochs,
+               "first_epoch": e0,
                "interp_ms_per_epoch": it / epochs, "epochs_timed": epochs,
                "total_programs": sum(ns), "soups": len(ns), "wg": wg,
                "tapes_per_lane": tpl, "steps": steps, "ops": ops, "autotune": tuned}
@@ -153,6 +205,40 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
             out["sched"] = {"trainings": tr, "last_loss": loss,
                             "bucket_slots": [int(x) for x in buckets]}
         return out
+
+
+def run_single_same_soups(capi, spec, first, epochs):
+    """The soups of a batch run one at a time through the plain single-soup
+    path (the library's default multi-epoch path per soup), same languages,
+    sizes, seeds, mutation probabilities and epochs first..first+epochs-1 as
+    the batched leg's timed span.  Times add up over the soups."""
+    langs, ns, seeds, probs = spec
+    tot = 0.0
+    ops = 0
+    for lang, n, seed, prob in zip(langs, ns, seeds, probs):
+        with capi.Soup(lang, n) as g:
+            g.init_programs(capi.epoch_seed(seed, 0))
+            if first:
+                g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, first, prob)
+            g.sync()
+            g.reset_counters()
+            t, _ = g.run_epochs_timed(capi.PAIR_SHUFFLE, seed, first, epochs, prob)
+            o, _, _ = g.read_counters(with_steps=True)
+        tot += t
+        ops += o
+    return {"instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / epochs,
+            "first_epoch": first, "epochs_timed": epochs, "ops": ops}
+
+
+def same_soups_leg(capi, spec, learned, plain):
+    """learned batched vs the plain single-soup kernel on the same soups."""
+    single = run_single_same_soups(capi, spec, learned["first_epoch"], learned["epochs_timed"])
+    return {"plain_single_same_soups": single,
+            "same_work": single["ops"] == learned["ops"],
+            "learned_batched_vs_plain_single_same_soups":
+                learned["instr_per_s"] / single["instr_per_s"],
+            "plain_batched_vs_plain_single_same_soups":
+                plain["instr_per_s"] / single["instr_per_s"]}
 
 
 def run_soup_timed(capi, lang, n, seed, prob, warmup, epochs, mode=None, pattern=None,
@@ -184,11 +270,85 @@ def run_soup_timed(capi, lang, n, seed, prob, warmup, epochs, mode=None, pattern
             "epochs_timed": epochs}
 
 
-def run_other_configs(capi, epochs):
+BROTLI_LIB = "/usr/lib/x86_64-linux-gnu/libbrotlienc.so.1"
+
+
+def brotli_size(data) -> int:
+    """BrotliEncoderCompress(2, 24, BROTLI_MODE_GENERIC) of the soup, the
+    reference's entropy measurement (common_language.h:502-507), through the
+    system libbrotlienc (ctypes releases the GIL for the call)."""
+    import ctypes
+    import numpy as np
+    lib = ctypes.CDLL(BROTLI_LIB)
+    lib.BrotliEncoderMaxCompressedSize.restype = ctypes.c_size_t
+    lib.BrotliEncoderMaxCompressedSize.argtypes = [ctypes.c_size_t]
+    lib.BrotliEncoderCompress.restype = ctypes.c_int
+    lib.BrotliEncoderCompress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t),
+                                          ctypes.c_void_p]
+    src = np.ascontiguousarray(data, dtype=np.uint8)
+    cap = lib.BrotliEncoderMaxCompressedSize(src.size)
+    dst = np.empty(cap, dtype=np.uint8)
+    size = ctypes.c_size_t(cap)
+    if not lib.BrotliEncoderCompress(2, 24, 0, src.size, src.ctypes.data, ctypes.byref(size),
+                                     dst.ctypes.data):
+        raise RuntimeError("BrotliEncoderCompress failed")
+    return int(size.value)
+
+
+def run_large_soup(capi, epochs):
+    """The large-soup run: bff_noheads, 2^22 programs, seed 7, P = 257698
+    (p = 0.00024); every epoch is followed by the device self-rep check of
+    all programs, and every 64 epochs the soup is copied back and brotli-
+    compressed on a host thread while the next epochs run."""
+    from concurrent.futures import ThreadPoolExecutor
+    n, seed, P4 = 1 << 22, 7, 257698
+    first = 2
+    with capi.Soup("bff_noheads", n) as g, ThreadPoolExecutor(1) as pool:
+        g.init_programs(capi.epoch_seed(seed, 0))
+        g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, first, P4)
+        g.selfrep(capi.epoch_seed(seed, first - 1), n)  # first-use allocation, untimed
+        g.sync()
+        g.reset_counters()
+        ep_s = sr_s = 0.0
+        best = []
+        pending = []
+        t0 = time.perf_counter()
+        for e in range(first, first + epochs):
+            t1 = time.perf_counter()
+            g.run_epochs(capi.PAIR_SHUFFLE, seed, e, 1, P4)
+            g.sync()
+            t2 = time.perf_counter()
+            scores = g.selfrep(capi.epoch_seed(seed, e), n)
+            t3 = time.perf_counter()
+            ep_s += t2 - t1
+            sr_s += t3 - t2
+            best.append(int(scores.max()))
+            if e % 64 == 0:
+                pending.append((e, pool.submit(brotli_size, g.read_soup())))
+        brotli = [(e, f.result()) for e, f in pending]
+        total = time.perf_counter() - t0
+        ops, _, _ = g.read_counters(with_steps=True)
+    return {
+        "workload": (f"bff_noheads, 2^22 programs, seed 7, P=257698 (p=0.00024), epochs "
+                     f"{first}..{first + epochs - 1}, self-rep check of all programs after every "
+                     f"epoch, brotli of the soup every 64 epochs on a host thread"),
+        "epochs_timed": epochs, "instr_per_s": ops / ep_s, "ms_per_epoch": ep_s * 1e3 / epochs,
+        "selfrep_ms_per_epoch": sr_s * 1e3 / epochs,
+        "epochs_per_s_with_selfrep": epochs / total, "total_s": total,
+        "brotli_bpb": {str(e): b * 8 / (n * 64) for e, b in brotli},
+        "max_selfrep_score": max(best) if best else None}
+
+
+def run_other_configs(capi, epochs, large_epochs):
     """The other benchmark configurations, each on one GPU."""
     from cubff_amd.patterns import grid_pattern
     out = {}
     P = 1 << 18
+    if large_epochs > 0:
+        out["large_soup"] = run_large_soup(capi, large_epochs)
+    if epochs <= 0:
+        return out
     langs = {}
     for lang in ("bff", "bff8", "forth", "forthtrivial", "subleq", "rsubleq4"):
         # One launch per epoch, the multi-epoch dataflow kernel, and the
@@ -209,25 +369,6 @@ def run_other_configs(capi, epochs):
     out["all_interpreters"] = {
         "workload": "2^17 programs, seed 1, Fisher-Yates pairing; plus bff_noheads on a "
                     "512x256 grid with radius-2 neighbour pairing", "langs": langs}
-    # Large soup with the self-rep check on device every epoch.
-    n, seed, P4 = 1 << 22, 7, 257698
-    with capi.Soup("bff_noheads", n) as g:
-        g.init_programs(capi.epoch_seed(seed, 0))
-        g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, 2, P4)
-        g.sync()
-        g.reset_counters()
-        k = max(2, epochs // 4)
-        t0 = time.perf_counter()
-        tot, kern = g.run_epochs_timed(capi.PAIR_SHUFFLE, seed, 2, k, P4)
-        ops, _, _ = g.read_counters(with_steps=True)
-        t1 = time.perf_counter()
-        g.selfrep(capi.epoch_seed(seed, 2 + k - 1), n)
-        t_sr = time.perf_counter() - t1
-    out["large_soup"] = {
-        "workload": "bff_noheads, 2^22 programs, seed 7, P=257698 (p=0.00024)",
-        "instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / k, "epochs_timed": k,
-        "selfrep_ms_per_epoch": t_sr * 1e3,
-        "epochs_per_s_with_selfrep": 1e3 / (tot / k + t_sr * 1e3)}
     # Learned scheduler on mixed languages.
     langs5 = ["bff" if s % 2 == 0 else "forth" for s in range(100, 132)]
     ns5 = [65536] * 32
@@ -239,6 +380,8 @@ def run_other_configs(capi, epochs):
                                 spec=(langs5, ns5, seeds5, probs5))
     res["learned_vs_plain_batched"] = (res["learned_sched"]["instr_per_s"] /
                                        res["plain"]["instr_per_s"])
+    res.update(same_soups_leg(capi, (langs5, ns5, seeds5, probs5), res["learned_sched"],
+                              res["plain"]))
     res["workload"] = ("32 soups x 2^16 programs, seeds 100..131, bff on even seeds and "
                        "forth on odd seeds, Fisher-Yates")
     out["mixed_sched"] = res
@@ -259,6 +402,9 @@ def main():
                     help="timed epochs of the 64-soup batched runs (0 = skip)")
     ap.add_argument("--config-epochs", type=int, default=16,
                     help="timed epochs of the other configurations (0 = skip)")
+    ap.add_argument("--large-epochs", type=int, default=256,
+                    help="timed epochs of the 2^22-program run with the self-rep check every "
+                         "epoch (0 = skip)")
     ap.add_argument("--pipeline", choices=["auto", "loop", "dataflow", "defer"], default="auto",
                     help="multi-epoch path of the single-soup run (auto: the library's choice "
                          "for the language)")
@@ -313,9 +459,14 @@ def main():
     K = args.steps
     dt = total_ms / 1e3
     work = (n // 2) * 128 * C_MUT * K + steps * c_step(lang)
-    lane_ops = work / (kern_ms / 1e3)
+    # On the deferring path the work is shared by the epoch kernel and the
+    # resolver running beside it on a second stream, so the rate is taken
+    # over the whole epoch span (both kernels and the end-of-run drain), not
+    # over the epoch kernel's launches alone.
+    basis_ms = total_ms if path == "defer" else kern_ms
+    lane_ops = work / (basis_ms / 1e3)
     algo_bytes = (n // 2) * BYTES_PER_SLOT[path]
-    achieved = algo_bytes / (kern_ms / 1e3 / K)
+    achieved = algo_bytes / (basis_ms / 1e3 / K)
     line = {
         "metric": METRIC,
         "value": ops / dt,
@@ -356,6 +507,8 @@ def main():
                                 "+ scatter)",
                        "dataflow": "k_flow"}[path],
             "kernel_ms_avg": kern_ms / K,
+            "time_basis": ("epoch span (epoch kernel + concurrent resolver + drain), "
+                           "ms_per_step" if path == "defer" else "kernel_ms_avg"),
             "work_per_launch": work / K,
             "model": "W = (N/2)*128*32 + sum(steps)*C_step",
             "limiter": "latency of the longest interaction of the epoch",
@@ -374,14 +527,14 @@ def main():
             "workload": ("64 bff_noheads soups, seeds 0..63, sizes cycling 2^12/2^14/2^16/"
                          "2^17 programs, mutation log-spaced 1e-5..1e-3, Fisher-Yates"),
             "plain": plain, "learned_sched": learned,
-            "learned_vs_plain_single_soup": learned["instr_per_s"] / line["value"],
-            "plain_batched_vs_plain_single_soup": plain["instr_per_s"] / line["value"],
             "learned_vs_plain_batched": learned["instr_per_s"] / plain["instr_per_s"],
         }
-    if args.config_epochs > 0:
-        line["configs"] = run_other_configs(capi, args.config_epochs)
+        line["batched"].update(same_soups_leg(capi, batched_config(), learned, plain))
+    if args.config_epochs > 0 or args.large_epochs > 0:
+        line["configs"] = run_other_configs(capi, args.config_epochs, args.large_epochs)
     if args.cpu_seconds > 0:
-        line["cpu_baseline"] = cpu_baseline(lang, n, seed, prob, args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(lang, n, seed, prob, args.warmup, args.steps,
+                                            args.cpu_seconds)
     print(json.dumps(line))
 
 

@@ -24,6 +24,9 @@ struct RefState {
   float brotli_bpb;
   float h0;
   float higher_entropy;
+  // The reference's own rate over the epochs since the previous callback,
+  // callback time excluded (common_language.h:508-512, :539).
+  float mops_s;
 };
 
 typedef int (*RefCallback)(const RefState *state, void *user);
@@ -76,6 +79,7 @@ int ref_run_simulation(const char *lang, unsigned long long num_programs,
     rs.brotli_bpb = st.brotli_bpb;
     rs.h0 = st.h0;
     rs.higher_entropy = st.higher_entropy;
+    rs.mops_s = st.mops_s;
     return cb(&rs, user) != 0;
   });
   return 0;

@@ -34,6 +34,7 @@ class RefState(ctypes.Structure):
         ("brotli_bpb", ctypes.c_float),
         ("h0", ctypes.c_float),
         ("higher_entropy", ctypes.c_float),
+        ("mops_s", ctypes.c_float),
     ]
 
 
