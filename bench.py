@@ -186,7 +186,7 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
         if autotune:
             wg, tpl, table = b.autotune(e0, 3, sched)
             tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
-            e0 += 36
+            e0 = b.next_epoch()
         b.reset_counters()
         tot, it = b.run_epochs_timed(e0, epochs, sched)
         ops = steps = 0

@@ -38,6 +38,8 @@ EXPORTS = [
     "cubff_batch_write_soup", "cubff_batch_read_counters",
     "cubff_batch_reset_counters", "cubff_batch_read_pair_stats",
     "cubff_batch_read_shuffle", "cubff_batch_sched_stats",
+    "cubff_batch_set_pipeline", "cubff_batch_set_defer_budget", "cubff_batch_defer_stats",
+    "cubff_batch_next_epoch",
     "cubff_mlp_last_error", "cubff_mlp_num_params", "cubff_mlp_create",
     "cubff_mlp_destroy", "cubff_mlp_set_params", "cubff_mlp_get_params",
     "cubff_mlp_forward", "cubff_mlp_predict", "cubff_mlp_predict_pieces", "cubff_mlp_gradients",
@@ -129,6 +131,10 @@ def load():
         "cubff_batch_read_pair_stats": (_i, [_vp, ctypes.c_uint32, _u32p, _u32p, _u32p]),
         "cubff_batch_read_shuffle": (_i, [_vp, ctypes.c_uint32, _u32p]),
         "cubff_batch_sched_stats": (_i, [_vp, _u64p, ctypes.POINTER(ctypes.c_float), _u32p]),
+        "cubff_batch_set_pipeline": (_i, [_vp, _i]),
+        "cubff_batch_set_defer_budget": (_i, [_vp, ctypes.c_uint32]),
+        "cubff_batch_defer_stats": (_i, [_vp, _u64p]),
+        "cubff_batch_next_epoch": (_i, [_vp, _u64p]),
         "cubff_mlp_last_error": (ctypes.c_char_p, []),
         "cubff_mlp_num_params": (_u64, []),
         "cubff_mlp_create": (_i, [ctypes.POINTER(_vp), _u64, _u64, _i]),
@@ -492,6 +498,26 @@ class Batch:
         s = np.zeros(self.n[i], dtype=np.uint32)
         _bcheck(load().cubff_batch_read_shuffle(self._h, i, _ptr(s, _u32p)))
         return s
+
+    def set_pipeline(self, mode: int):
+        """FLOW_AUTO / FLOW_DEFER (deferring segments for runs of two or more
+        epochs without graph replay) or FLOW_OFF (interpreter rounds only)."""
+        _bcheck(load().cubff_batch_set_pipeline(self._h, int(mode)))
+
+    def set_defer_budget(self, budget: int):
+        _bcheck(load().cubff_batch_set_defer_budget(self._h, budget))
+
+    def next_epoch(self) -> int:
+        """The epoch after the last one run."""
+        d = ctypes.c_uint64(0)
+        _bcheck(load().cubff_batch_next_epoch(self._h, ctypes.byref(d)))
+        return int(d.value)
+
+    def defer_stats(self) -> int:
+        """Interactions handed to the resolver since creation."""
+        d = ctypes.c_uint64(0)
+        _bcheck(load().cubff_batch_defer_stats(self._h, ctypes.byref(d)))
+        return int(d.value)
 
     def sched_stats(self):
         """(trainings, last training loss, slots per length bucket)."""

@@ -13,22 +13,13 @@
 //   k_b_end_epoch per-soup counter sums, num_runs, epoch counter
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "batch.h"
 #include "device_common.h"
 #include "languages.h"
 
 namespace cubff {
-
-// Soup of a program / slot index: prefix-sum arrays of length num_soups + 1.
-__device__ __forceinline__ uint32_t soup_of(const uint64_t *base,
-                                            uint32_t num_soups, uint64_t x) {
-  uint32_t lo = 0, hi = num_soups;  // base[lo] <= x < base[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) / 2;
-    if (base[mid] <= x) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 // A workgroup's copy of a prefix-sum array in LDS when it has at most
 // kBaseTable entries (else the array itself): the per-soup lookups of every

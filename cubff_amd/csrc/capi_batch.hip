@@ -1,5 +1,6 @@
 // C-ABI of the batched-soup path (declared in include/cubff_hip.h).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <string.h>
 
 #include <string>
@@ -41,6 +42,7 @@ struct cubff_batch {
   // Per language present: the identity slot list and its scheduled order.
   std::vector<SchedList> lists;  // offsets into plain_list / order_list
   uint32_t *plain_list = nullptr;   // total_slots
+  uint32_t *claims = nullptr;  // per list: the deferring kernel's next entry
   uint32_t *order_list = nullptr;   // total_slots (scheduled order)
   // Learned order of the next epoch, built on stream2 beside the
   // interpreter from the next pairs' programs as this epoch staged them
@@ -55,6 +57,20 @@ struct cubff_batch {
   hipGraphExec_t graph = nullptr;
   int graph_sched = -1, graph_wg = 0;
   uint32_t graph_per_lane = 0;
+  // Deferring epochs (defer.hip) for the batch's first language that
+  // batch_defers() names (lists[defer_li]): runs of two or more epochs
+  // without graph replay go in segments whose long interactions a resolver
+  // on stream_r finishes beside the later epochs.  CUBFF_FLOW_OFF keeps
+  // every epoch to the interpreter rounds.
+  int pipeline = CUBFF_FLOW_AUTO;
+  int defer_li = -1;
+  hipStream_t stream_r = nullptr;
+  uint32_t *owner = nullptr;  // total_programs
+  DeferItem *ditems = nullptr;
+  uint32_t *dqueue = nullptr;
+  DeferCtl *dctl = nullptr;
+  uint64_t defer_cap = 0, dqueue_dirty = 0, defer_items = 0;
+  uint32_t defer_gen = 0, defer_budget = 768;
 };
 
 namespace {
@@ -103,7 +119,10 @@ void drop_graph(cubff_batch *b) {
 // One epoch on the batch stream: stage, interpret (in order_list's order
 // under the learned scheduler), end-of-epoch bookkeeping.  interp_ev
 // (nullable) brackets the interpreter launches with events.
-int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
+// With da (a deferring segment), the deferring language's slots go to the
+// deferring epoch kernel as epoch e_rel of the segment.
+int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1,
+                  const DeferArgs *da = nullptr, uint32_t e_rel = 0) {
   const hipStream_t st = b->stream;
   if (sched != CUBFF_SCHED_PLAIN && sched != CUBFF_SCHED_LEARNED)
     return fail(CUBFF_EINVAL, "unknown scheduling mode");
@@ -119,6 +138,7 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   // workgroups are done, rather than its start, when they fill every CU
   // (measured: 1.8-2.0 ms behind the pairing vs 3.0 ms from the fork, 0.8 ms
   // alone).
+  HIP_TRY(batch_fill_u32(st, b->claims, 0, b->lists.size()));
   HIP_TRY(hipEventRecord(b->fork, st));
   HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
   HIP_TRY(batch_pair_shuffle(b->stream2, b->d, 2));
@@ -132,12 +152,21 @@ int enqueue_epoch(cubff_batch *b, int sched, hipEvent_t ev0, hipEvent_t ev1) {
   bool side = false;
   for (size_t li = 0; li < b->lists.size(); li++) {
     const auto &l = b->lists[li];
-    const hipStream_t s = (b->stream_i && (li & 1)) ? b->stream_i : st;
+    const bool deferring = da && (int)li == b->defer_li;
+    const hipStream_t s = (!deferring && b->stream_i && (li & 1)) ? b->stream_i : st;
     if (s != st && !side) {
       HIP_TRY(hipStreamWaitEvent(s, b->fork, 0));
       side = true;
     }
-    HIP_TRY(batch_interp(s, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
+    if (deferring) {
+      DeferArgs ea = *da;
+      ea.list = list + l.begin;
+      ea.slots = (uint32_t)l.count;
+      ea.claim = b->claims + li;
+      HIP_TRY(launch_b_epoch_defer(st, l.lang, ea, e_rel, b->wg, b->per_lane));
+    } else {
+      HIP_TRY(batch_interp(s, l.lang, b->d, list + l.begin, l.count, b->wg, b->per_lane));
+    }
   }
   if (side) {
     HIP_TRY(hipEventRecord(b->join_i, b->stream_i));
@@ -230,6 +259,192 @@ int ensure_learned(cubff_batch *b) {
   return CUBFF_OK;
 }
 
+constexpr unsigned kBatchResolveWaves = 512;  // resolver wavefronts
+constexpr double kBatchOweWaitS = 200e-6;       // as for one soup
+constexpr double kBatchSpinS = 10.0;            // longest wait before the run is abandoned
+constexpr double kBatchIdleS = 0.1;             // resolver leaves with nothing deferred
+// List entries of one deferring segment: every slot of the deferring
+// language in every epoch in the worst case (64-byte entry, 4-byte queue
+// slot), within 2 GiB; at most 64 epochs.
+constexpr uint64_t kBatchDeferBytes = 1ull << 31;
+
+// Deferring segments: for the plain order by default, for both orders with
+// CUBFF_FLOW_DEFER.  The learned order pays in the interpreter rounds,
+// whose wavefronts it fills with pairs of like length; the deferring
+// kernel's lanes take new pairs as they finish, which leaves the order
+// little to gain while its inference still costs.
+bool defer_active(const cubff_batch *b, int use_graph, int sched) {
+  if (use_graph || b->defer_li < 0 || b->pipeline == CUBFF_FLOW_OFF) return false;
+  return b->pipeline == CUBFF_FLOW_DEFER || sched != CUBFF_SCHED_LEARNED;
+}
+
+uint64_t defer_segment_epochs_b(const cubff_batch *b) {
+  const uint64_t slots = std::max<uint64_t>(1, b->lists[b->defer_li].count);
+  return std::max<uint64_t>(2, std::min<uint64_t>(64, kBatchDeferBytes / (68 * slots)));
+}
+
+// The resolver's stream: a CU-masked stream (every CU enabled), which the
+// runtime gives a hardware queue of its own rather than one shared with
+// another stream; a persistent resolver sharing the epoch stream's queue
+// would hold back the epoch kernels it waits for (the resolver then leaves
+// after its idle limit and the epochs run without deferral).
+hipError_t create_resolver_stream(int device, hipStream_t *out) {
+  int cus = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess || cus <= 0) return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+  std::vector<uint32_t> mask((cus + 31) / 32, 0xffffffffu);
+  if (cus % 32) mask.back() = (1u << (cus % 32)) - 1;
+  e = hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data());
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    e = hipStreamCreateWithFlags(out, hipStreamNonBlocking);
+  }
+  return e;
+}
+
+int ensure_defer_b(cubff_batch *b, uint64_t epochs) {
+  const uint64_t need = epochs * b->lists[b->defer_li].count;
+  int r;
+  if (!b->stream_r) HIP_TRY(create_resolver_stream(b->device, &b->stream_r));
+  if (!b->owner && (r = alloc(&b->owner, b->d.total_programs))) return r;
+  if (!b->dctl && (r = alloc(&b->dctl, 1))) return r;
+  if (b->defer_cap < need) {
+    release(b->ditems);
+    release(b->dqueue);
+    b->defer_cap = 0;
+    if ((r = alloc(&b->ditems, need)) || (r = alloc(&b->dqueue, need))) return r;
+    // Generation words start at 0 (runs use generations >= 1); queue slots
+    // start empty.
+    HIP_TRY(hipMemsetAsync(b->ditems, 0, need * sizeof(DeferItem), b->stream));
+    HIP_TRY(batch_fill_u32(b->stream, b->dqueue, kNone, need));
+    b->dqueue_dirty = 0;
+    b->defer_cap = need;
+  }
+  if (b->dqueue_dirty) HIP_TRY(batch_fill_u32(b->stream, b->dqueue, kNone, b->dqueue_dirty));
+  b->dqueue_dirty = 0;
+  return CUBFF_OK;
+}
+
+// Epochs [first, first + epochs) of the batch as one deferring segment.
+// iev (nullable, 2 * epochs events) brackets each epoch's interpreter
+// launches.
+int run_defer_segment_b(cubff_batch *b, int sched, uint64_t first, uint32_t epochs,
+                        hipEvent_t *iev) {
+  int r;
+  if ((r = ensure_defer_b(b, epochs))) return r;
+  const SchedList &l = b->lists[b->defer_li];
+  int khz = 0;
+  HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+  const BatchDev &d = b->d;
+  DeferArgs a{};
+  a.soup = d.soup;
+  a.owner = b->owner;
+  a.items = b->ditems;
+  a.queue = b->dqueue;
+  a.ctl = b->dctl;
+  a.slots = (uint32_t)l.count;
+  a.epochs = epochs;
+  a.capacity = (uint32_t)(epochs * l.count);
+  a.budget = b->defer_budget;
+  a.max_backlog = 2 * kBatchResolveWaves;
+  a.gen = ++b->defer_gen;
+  if (a.gen == 0) a.gen = b->defer_gen = 1;  // 0 marks never-written entries
+  a.first_epoch = first;
+  a.pair_ops = d.pair_ops;
+  a.pair_steps = d.pair_steps;
+  a.pair_work = d.pair_work;
+  a.counters = d.counters;
+  a.spin_ticks = (uint64_t)(kBatchSpinS * khz * 1000.0);
+  a.idle_ticks = (uint64_t)(kBatchIdleS * khz * 1000.0);
+  a.owe_ticks = (uint64_t)(kBatchOweWaitS * khz * 1000.0);
+  a.num_soups = d.num_soups;
+  a.prog_base = d.prog_base;
+  a.slot_base = d.slot_base;
+  a.sm_seeds = d.sm_seed;
+  a.num_idx = d.num_indices;
+  a.probs = d.prob;
+  a.tgt = d.tgt;
+  a.stage = d.stage;
+  const hipStream_t st = b->stream;
+  hipEvent_t started = nullptr, resolved = nullptr;
+  auto cleanup = [&]() {
+    if (started) (void)hipEventDestroy(started);
+    if (resolved) (void)hipEventDestroy(resolved);
+  };
+  hipError_t e = hipEventCreateWithFlags(&started, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&resolved, hipEventDisableTiming);
+  if (e == hipSuccess) e = batch_fill_u32(st, b->owner, kNone, d.total_programs);
+  if (e == hipSuccess) e = launch_defer_start(st, b->dctl);
+  b->dqueue_dirty = b->defer_cap;  // until this segment ends well
+  if (e == hipSuccess) e = hipEventRecord(started, st);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->stream_r, started, 0);
+  if (e == hipSuccess) e = launch_resolve(b->stream_r, l.lang, a, kBatchResolveWaves);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(CUBFF_EHIP, std::string("deferred run setup failed: ") + hipGetErrorString(e));
+  }
+  for (uint32_t j = 0; r == CUBFF_OK && j < epochs; j++) {
+    r = enqueue_epoch(b, sched, iev ? iev[2 * j] : nullptr, iev ? iev[2 * j + 1] : nullptr, &a,
+                      j);
+    if (r == CUBFF_OK) r = after_epoch(b, sched);
+  }
+  // Close the list whatever happened, so the resolver leaves.
+  e = launch_defer_close(st, b->dctl);
+  if (e == hipSuccess) e = hipEventRecord(resolved, b->stream_r);
+  if (e == hipSuccess) e = hipStreamWaitEvent(st, resolved, 0);
+  DeferCtl ctl{};
+  if (e == hipSuccess) e = hipMemcpyAsync(&ctl, b->dctl, sizeof(ctl), hipMemcpyDeviceToHost, st);
+  hipError_t e2 = hipStreamSynchronize(b->stream_r);
+  if (e == hipSuccess) e = e2;
+  e2 = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = e2;
+  cleanup();
+  if (r) return r;
+  if (e != hipSuccess)
+    return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
+  const uint32_t listed = ctl.final_tail >= kDeferGone ? 0u : ctl.final_tail;
+  b->defer_items += listed;
+  b->dqueue_dirty = std::min<uint64_t>(std::max(listed, ctl.qtail), b->defer_cap);
+  if (ctl.resolved != listed)
+    return fail(CUBFF_EHIP, "deferred-interaction resolver stopped: a wait exceeded " +
+                                std::to_string(kBatchSpinS) + " s; soup state is undefined");
+  // The last epoch's per-slot counts of the interactions the resolver ran.
+  HIP_TRY(launch_b_defer_fixup(st, a, listed));
+  return CUBFF_OK;
+}
+
+// `count` epochs of a run without graph replay: deferring segments where
+// they apply, else one enqueued epoch after another.  iev as above.
+int run_epochs_plain(cubff_batch *b, int sched, uint64_t first, uint64_t count,
+                     hipEvent_t *iev) {
+  int r;
+  uint64_t i = 0;
+  // The learned schedule's first training runs synchronously after its
+  // epoch; that epoch goes first on its own (a resolver beside it would see
+  // the epoch stream stall and leave).
+  if (defer_active(b, 0, sched) && count >= 3 && sched == CUBFF_SCHED_LEARNED &&
+      b->sched.trainings == 0) {
+    if ((r = enqueue_epoch(b, sched, iev ? iev[0] : nullptr, iev ? iev[1] : nullptr))) return r;
+    if ((r = after_epoch(b, sched))) return r;
+    i = 1;
+  }
+  if (defer_active(b, 0, sched) && count - i >= 2) {
+    const uint64_t seg = defer_segment_epochs_b(b);
+    while (count - i >= 2) {
+      const uint64_t ep = std::min(seg, count - i);
+      if ((r = run_defer_segment_b(b, sched, first + i, (uint32_t)ep, iev ? iev + 2 * i : nullptr)))
+        return r;
+      i += ep;
+    }
+  }
+  for (; i < count; i++) {
+    if ((r = enqueue_epoch(b, sched, iev ? iev[2 * i] : nullptr, iev ? iev[2 * i + 1] : nullptr)))
+      return r;
+    if ((r = after_epoch(b, sched))) return r;
+  }
+  return CUBFF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -284,6 +499,7 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
       (r = alloc(&d.fy_base, P)) || (r = alloc(&d.fy_cursor, P)) ||
       (r = alloc(&d.fy_items, P)) || (r = alloc(&d.fy_first, P)) ||
       (r = alloc(&d.fy_succ, P)) || (r = alloc(&b->plain_list, S)) ||
+      (r = alloc(&b->claims, kNumLangs)) ||
       (r = alloc(&b->order_list, S))) {
     d.fy_cnt = cnt;
     d.counters = ctr;
@@ -317,6 +533,8 @@ int cubff_batch_create(cubff_batch **out, uint32_t num_soups, const int *langs,
         for (uint64_t g = b->slot_base[s]; g < b->slot_base[s + 1]; g++) plain.push_back((uint32_t)g);
     if (plain.size() > begin) b->lists.push_back({lang, begin, plain.size() - begin});
   }
+  for (size_t li = 0; li < b->lists.size() && b->defer_li < 0; li++)
+    if (batch_defers(b->lists[li].lang)) b->defer_li = (int)li;
   if (b->lists.size() > 1) {
     e = hipStreamCreateWithFlags(&b->stream_i, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join_i, hipEventDisableTiming);
@@ -380,6 +598,13 @@ int cubff_batch_destroy(cubff_batch *b) {
   release(b->d.fy_first);
   release(b->d.fy_succ);
   release(b->plain_list);
+  release(b->claims);
+  if (b->stream_r) (void)hipStreamSynchronize(b->stream_r);
+  release(b->owner);
+  release(b->ditems);
+  release(b->dqueue);
+  release(b->dctl);
+  if (b->stream_r) (void)hipStreamDestroy(b->stream_r);
   release(b->order_list);
   if (b->stream2) (void)hipStreamSynchronize(b->stream2);
   if (b->stream_i) (void)hipStreamSynchronize(b->stream_i);
@@ -440,10 +665,7 @@ int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
   if (sched == CUBFF_SCHED_LEARNED && (r = ensure_learned(b))) return r;
   if ((r = start_run(b, first_epoch, count, sched))) return r;
   if (!use_graph) {
-    for (uint64_t i = 0; i < count; i++) {
-      if ((r = enqueue_epoch(b, sched, nullptr, nullptr))) return r;
-      if ((r = after_epoch(b, sched))) return r;
-    }
+    if ((r = run_epochs_plain(b, sched, first_epoch, count, nullptr))) return r;
     return finish_run(b, sched);
   }
   if (!b->graph || b->graph_sched != sched || b->graph_wg != b->wg ||
@@ -479,10 +701,7 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch, uint64_t 
   std::vector<hipEvent_t> ev(2 * count + 2);
   for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
   HIP_TRY(hipEventRecord(ev[0], b->stream));
-  for (uint64_t i = 0; i < count; i++) {
-    if ((r = enqueue_epoch(b, sched, ev[2 + 2 * i], ev[3 + 2 * i]))) return r;
-    if ((r = after_epoch(b, sched))) return r;
-  }
+  if ((r = run_epochs_plain(b, sched, first_epoch, count, ev.data() + 2))) return r;
   if ((r = finish_run(b, sched))) return r;
   HIP_TRY(hipEventRecord(ev[1], b->stream));
   HIP_TRY(hipEventSynchronize(ev[1]));
@@ -510,14 +729,17 @@ int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch, uint64_t epochs_p
   // about the same stage of their evolution; times summed per shape.
   float sum[kShapes] = {};
   uint64_t e = first_epoch;
+  // Runs of deferring segments take two epochs at least: each shape is
+  // timed on two-epoch runs then (ms per epoch reported either way).
+  const uint64_t per = defer_active(b, 0, sched) ? 2 : 1;
   for (uint64_t round = 0; round < epochs_per_config; round++) {
     for (int sh = 0; sh < kShapes; sh++) {
       b->wg = kWg[sh / 4];
       b->per_lane = (uint32_t)(sh % 4) + 1;
       float tot = 0, it = 0;
-      if ((r = cubff_batch_run_epochs_timed(b, e, 1, sched, &tot, &it))) return r;
-      e++;
-      sum[sh] += tot;
+      if ((r = cubff_batch_run_epochs_timed(b, e, per, sched, &tot, &it))) return r;
+      e += per;
+      sum[sh] += tot / per;
     }
   }
   int best = 0;
@@ -530,6 +752,37 @@ int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch, uint64_t epochs_p
       ms_table[sh] = epochs_per_config ? sum[sh] / epochs_per_config : 0.f;
   if (best_wg) *best_wg = b->wg;
   if (best_tapes_per_lane) *best_tapes_per_lane = b->per_lane;
+  return CUBFF_OK;
+}
+
+int cubff_batch_set_pipeline(cubff_batch *b, int mode) {
+  int r;
+  if ((r = check(b))) return r;
+  if (mode != CUBFF_FLOW_OFF && mode != CUBFF_FLOW_AUTO && mode != CUBFF_FLOW_DEFER)
+    return fail(CUBFF_EINVAL, "batch pipeline must be CUBFF_FLOW_OFF, _AUTO or _DEFER");
+  b->pipeline = mode;
+  return CUBFF_OK;
+}
+
+int cubff_batch_next_epoch(cubff_batch *b, uint64_t *next) {
+  int r;
+  if ((r = check(b))) return r;
+  if (next) *next = b->drawn == UINT64_MAX ? 0 : b->last + 1;
+  return CUBFF_OK;
+}
+
+int cubff_batch_set_defer_budget(cubff_batch *b, uint32_t budget) {
+  int r;
+  if ((r = check(b))) return r;
+  if (budget == 0) return fail(CUBFF_EINVAL, "defer budget must be >= 1");
+  b->defer_budget = budget;
+  return CUBFF_OK;
+}
+
+int cubff_batch_defer_stats(cubff_batch *b, uint64_t *deferred) {
+  int r;
+  if ((r = check(b))) return r;
+  if (deferred) *deferred = b->defer_items;
   return CUBFF_OK;
 }
 

@@ -48,6 +48,8 @@
 // its L2).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "kernels.h"
 #include "bff_fast.h"
@@ -137,6 +139,17 @@ __device__ bool wait_eq(DeferCtl *c, const uint32_t *p, uint32_t want, uint64_t 
     for (uint32_t q = 0; q < nap; q++) __builtin_amdgcn_s_sleep(1);
     nap = min(nap * 2, 32u);
   }
+}
+
+// Progress of the epoch kernels, which the resolver's stall detection
+// watches: counters[1] (num_runs), bumped once per epoch kernel, for one
+// soup; the control block's heartbeat word for batched soups.
+template <bool kBatch>
+__device__ __forceinline__ uint64_t heartbeat(const DeferArgs &a) {
+  if constexpr (kBatch)
+    return __hip_atomic_load(&a.ctl->pad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    return __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -309,9 +322,254 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
   if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.counters[1], (unsigned long long)*num_indices_p);
 }
 
+// Mutation parameters of slot g of batched soups in epoch `epoch`
+// (common_language.h:166-180 per soup): the soup's size and epoch seed, the
+// slot's index within its soup, the soup's threshold, whether it is
+// evaluated (common_language.h:185).
+struct SlotSoup {
+  uint32_t s;
+  uint64_t n, seed_value, kl;
+  uint32_t prob;
+  bool eval;
+};
+__device__ __forceinline__ SlotSoup slot_soup(const DeferArgs &a, uint64_t g, uint64_t epoch,
+                                              const uint64_t *slot_base) {
+  SlotSoup r;
+  r.s = soup_of(slot_base, a.num_soups, g);
+  r.n = a.prog_base[r.s + 1] - a.prog_base[r.s];
+  r.seed_value = epoch_seed_pre(a.sm_seeds[r.s], epoch);
+  r.kl = g - a.slot_base[r.s];
+  r.prob = a.probs[r.s];
+  r.eval = r.kl < a.num_idx[r.s];
+  return r;
+}
+
+// The deferring epoch kernel of batched soups (languages BffFast runs): a
+// persistent kernel whose lanes take slots of a.list[0, a.slots) as they
+// free up (an atomic on *a.claim per wavefront refill), each slot starting
+// from the tape the stage kernel gathered and mutated.  A lane moves
+// through the states
+//   free -> wait   (claimed; its programs' pending writers are checked
+//                   without blocking the wavefront: a program owed by an
+//                   unfinished list entry is waited for until owe_ticks
+//                   after the claim, then the slot is deferred)
+//        -> run    (a resumable BffFast run advanced kRunSlice commands per
+//                   turn; past the command budget the slot is deferred)
+//        -> done / defer -> free,
+// and wavefronts write back finished slots and create the list entries of
+// deferred ones together, once enough lanes are waiting for that (or
+// nothing else is left).  A half whose program the resolver wrote is read
+// again, coherently, and mutated again (the stage may have read it while it
+// was being written).  Entries, links and the ready queue follow
+// k_epoch_defer's protocol.
+constexpr uint32_t kRunSlice = 8;
+constexpr uint32_t kServiceMin = 16;
+constexpr uint32_t kSlotBaseLds = 256;  // soups + 1 whose slot bases fit in LDS
+enum : uint32_t { kLaneFree = 0, kLaneWait, kLaneRun, kLaneDone, kLaneDefer };
+
+template <int LANG, int LANES>
+__global__ __launch_bounds__(LANES) void k_b_refill_defer(DeferArgs a, uint32_t e_rel) {
+  using L = typename LangOf<LANG>::T;
+  using F = typename FastOf<LANG>::T;
+  using Tape = typename L::template Tape<LANES>;
+  constexpr int kNx = (Tape::kRegionBytes + 15) / 16 * 16;
+  constexpr int kAnchor = kNx + (LinTape<LANES>::kRegionBytes + 15) / 16 * 16;
+  __shared__ __align__(16) uint8_t lds[kAnchor + LdsTape<LANES>::kRegionBytes];
+  __shared__ uint8_t lut[256];
+  __shared__ uint64_t sbase_lds[kSlotBaseLds];
+  __shared__ uint32_t block_budget;
+  const int lane = threadIdx.x, wl = lane & 63;
+  fill_fast_lut<LANG>(lut);
+  // The soups' slot bases in LDS for the per-slot soup lookups.
+  const bool sb_lds = a.num_soups + 1 <= kSlotBaseLds;
+  if (sb_lds)
+    for (uint32_t x = lane; x <= a.num_soups; x += LANES) sbase_lds[x] = a.slot_base[x];
+  const uint64_t *slot_base = sb_lds ? sbase_lds : a.slot_base;
+  if (lane == 0) {
+    // While the resolver is behind, run every slot to the end here and wait
+    // for owed programs as long as it takes (see k_epoch_defer).
+    const uint32_t backlog = ld_agent(&a.ctl->tail) - ld_agent(&a.ctl->resolved);
+    block_budget = backlog > a.max_backlog || ld_agent(&a.ctl->abort) ? 0xffffffffu : a.budget;
+    if (blockIdx.x == 0)
+      __hip_atomic_fetch_add(&a.ctl->pad, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const uint32_t budget0 = block_budget;
+  const bool wait_all = budget0 == 0xffffffffu;
+  const auto t = Tape::at(lds, lane);
+  const auto nx = LinTape<LANES>::at(lds + kNx, lane);
+  const auto anchor = LdsTape<LANES>::at(lds + kAnchor, lane);
+  const LdsByteLut blut{(const lds_u8 *)lut};
+  const uint64_t epoch = a.first_epoch + e_rel;
+  typename F::Run r;
+  r.done = true;
+  r.iters = 0;
+  uint32_t st = kLaneFree, g = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone, budget = budget0;
+  uint64_t deadline = 0;
+  SlotSoup q{};
+  bool exhausted = false;
+  // The slot's input tape: the staged one, with the halves the resolver
+  // wrote read again; then the start of its run.
+  const auto start_slot = [&]() {
+    after_flag();
+    const uint4 *src = reinterpret_cast<const uint4 *>(a.stage + (uint64_t)g * kTape2);
+    uint32_t words[kTape2 / 4];
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+      const uint4 v = src[w];
+      words[4 * w] = v.x;
+      words[4 * w + 1] = v.y;
+      words[4 * w + 2] = v.z;
+      words[4 * w + 3] = v.w;
+    }
+#pragma unroll
+    for (int w = 0; w < kTape2 / 4; w++) t.set_word(w, words[w]);
+    const bool reread = o1 == kResolved || o2 == kResolved;
+    if (o1 == kResolved) {
+      load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+      mutate_half(t, 0, q.n, q.seed_value, q.kl, q.prob);
+    }
+    if (o2 == kResolved) {
+      load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
+      mutate_half(t, 1, q.n, q.seed_value, q.kl, q.prob);
+    }
+    if (!q.eval) {
+      r.done = true;
+      r.iters = 0;
+    } else if (reread) {
+      F::start(r, t, nx, blut, kEvalSteps);
+    } else {
+      F::start_words(r, words, nx, blut, kEvalSteps);
+    }
+    st = kLaneRun;
+  };
+  for (;;) {
+    const uint64_t need = __ballot(st == kLaneFree || st == kLaneDone || st == kLaneDefer);
+    const uint32_t nneed = (uint32_t)__popcll(need);
+    const bool busy = __any(st == kLaneRun || st == kLaneWait);
+    if (nneed >= kServiceMin || (nneed > 0 && !busy)) {
+      if (st == kLaneDone) {  // write back (common_language.h:186-192)
+        uint32_t steps = 0, work = 0, ops = 0;
+        if (q.eval) ops = F::finish(r, kEvalSteps, &steps, &work);
+        store_row(t, 0, a.soup + (uint64_t)p1 * kTape);
+        store_row(t, 1, a.soup + (uint64_t)p2 * kTape);
+        if (o1 != kNone) st_agent(a.owner + p1, kNone);
+        if (o2 != kNone) st_agent(a.owner + p2, kNone);
+        a.pair_ops[g] = ops;
+        a.pair_steps[g] = steps;
+        a.pair_work[g] = work;
+        st = kLaneFree;
+      }
+      const bool defer = st == kLaneDefer;
+      const uint32_t at = wave_alloc_u32(&a.ctl->tail, defer ? 1u : 0u);
+      // The resolver left before anything was deferred: run the slot here
+      // to the end.
+      if (defer && at >= kDeferGone) {
+        budget = 0xffffffffu;
+        start_slot();
+      }
+      const bool put = defer && at < a.capacity;
+      if (defer && at >= a.capacity && at < kDeferGone) st_agent(&a.ctl->abort, 1);
+      const uint32_t d1 = o1 < kResolved ? o1 : kNone;
+      const uint32_t d2 = (o2 < kResolved && p2 != p1) ? o2 : kNone;
+      DeferItem *it = a.items + (put ? at : 0u);
+      if (put) {
+        st_agent(&it->e, e_rel | (q.eval ? 0u : kNoEval));
+        st_agent(&it->k, g);
+        st_agent(&it->p1, p1);
+        st_agent(&it->p2, p2);
+        st_agent(&it->dep1, d1);
+        st_agent(&it->dep2, d2);
+        st_agent(&it->succ1, kNone);
+        st_agent(&it->succ2, kNone);
+        st_agent(&it->pending, (uint32_t)(d1 != kNone) + (uint32_t)(d2 != kNone) + 1u);
+      }
+      if (__ballot(put)) publish_point();
+      if (put) {
+        st_agent(&it->ready, a.gen);
+        st_agent(a.owner + p1, at);
+        st_agent(a.owner + p2, at);
+        uint32_t sat = 0;
+        if (d1 != kNone) {
+          DeferItem *d = a.items + d1;
+          uint32_t *slot = ld_agent(&d->p1) == p1 ? &d->succ1 : &d->succ2;
+          sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 kLinkDone;
+        }
+        if (d2 != kNone) {
+          DeferItem *d = a.items + d2;
+          uint32_t *slot = ld_agent(&d->p1) == p2 ? &d->succ1 : &d->succ2;
+          sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                 kLinkDone;
+        }
+        if (__hip_atomic_fetch_sub(&it->pending, sat + 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) == sat + 1u)
+          push_ready(a, at);
+        a.pair_ops[g] = 0;
+        a.pair_steps[g] = 0;
+        a.pair_work[g] = budget;
+      }
+      if (defer && !(at >= kDeferGone)) st = kLaneFree;
+      if (!exhausted) {
+        const uint64_t fm = __ballot(st == kLaneFree);
+        const uint32_t nf = (uint32_t)__popcll(fm);
+        uint32_t base = 0;
+        if (wl == 0 && nf) base = atomicAdd(a.claim, nf);
+        base = __shfl(base, 0, 64);
+        exhausted = (uint64_t)base + nf >= a.slots;
+        const uint64_t e = (uint64_t)base + __popcll(fm & ((1ull << wl) - 1));
+        if (st == kLaneFree && e < a.slots) {
+          g = a.list[e];
+          q = slot_soup(a, g, epoch, slot_base);
+          p1 = a.tgt[2 * (uint64_t)g];
+          p2 = a.tgt[2 * (uint64_t)g + 1];
+          o1 = ld_agent(a.owner + p1);
+          o2 = ld_agent(a.owner + p2);
+          budget = budget0;
+          deadline = wall_clock64() + (wait_all ? a.spin_ticks : a.owe_ticks);
+          st = kLaneWait;
+        }
+      }
+    }
+    if (!__any(st != kLaneFree)) {
+      if (exhausted) break;
+      continue;
+    }
+    if (st == kLaneWait) {
+      if (o1 < kResolved && ld_agent(&a.items[o1].done) == a.gen) o1 = kResolved;
+      if (o2 < kResolved && ld_agent(&a.items[o2].done) == a.gen) o2 = kResolved;
+      if (o1 >= kResolved && o2 >= kResolved) {
+        start_slot();
+      } else if (ld_agent(&a.ctl->abort) || wall_clock64() > deadline) {
+        if (wait_all && !ld_agent(&a.ctl->abort)) st_agent(&a.ctl->abort, 1);
+        st = kLaneDefer;
+      }
+    }
+    if (st == kLaneRun) {
+      if (!r.done) F::template advance<true>(r, t, nx, anchor, blut, kEvalSteps, kRunSlice);
+      if (r.done) st = kLaneDone;
+      else if (r.iters >= budget) st = kLaneDefer;
+    }
+    // Only lanes waiting on owed programs: give the memory system a rest.
+    if (!__any(st == kLaneRun) && __any(st == kLaneWait)) __builtin_amdgcn_s_sleep(2);
+  }
+}
+
+// After a batched segment: the last epoch's per-slot counts of the
+// entries the resolver ran.
+__global__ void k_b_defer_fixup(DeferArgs a, uint32_t listed) {
+  const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= listed) return;
+  const DeferItem &it = a.items[x];
+  if ((it.e & ~kNoEval) + 1 == a.epochs) {
+    a.pair_ops[it.k] = it.ops;
+    a.pair_steps[it.k] = it.steps;
+  }
+}
+
 // The resolver: one deferred interaction per wavefront at a time, run by
 // lane 0 (the list is short and its entries are the long interactions).
-template <int LANG>
+template <int LANG, bool kBatch>
 __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
   using L = typename LangOf<LANG>::T;
   using Tape = typename L::template Tape<1>;
@@ -348,8 +606,7 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
       // The epoch kernels bump counters[1] once each: while they progress,
       // waiting here is not a stall.
       uint64_t t0 = wall_clock64();
-      unsigned long long beat = __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
+      uint64_t beat = heartbeat<kBatch>(a);
       uint32_t nap = 1;
       for (uint32_t poll = 0;; poll++) {
         const uint32_t v = ld_agent(a.queue + qi);
@@ -362,8 +619,7 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
           const uint32_t traw = ld_agent(&c->tail);
           if (traw >= kDeferGone) break;  // another wavefront left with the list empty
           if (ld_agent(&c->closed) && qi >= ld_agent(&c->final_tail)) break;
-          const unsigned long long b2 =
-              __hip_atomic_load(&a.counters[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t b2 = heartbeat<kBatch>(a);
           if (b2 != beat) {
             beat = b2;
             t0 = wall_clock64();
@@ -403,7 +659,14 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     const bool eval = !(ew & kNoEval);
     load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
     load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
-    mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+    uint32_t s = 0;
+    if constexpr (kBatch) {
+      const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
+      s = q.s;
+      mutate(t, q.n, q.seed_value, q.kl, q.prob);
+    } else {
+      mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+    }
     uint32_t steps = 0, ops = 0;
     if (eval) {
       if constexpr (kFast)
@@ -416,9 +679,14 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     }
     store_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
     store_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
-    if (ops) atomicAdd(&a.counters[0], (unsigned long long)ops);
-    if (steps) atomicAdd(&a.counters[2], (unsigned long long)steps);
-    if (e + 1 == a.epochs) {
+    unsigned long long *ctr = a.counters + (kBatch ? 4 * (uint64_t)s : 0);
+    if (ops) atomicAdd(&ctr[0], (unsigned long long)ops);
+    if (steps) atomicAdd(&ctr[2], (unsigned long long)steps);
+    if constexpr (kBatch) {
+      // The slot's per-epoch counts are written after the segment, from here.
+      st_agent(&it->ops, ops);
+      st_agent(&it->steps, steps);
+    } else if (e + 1 == a.epochs) {
       a.pair_ops[k] = ops;
       a.pair_steps[k] = steps;
     }
@@ -474,7 +742,10 @@ hipError_t epoch_defer_lang(hipStream_t st, const DeferArgs &a, const uint32_t *
 
 template <int L>
 hipError_t resolve_lang(hipStream_t st, const DeferArgs &a, unsigned waves) {
-  hipLaunchKernelGGL(k_resolve<L>, dim3(waves), dim3(64), 0, st, a);
+  if (a.num_soups)
+    hipLaunchKernelGGL((k_resolve<L, true>), dim3(waves), dim3(64), 0, st, a);
+  else
+    hipLaunchKernelGGL((k_resolve<L, false>), dim3(waves), dim3(64), 0, st, a);
   return hipGetLastError();
 }
 
@@ -514,6 +785,56 @@ hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,
 
 hipError_t launch_resolve(hipStream_t st, int lang, const DeferArgs &a, unsigned waves) {
   CUBFF_DEFER_LANG(lang, resolve_lang, (st, a, waves))
+}
+
+bool batch_defers(int lang) {
+  return lang == kBff || lang == kBffNoheads || lang == kBffNoheads4bit || lang == kBffPerm;
+}
+
+template <int LANES>
+hipError_t b_refill_defer_lanes(hipStream_t st, int lang, const DeferArgs &a, uint32_t e_rel,
+                                uint32_t waves_per_simd) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      return hipErrorInvalidDevice;
+    cus = v;
+  }
+  const uint64_t want = (uint64_t)cus * 4 * waves_per_simd * 64 / LANES;
+  const dim3 grid((unsigned)std::min<uint64_t>(want, blocks_of(a.slots, LANES))), block(LANES);
+  switch (lang) {
+#define CUBFF_RD_CASE(LG) \
+  case LG: hipLaunchKernelGGL((k_b_refill_defer<LG, LANES>), grid, block, 0, st, a, e_rel); break;
+    CUBFF_RD_CASE(kBff)
+    CUBFF_RD_CASE(kBffNoheads)
+    CUBFF_RD_CASE(kBffNoheads4bit)
+    CUBFF_RD_CASE(kBffPerm)
+#undef CUBFF_RD_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_b_epoch_defer(hipStream_t st, int lang, const DeferArgs &a, uint32_t e_rel,
+                                int wg, uint32_t waves_per_simd) {
+  if (a.slots == 0) return hipSuccess;
+  if (!a.num_soups || !a.list || !a.stage || !a.tgt || !a.claim) return hipErrorInvalidValue;
+  if (waves_per_simd == 0) waves_per_simd = 1;
+  switch (wg) {
+    case 64: return b_refill_defer_lanes<64>(st, lang, a, e_rel, waves_per_simd);
+    case 128: return b_refill_defer_lanes<128>(st, lang, a, e_rel, waves_per_simd);
+    case 256: return b_refill_defer_lanes<256>(st, lang, a, e_rel, waves_per_simd);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_b_defer_fixup(hipStream_t st, const DeferArgs &a, uint32_t listed) {
+  if (listed == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_b_defer_fixup, dim3(blocks_of(listed, 256)), dim3(256), 0, st, a, listed);
+  return hipGetLastError();
 }
 
 }  // namespace cubff

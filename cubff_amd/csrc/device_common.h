@@ -82,4 +82,16 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
+// Soup of a program / slot index of batched soups: base is a prefix-sum
+// array of length num_soups + 1.
+__device__ __forceinline__ uint32_t soup_of(const uint64_t *base, uint32_t num_soups,
+                                            uint64_t x) {
+  uint32_t lo = 0, hi = num_soups;  // base[lo] <= x < base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (base[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 }  // namespace cubff

@@ -71,10 +71,10 @@ struct alignas(16) DeferCtl {
 // written (done); the entries that wait on this one for its first and its
 // second program (kNone until one links, kLinkDone once this entry is
 // done), and the count of its unfinished dependencies (plus one while its
-// creator links it).
+// creator links it); the interaction's ops and steps once run (batches).
 struct DeferItem {
   uint32_t e, k, p1, p2, dep1, dep2, ready, done;
-  uint32_t succ1, succ2, pending, pad[5];
+  uint32_t succ1, succ2, pending, ops, steps, pad[3];
 };
 constexpr uint32_t kLinkDone = 0xfffffffdu;
 
@@ -101,6 +101,20 @@ struct DeferArgs {
   uint64_t spin_ticks;
   uint64_t idle_ticks;  // resolver: leave if nothing arrives and no epoch ends this long
   uint64_t owe_ticks;   // epoch kernel: wait this long for an owing entry before deferring
+  // Batched soups (num_soups > 0; batch.h): soup s owns programs
+  // [prog_base[s], prog_base[s+1]) and pair slots [slot_base[s],
+  // slot_base[s+1]); `k` of a list entry is a slot over all soups.  An
+  // epoch launch runs the slots list[0, slots); each slot's gathered and
+  // mutated tape is stage[slot] and its programs tgt[2 slot], tgt[2 slot+1].
+  // Counters are per soup (counters + 4 s); per-slot ops and steps of every
+  // epoch go to pair_ops / pair_steps (0 for a deferred slot, whose counts
+  // the resolver adds to its soup's counters).
+  uint32_t num_soups;
+  const uint64_t *prog_base, *slot_base, *sm_seeds, *num_idx;
+  const uint32_t *probs, *list, *tgt;
+  const uint8_t *stage;
+  uint32_t *pair_work;  // per slot: commands executed (the budget for a deferred slot)
+  uint32_t *claim;      // next unclaimed position of list (0 at the launch)
 };
 
 hipError_t launch_init(hipStream_t st, uint8_t *soup, uint64_t n,
@@ -172,5 +186,13 @@ hipError_t launch_defer_close(hipStream_t st, DeferCtl *c);
 hipError_t launch_epoch_defer(hipStream_t st, int lang, const DeferArgs &a,
                               const uint32_t *shuf, const uint64_t *num_indices, uint32_t e_rel);
 hipError_t launch_resolve(hipStream_t st, int lang, const DeferArgs &a, unsigned waves);
+// Batched soups: the deferring epoch kernel over a.list[0, a.slots) for a
+// language BffFast runs (batch_defers()), persistent with workgroups of wg
+// lanes, waves_per_simd wavefronts per SIMD; and after the resolver is
+// done, the last epoch's per-slot counts of the first `listed` entries.
+bool batch_defers(int lang);
+hipError_t launch_b_epoch_defer(hipStream_t st, int lang, const DeferArgs &a, uint32_t e_rel,
+                                int wg, uint32_t waves_per_simd);
+hipError_t launch_b_defer_fixup(hipStream_t st, const DeferArgs &a, uint32_t listed);
 
 }  // namespace cubff

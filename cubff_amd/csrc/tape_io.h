@@ -49,4 +49,15 @@ __device__ __forceinline__ void mutate(const Tape &t, uint64_t n,
   }
 }
 
+// The same draws for the bytes of one half only.
+template <class Tape>
+__device__ __forceinline__ void mutate_half(const Tape &t, int half, uint64_t n,
+                                            uint64_t seed_value, uint64_t k, uint32_t prob) {
+  const uint64_t base = (n * seed_value + k) * (uint64_t)kTape2;
+  for (int b = half * kTape; b < (half + 1) * kTape; b++) {
+    const uint64_t r = splitmix64(base + b);
+    if (((uint32_t)(r >> 8) & 0x3fffffffu) < prob) t.set(b, (uint8_t)r);
+  }
+}
+
 }  // namespace cubff

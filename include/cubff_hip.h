@@ -272,10 +272,13 @@ int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch,
                                  float *interp_ms);
 /* Runs epochs_per_config live epochs under each of the 12 launch shapes
  * (workgroup 64/128/256 x 1-4 tapes per lane), in rounds of one epoch per
- * shape so that every shape sees the soups at the same stage, keeps the
- * fastest by total time and reports it; ms_table (nullable, 12 floats)
- * receives ms/epoch per shape.  Results do not depend on the launch shape,
- * so the tuning epochs are real epochs. */
+ * shape (two when runs go through deferring segments, which then run
+ * 2 * epochs_per_config per shape) so that every shape sees the soups at
+ * the same stage, keeps the fastest by total time and reports it; ms_table
+ * (nullable, 12 floats) receives ms/epoch per shape.  Results do not depend
+ * on the launch shape, so the tuning epochs are real epochs.  For the
+ * persistent interpreters "tapes per lane" is the wavefronts per SIMD of
+ * their grid. */
 int cubff_batch_autotune(cubff_batch *b, uint64_t first_epoch,
                          uint64_t epochs_per_config, int sched, int *best_wg,
                          uint32_t *best_tapes_per_lane, float *ms_table);
@@ -296,6 +299,24 @@ int cubff_batch_read_shuffle(cubff_batch *b, uint32_t soup, uint32_t *s);
 /* Learned-scheduler statistics: trainings so far, the loss of the last
  * training round, and the last epoch's slot count per length bucket
  * (8 counts, bucket 0 = longest predicted). */
+/* Multi-epoch path of runs of two or more epochs without graph replay:
+ * deferring segments run the first language of the batch that the
+ * deferring epoch kernel supports (bff, bff_noheads, bff_noheads_4bit,
+ * bff_perm) with a persistent kernel whose lanes take new pairs as they
+ * finish and hand interactions past the command budget to a resolver that
+ * finishes them beside the later epochs (as cubff_set_pipeline's
+ * CUBFF_FLOW_DEFER does for one soup).  CUBFF_FLOW_AUTO (default) uses them
+ * under CUBFF_SCHED_PLAIN, CUBFF_FLOW_DEFER under both schedules,
+ * CUBFF_FLOW_OFF never (every epoch through the interpreter rounds).
+ * Results are the same. */
+int cubff_batch_set_pipeline(cubff_batch *b, int mode);
+/* The epoch after the last one run (0 before any run). */
+int cubff_batch_next_epoch(cubff_batch *b, uint64_t *next);
+/* Command budget of an interaction in the deferring epoch kernel (default
+ * 768; >= 1). */
+int cubff_batch_set_defer_budget(cubff_batch *b, uint32_t budget);
+/* Interactions handed to the resolver since the batch was created. */
+int cubff_batch_defer_stats(cubff_batch *b, uint64_t *deferred);
 int cubff_batch_sched_stats(cubff_batch *b, uint64_t *trainings,
                             float *last_loss, uint32_t *bucket_counts);
 

@@ -67,7 +67,7 @@ template <int MAP, bool HEADS, bool ACCEL>
 int check(const char *name, int orc_lang, int count, uint64_t seed) {
   HostLut<MAP> lut;
   uint8_t tape[129], ref[128], nxb[129];
-  uint64_t cap_runs = 0, over = 0, ops_sum = 0, full_runs = 0;
+  uint64_t cap_runs = 0, over = 0, ops_sum = 0, full_runs = 0, resumed = 0;
   for (int r = 0; r < count; r++) {
     make_tape(MAP, seed * 1000003 + r, tape);
     const uint64_t x = sm(seed ^ (0x9e37ull * (r + 1)));
@@ -113,10 +113,50 @@ int check(const char *name, int orc_lang, int count, uint64_t seed) {
     }
     cap_runs += steps == stepcount;
     ops_sum += ops;
+    // The same run resumed: start (or start_words, from the tape's words),
+    // then advance() in slices of 1..64 commands until it ends (with the
+    // cycle skip every other tape).
+    {
+      using F = BffFast<MAP, HEADS, ACCEL>;
+      uint8_t t2[129], nx2[129], anchor2[128];
+      make_tape(MAP, seed * 1000003 + r, t2);
+      typename F::Run run;
+      if (r & 2) {
+        uint32_t words[32];
+        for (int w = 0; w < 32; w++) words[w] = HostBytes{t2}.word(w);
+        F::start_words(run, words, HostBytes{nx2}, lut, stepcount);
+      } else {
+        F::start(run, HostBytes{t2}, HostBytes{nx2}, lut, stepcount);
+      }
+      uint64_t calls = 0;
+      while (!run.done) {
+        const uint32_t slice = 1 + (uint32_t)(sm(x + 77 * calls) % 64);
+        if (full)
+          F::template advance<true>(run, HostBytes{t2}, HostBytes{nx2}, HostBytes{anchor2}, lut,
+                                    stepcount, slice);
+        else
+          F::template advance<false>(run, HostBytes{t2}, HostBytes{nx2},
+                                     typename F::NoAux{}, lut, stepcount, slice);
+        if (++calls > 100000) {
+          printf("%s tape %d: resumed run does not end\n", name, r);
+          return 1;
+        }
+      }
+      uint32_t steps2 = 0, work2 = 0;
+      const uint32_t ops2 = F::finish(run, stepcount, &steps2, &work2);
+      if (ops2 != ref_ops || steps2 != ref_steps || memcmp(t2, ref, 128) != 0) {
+        printf("%s tape %d cap %u: resumed ops %u/%llu steps %u/%llu tape %s\n", name, r,
+               stepcount, ops2, (unsigned long long)ref_ops, steps2,
+               (unsigned long long)ref_steps, memcmp(t2, ref, 128) ? "differs" : "same");
+        return 1;
+      }
+      resumed++;
+    }
   }
   printf("%s: %d tapes ok (%llu at the step limit, %llu over budget, %llu with the cycle skip, "
-         "%llu ops)\n", name, count, (unsigned long long)cap_runs, (unsigned long long)over,
-         (unsigned long long)full_runs, (unsigned long long)ops_sum);
+         "%llu resumed in slices, %llu ops)\n", name, count, (unsigned long long)cap_runs,
+         (unsigned long long)over, (unsigned long long)full_runs, (unsigned long long)resumed,
+         (unsigned long long)ops_sum);
   return 0;
 }
 

@@ -81,7 +81,10 @@ def test_batch_autotune_keeps_results(capi):
             x.init([s[2] for s in spec], [s[3] for s in spec])
         wg, tpl, table = a.autotune(0, 1)
         assert wg in (64, 128, 256) and 1 <= tpl <= 4 and all(t > 0 for t in table)
-        b.run_epochs(0, 12)
+        # 12 tuning epochs, 24 where they run as two-epoch deferring segments.
+        tuned = a.next_epoch()
+        assert tuned in (12, 24)
+        b.run_epochs(0, tuned)
         for i in range(len(spec)):
             assert np.array_equal(a.read_soup(i), b.read_soup(i))
 
@@ -130,3 +133,36 @@ def test_config3_full_size_matches_oracle(capi):
                 assert np.array_equal(b.read_soup(k), ref.soup), (k, e)
                 ops, steps = b.read_pair_stats(k)
                 assert np.array_equal(ops, po) and np.array_equal(steps, ps), (k, e)
+
+
+@pytest.mark.parametrize("budget,sched", [(768, "plain"), (6, "plain"), (24, "learned")])
+def test_batch_deferring_segments_match_oracle(capi, budget, sched):
+    # Runs of several epochs go through deferring segments (the bff-family
+    # soups: long interactions resolved beside later epochs; the others
+    # through the interpreter launches).  A small command budget defers most
+    # interactions and builds long chains of entries.  After every run the
+    # soups, pairings, counters and the last epoch's per-slot counts equal
+    # the oracle's.
+    E = 7
+    refs = [_oracle_run(l, n, s, p, E) for (l, n, s, p) in SPEC]
+    mode = capi.SCHED_LEARNED if sched == "learned" else capi.SCHED_PLAIN
+    with capi.Batch([x[0] for x in SPEC], [x[1] for x in SPEC]) as b:
+        b.init([x[2] for x in SPEC], [x[3] for x in SPEC])
+        b.set_defer_budget(budget)
+        b.set_pipeline(capi.FLOW_DEFER)  # under both schedules
+        print("deferring segments", budget, sched, flush=True)
+        for first, count in ((0, 3), (3, 4)):
+            b.run_epochs(first, count, mode)
+            e = first + count - 1
+            for i, (l, n, s, p) in enumerate(SPEC):
+                soup, po, ps, _ = refs[i][e]
+                assert np.array_equal(b.read_shuffle(i), O.pairing_shuffle(n, s, e)), (i, e)
+                assert np.array_equal(b.read_soup(i), soup), (i, e)
+                ops, steps = b.read_pair_stats(i)
+                assert np.array_equal(ops, po) and np.array_equal(steps, ps), (i, e)
+                tot = sum(refs[i][x][3] for x in range(e + 1))
+                assert b.read_counters(i)[:2] == (tot, (e + 1) * (n // 2)), (i, e)
+        deferred = b.defer_stats()
+        print("deferred", deferred, flush=True)
+        if budget < 100:  # these soups' interactions rarely run past 768 commands
+            assert deferred > 0

@@ -208,7 +208,8 @@ def test_learned_buckets_cut_interpreter_time(capi):
     # next epoch's inference runs beside the interpreter.  Two tapes per
     # lane, the shape the autotuner picks for this batch (with one, the
     # interpreter's workgroups leave the inference no room and whole epochs
-    # are slower).
+    # are slower).  Both arms run the interpreter rounds, the method the
+    # order serves (plain runs default to deferring segments instead).
     S = 64
     ns = [[4096, 16384, 65536, 131072][k % 4] for k in range(S)]
     probs = [int(round(10 ** (-5 + 2 * k / 63) * (1 << 30))) for k in range(S)]
@@ -220,6 +221,7 @@ def test_learned_buckets_cut_interpreter_time(capi):
         with capi.Batch(["bff_noheads"] * S, ns) as b:
             b.init(list(range(S)), probs)
             b.set_launch(64, 2)
+            b.set_pipeline(capi.FLOW_OFF)
             b.run_epochs(0, 8, sched)
             tot, it = b.run_epochs_timed(8, 12, sched)
             interp[name] = it
