@@ -1,0 +1,466 @@
+// BFF interpreter of the standard jump logic (bff, bff_noheads,
+// bff_noheads_4bit, bff_perm) laid out for the shortest per-command chain
+// of one lane.  An epoch's budgeted interaction lasts as long as its lane's
+// commands take, so what counts is the instructions one command issues,
+// not throughput.
+//
+// Per lane and tape position p there is one 32-bit LDS word:
+//   byte 0  the tape byte,
+//   byte 1  its code: class in bits 4-6 (none, head move, write, '[', ']');
+//           in bits 0-3 the BffKind of a command byte, and for any other
+//           byte a lower bound (1..15) on how many increments or
+//           decrements it is from a command byte,
+//   byte 2  the next command position after p (128 if none), kept at
+//           command positions,
+//   byte 3  the matching bracket of a bracket at p (0xff if unmatched).
+// A command is then three word loads issued together (at pc and under the
+// two heads): opcode, next command and match come with the byte at pc, the
+// class of the byte a write replaces comes with the bytes under the heads.
+// The run of comment bytes after a command (no-op steps, bff.inc.h:346-347)
+// is skipped through byte 2.  Every command stores the low half-word under
+// the written head (the byte it writes and its code; commands that write
+// nothing store head 0's back), so no branch picks a store.
+//
+// Everything that is not the common case goes through one branch:
+//   * a write that changes a byte's class, or an increment / decrement of
+//     a command byte or of a byte whose bound says a command may be one
+//     step away (codes are exact there; elsewhere the common path lowers
+//     the bound by one): class masks, next-command bytes and matches
+//     follow, and the cached jumps whose range the byte lies in are dropped;
+//   * a taken jump other than the cached backward and forward one: its
+//     target comes from the match byte (the scan of bff.inc.h:317-344
+//     done ahead of time) and is cached;
+//   * a taken backward jump of a loop whose body allows loop acceleration.
+//
+// Loop acceleration (exact, as in the budgeted interpreter of bff_fast.h):
+// a taken backward jump from the same `]` exactly one body iteration after
+// the last one, whose body between the brackets holds no bracket, repeats
+// the last iteration when (a) the body writes nothing -- the tape is
+// constant, the heads advance by fixed deltas and the loop runs until the
+// byte under head 0 at the `]` is null, found by walking head 0's orbit (at
+// most 128 reads) -- or (b) the body moves no head and the bytes under the
+// heads are the last iteration's: the whole state repeats.  Whole
+// iterations are counted (steps and ops) without being run, up to the step
+// cap and the exiting iteration, which runs normally.  "Exactly one
+// iteration" is checked by the command count: a straight body runs its
+// commands and the `]` between two taken visits of the `]`, and any other
+// path between them runs more.  A loop whose state kept changing over
+// three iterations is not looked at again until its jump is cached anew.
+//
+// Steps add the skipped comment bytes to the executed commands, capped at
+// stepcount; ops (the value Evaluate returns, bff.inc.h:356-389) are the
+// executed commands plus those of skipped iterations.
+#pragma once
+
+#include "bff_fast.h"
+
+namespace cubff {
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
+
+// The per-lane words in LDS for LANES lanes: word (p, lane) at
+// (p * (LANES + 1) + lane) * 4, so lanes at one position use consecutive
+// words and any word is one multiply-add away.  Position 128 is the word a
+// lane whose run has stopped reads.
+template <int LANES>
+struct LeanLds {
+  static constexpr int kStride = LANES + 1;
+  static constexpr int kRegionBytes = (kTape2 + 1) * kStride * 4;
+  lds_u32 *p;
+  __device__ __forceinline__ static LeanLds at(uint8_t *region, int lane) {
+    return LeanLds{(lds_u32 *)region + lane};
+  }
+  __device__ __forceinline__ uint32_t word(uint32_t pos) const { return p[pos * kStride]; }
+  __device__ __forceinline__ void set_word(uint32_t pos, uint32_t w) const { p[pos * kStride] = w; }
+  __device__ __forceinline__ void set_low(uint32_t pos, uint32_t v) const {
+    *(lds_u16 *)(p + pos * kStride) = (uint16_t)v;
+  }
+  __device__ __forceinline__ void set_byte(uint32_t pos, int b, uint32_t v) const {
+    ((lds_u8 *)(p + pos * kStride))[b] = (uint8_t)v;
+  }
+};
+
+template <int MAP, bool HEADS>
+struct BffLean {
+  using C = BffClass<MAP>;
+  // Per-kind fields, 2 bits each, indexed by BffKind: head-0 delta + 1,
+  // head-1 delta + 1, byte delta + 1 (kNull and kNoop change nothing).
+  static constexpr uint32_t kTH0 = 0x558555;
+  static constexpr uint32_t kTH1 = 0x585555;
+  static constexpr uint32_t kTDelta = 0x555525;
+  static constexpr uint32_t kNoMatch = 0xff;
+
+  __host__ __device__ __forceinline__ static uint32_t class_of_kind(uint32_t k) {
+    return k == kLoopStart ? kClsOpen
+                           : (k == kLoopEnd ? kClsClose
+                                            : ((k - kPlus) < 4u ? kClsWrite
+                                                                : ((k - kDec0) < 4u ? kClsMove : kClsNone)));
+  }
+  // Byte 1 of a position's word for tape byte v: kind and class of a
+  // command byte; for any other byte the number of increments or
+  // decrements to the nearest command byte, at most 15.
+  __host__ __device__ static uint32_t code_of(uint32_t v) {
+    if (C::cmd(v)) {
+      const uint32_t k = C::kind(v);
+      return k | (class_of_kind(k) << 4);
+    }
+    uint32_t d = 1;
+    while (d < 15 && !C::cmd((v + d) & 0xff) && !C::cmd((v - d) & 0xff)) d++;
+    return d;
+  }
+
+  struct Run {
+    BffMasks m;                          // class masks of the tape
+    int pc, h0, h1;
+    uint32_t i, iters, extra;
+    bool done;
+    // Jump caches: the last taken backward / forward jump's source, next
+    // command and steps (the jump itself and the comment bytes after its
+    // target); for the backward one, whether its body allows acceleration,
+    // holds a write, and the commands of one iteration.
+    int bsrc, bq, bnext, binc, fsrc, fq, fnext, finc;
+    bool bacc, bwrite;
+    uint32_t bcmds;
+    // Loop anchor: its `]`, heads, steps and commands after it, the bytes
+    // under the heads; failed comparisons in a row.
+    int aq, ah0, ah1;
+    uint32_t ai, aiters, aw0, aw1, afail;
+  };
+
+  // Matches of every bracket (byte 3), from the masks.
+  template <class W>
+  __host__ __device__ static void match_all(const W &w, const BffMasks &m) {
+    Mask128 br{m.open.lo | m.close.lo, m.open.hi | m.close.hi};
+    Mask128 un{0, 0};  // '[' not matched yet
+    while (br.lo | br.hi) {
+      const int p = br.lo ? __builtin_ctzll(br.lo) : 64 + __builtin_ctzll(br.hi);
+      br.flip(p, true);
+      const bool open = p < 64 ? ((m.open.lo >> p) & 1) : ((m.open.hi >> (p - 64)) & 1);
+      if (open) {
+        un.flip(p, true);
+        w.set_byte(p, 3, kNoMatch);
+      } else {
+        const int q = un.prev_upto(p - 1);
+        if (q >= 0) {
+          un.flip(q, true);
+          w.set_byte(q, 3, (uint32_t)p);
+        }
+        w.set_byte(p, 3, q >= 0 ? (uint32_t)q : kNoMatch);
+      }
+    }
+  }
+
+  // Words, masks and matches of a tape whose 32 words (byte 4w + j is
+  // byte j of words[w]) are in registers, and the initial state.  lut(v)
+  // is code_of(v) from a table.
+  template <class W, class L>
+  __host__ __device__ __forceinline__ static void start_words(Run &r, const W &w, const L &lut,
+                                                              const uint32_t *words,
+                                                              uint32_t stepcount) {
+    uint32_t mv[4] = {0, 0, 0, 0}, wr[4] = {0, 0, 0, 0}, op[4] = {0, 0, 0, 0},
+             cl[4] = {0, 0, 0, 0};
+    uint32_t next = kTape2;
+#pragma unroll
+    for (int p = kTape2 - 1; p >= 0; p--) {
+      const uint32_t v = (words[p >> 2] >> (8 * (p & 3))) & 0xff;
+      const uint32_t code = lut(v);
+      const uint32_t oh = 1u << (code >> 4);  // class as one bit
+      mv[p >> 5] |= ((oh >> kClsMove) & 1) << (p & 31);
+      wr[p >> 5] |= ((oh >> kClsWrite) & 1) << (p & 31);
+      op[p >> 5] |= ((oh >> kClsOpen) & 1) << (p & 31);
+      cl[p >> 5] |= ((oh >> kClsClose) & 1) << (p & 31);
+      w.set_word(p, v | (code << 8) | (next << 16) | (kNoMatch << 24));
+      next = (code >> 4) != kClsNone ? (uint32_t)p : next;
+    }
+    const auto m64 = [](const uint32_t *x) {
+      return Mask128{x[0] | ((uint64_t)x[1] << 32), x[2] | ((uint64_t)x[3] << 32)};
+    };
+    BffMasks &m = r.m;
+    m.move = m64(mv);
+    m.write = m64(wr);
+    m.open = m64(op);
+    m.close = m64(cl);
+    m.cmd = Mask128{m.move.lo | m.write.lo | m.open.lo | m.close.lo,
+                    m.move.hi | m.write.hi | m.open.hi | m.close.hi};
+    // Position 128: where a lane that has stopped parks.
+    w.set_word(kTape2, (kNoop << 8) | ((uint32_t)kTape2 << 16) | (kNoMatch << 24));
+    if ((m.open.lo | m.open.hi | m.close.lo | m.close.hi) != 0) match_all(w, m);
+    init(r, words[0] & 0xff, (words[0] >> 8) & 0xff, stepcount);
+  }
+
+  __host__ __device__ __forceinline__ static void init(Run &r, uint32_t b0, uint32_t b1,
+                                                       uint32_t stepcount) {
+    int pc;
+    if constexpr (HEADS) {  // InitialState, bff.inc.h:269-275
+      r.h0 = b0 & (kTape2 - 1);
+      r.h1 = b1 & (kTape2 - 1);
+      pc = 2;
+    } else {  // bff.inc.h:277-282
+      r.h0 = 0;
+      r.h1 = 0;
+      pc = 0;
+    }
+    r.iters = 0;
+    r.extra = 0;
+    const int first = r.m.cmd.next_from(pc);
+    r.i = fast_umin((uint32_t)(first - pc), stepcount);
+    r.pc = first;
+    r.done = !(r.i < stepcount && first < kTape2);
+    r.bsrc = -1; r.bq = -1; r.bnext = 0; r.binc = 0;
+    r.fsrc = -1; r.fq = kTape2; r.fnext = 0; r.finc = 0;
+    r.bacc = r.bwrite = false;
+    r.bcmds = 0;
+    r.aq = -1; r.ah0 = 0; r.ah1 = 0;
+    r.ai = 0; r.aiters = 0; r.aw0 = 0; r.aw1 = 0; r.afail = 0;
+  }
+
+  // Whether `c` holds in any lane of the wavefront (the lane itself on the
+  // host, where runs go one at a time).
+  __host__ __device__ __forceinline__ static bool any_lane(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ballot_w64(c) != 0;
+#else
+    return c;
+#endif
+  }
+
+  // Runs commands until the run ends (r.done) or `budget` (>= 1) has been
+  // spent in this call: commands plus the tape reads of loop
+  // acceleration's orbit walks.  With RESUME a later call goes on from the
+  // next command; without, a run stopped by the budget is only reported
+  // (r.done false).
+  //
+  // The wavefront stays in the loop while any lane runs: a lane that has
+  // stopped parks on position 128, whose word is a comment with no next
+  // command, so its loads, store and updates change nothing.  The common
+  // path is straight-line code (selects, no branch, no lane leaving the
+  // loop: a per-lane loop exit measured ~1.6x slower per command); the
+  // rare cases are one branch, which a wavefront skips when none of its
+  // lanes needs it.
+  template <bool RESUME, class W, class L>
+  __host__ __device__ __forceinline__ static void advance(Run &r, const W &w, const L &lut,
+                                                          uint32_t stepcount, uint32_t budget) {
+    BffMasks m = r.m;
+    int pc = r.done ? kTape2 : r.pc, h0 = r.h0, h1 = r.h1;
+    int rpc = r.done ? kTape2 : r.pc;
+    uint32_t i = r.i, iters = r.iters, extra = r.extra;
+    int bsrc = r.bsrc, bq = r.bq, bnext = r.bnext, binc = r.binc;
+    int fsrc = r.fsrc, fq = r.fq, fnext = r.fnext, finc = r.finc;
+    uint32_t bacc = r.bacc ? 1u : 0u;
+    uint32_t bwrite = r.bwrite ? 1u : 0u;
+    uint32_t bcmds = r.bcmds;
+    int aq = r.aq, ah0 = r.ah0, ah1 = r.ah1;
+    uint32_t ai = r.ai, aiters = r.aiters, aw0 = r.aw0, aw1 = r.aw1, afail = r.afail;
+    // i >= stepcount and iters >= lim as sign bits (all stay below 2^31);
+    // the tape reads of orbit walks lower lim.
+    const uint32_t sc1 = stepcount - 1;
+    uint32_t lim1 = iters + fast_umin(budget, 1u << 30) - 1;
+    int next = rpc;
+    while (any_lane(pc < kTape2)) {
+      const uint32_t wp = w.word(pc);
+      const uint32_t w0 = w.word(h0);
+      const uint32_t w1 = w.word(h1);
+      keep_loads_together();
+      const uint32_t k = (wp >> 8) & 15;
+      const uint32_t k2 = 2 * k;
+      const uint32_t nx = (wp >> 16) & 0xff;
+      const bool c01 = k == kCopy01, c10 = k == kCopy10;
+      const uint32_t wsrc = c10 ? w1 : w0;  // the word whose byte is written
+      const uint32_t wold = c01 ? w1 : w0;  // the word written over
+      const int waddr = c01 ? h1 : h0;
+      const bool pm = (k - kPlus) < 2u;
+      const uint32_t vnew = ((wsrc & 0xff) + ((kTDelta >> k2) & 3) - 1) & 0xff;
+      // The code moves with a copied byte; an increment or decrement
+      // lowers a non-command byte's bound by one.
+      w.set_low(waddr, vnew | ((wsrc & 0xff00) - (pm ? 0x100u : 0u)));
+      const uint32_t cold = (wold >> 12) & 7, csrc = (wsrc >> 12) & 7;
+      // An increment or decrement is rare when its byte is a command or
+      // may be one step from one (code < 2 or >= 16).
+      const bool wrare = (csrc != cold) | (pm & (((wold >> 8) & 0x7f) - 2 >= 14u));
+      const uint32_t v0 = w0 & 0xff, v1 = w1 & 0xff;
+      h0 = (h0 + (int)((kTH0 >> k2) & 3) - 1) & (kTape2 - 1);
+      h1 = (h1 + (int)((kTH1 >> k2) & 3) - 1) & (kTape2 - 1);
+      // t: 1 for a '[' over a null byte, 2 for a ']' over a non-null one.
+      const uint32_t t = k2 + (C::null(v0) ? 1u : 0u);
+      const bool tb = t == 2;
+      const bool taken = (t - 1) < 2u;
+      const int jsrc = tb ? bsrc : fsrc;
+      next = taken ? (tb ? bnext : fnext) : (int)nx;
+      uint32_t inc = taken ? (uint32_t)(tb ? binc : finc) : nx - (uint32_t)pc;
+      const uint32_t miss_b = taken ? (uint32_t)(pc ^ jsrc) : 0u;
+      const bool rare = wrare | (miss_b != 0) | (tb & (bacc != 0));
+      {
+        if (__builtin_expect(rare, 0)) {
+          if (wrare) {
+            // A write only (no jump).  Its code, then what the class change
+            // moves: masks, next-command bytes, matches, the cached jumps
+            // it falls in.
+            const uint32_t code = pm ? lut(vnew) : ((wsrc >> 8) & 0xff);
+            const uint32_t cnew = code >> 4;
+            if (pm) w.set_byte(waddr, 1, code);
+            if (cnew != cold) {
+              const bool was = cold != kClsNone, is = cnew != kClsNone;
+              m.cmd.flip(waddr, was != is);
+              m.open.flip(waddr, (cold == kClsOpen) != (cnew == kClsOpen));
+              m.close.flip(waddr, (cold == kClsClose) != (cnew == kClsClose));
+              m.write.flip(waddr, (cold == kClsWrite) != (cnew == kClsWrite));
+              m.move.flip(waddr, (cold == kClsMove) != (cnew == kClsMove));
+              if (was != is) {
+                const int prev = m.cmd.prev_upto(waddr - 1);
+                const int after = m.cmd.next_from(waddr + 1);
+                if (prev >= 0) w.set_byte(prev, 2, is ? (uint32_t)waddr : (uint32_t)after);
+                if (is) w.set_byte(waddr, 2, (uint32_t)after);
+              }
+              if (cold >= kClsOpen || cnew >= kClsOpen) {
+                // The brackets moved: every match may have changed.
+                match_all(w, m);
+                bsrc = -1;
+                fsrc = -1;
+              }
+              // A cached jump stands unless the byte lies in the body of
+              // the backward one (its classes and command count) or between
+              // the forward one's match and its next command.
+              if (waddr > bq && waddr < bsrc) bsrc = -1;
+              if (waddr > fq && waddr <= fnext) fsrc = -1;
+              if (bsrc < 0) aq = -1;
+              next = m.cmd.next_from(pc + 1);
+              inc = (uint32_t)(next - pc);
+            }
+          } else {
+            if (miss_b != 0) {
+              // The jump's target from the match byte: the next command
+              // after the match (none: past the tape), or for an unmatched
+              // bracket the end of the run after this step.
+              const uint32_t q = wp >> 24;
+              if (q == kNoMatch) {
+                next = kTape2;
+                inc = 1;
+              } else {
+                next = (int)((w.word(q) >> 16) & 0xff);
+                inc = (uint32_t)next - q;
+              }
+              if (tb) {
+                bsrc = pc;
+                bq = q == kNoMatch ? -1 : (int)q;
+                bnext = next;
+                binc = (int)inc;
+                bacc = 0;
+                if (q != kNoMatch) {
+                  const Mask128 body = mask_between((int)q, pc);
+                  const bool straight = !mask_any(m.open, body) && !mask_any(m.close, body);
+                  const bool bw = mask_any(m.write, body), bmove = mask_any(m.move, body);
+                  bwrite = bw ? 1u : 0u;
+                  bacc = straight && !(bw && bmove) ? 1u : 0u;
+                  const Mask128 cm{m.cmd.lo & body.lo, m.cmd.hi & body.hi};
+                  bcmds = (uint32_t)(__builtin_popcountll(cm.lo) + __builtin_popcountll(cm.hi)) + 1;
+                }
+                afail = 0;
+              } else {
+                fsrc = pc;
+                fq = q == kNoMatch ? kTape2 : (int)q;
+                fnext = next;
+                finc = (int)inc;
+              }
+            }
+            if (tb && bacc) {
+              const uint32_t ib = i + 1;  // steps after this `]`
+              if (pc == aq && iters + 1 == aiters + bcmds &&
+                  (!bwrite || (v0 == aw0 && v1 == aw1))) {
+                // One more iteration with the last one's state advanced by
+                // fixed deltas: skip whole iterations.
+                const uint32_t len = ib - ai;
+                uint32_t reps = (stepcount - ib) / len;
+                const int d0 = (h0 - ah0) & (kTape2 - 1);
+                const int d1 = (h1 - ah1) & (kTape2 - 1);
+                if (d0 != 0) {
+                  // First later iteration whose `]` sees a null byte (the
+                  // tape is constant: no write in the body), 8 at a time.
+                  const uint32_t lim = fast_umin(reps, (uint32_t)kTape2);
+                  uint32_t j = 1;
+                  bool found = false;
+                  for (; j <= lim; j += 8) {
+                    uint32_t hit = 0;
+#pragma unroll
+                    for (int u = 0; u < 8; u++)
+                      hit |= (uint32_t)C::null(w.word((h0 + (j + u) * d0) & (kTape2 - 1)) & 0xff)
+                             << u;
+                    if (lim - j < 7) hit &= (1u << (lim - j + 1)) - 1;
+                    if (hit) {
+                      j += __builtin_ctz(hit);
+                      found = true;
+                      break;
+                    }
+                  }
+                  if (found) reps = j - 1;
+                  lim1 -= found ? j : lim + 1;
+                }
+                i += reps * len;
+                extra += reps * bcmds;
+                h0 = (h0 + (int)(reps * d0)) & (kTape2 - 1);
+                h1 = (h1 + (int)(reps * d1)) & (kTape2 - 1);
+                aq = -1;
+              } else {
+                // A new anchor; a loop whose state changed over three
+                // iterations in a row is left alone.
+                afail = pc == aq ? afail + 1 : 0;
+                if (afail >= 3) bacc = 0;
+                aq = pc;
+                ah0 = h0;
+                ah1 = h1;
+                ai = ib;
+                aiters = iters + 1;
+                aw0 = v0;
+                aw1 = v1;
+              }
+            }
+          }
+        }
+      }
+      i += inc;
+      const bool act = pc < kTape2;
+      iters += act ? 1u : 0u;
+      rpc = act ? next : rpc;
+      const uint32_t stop = ((uint32_t)next >> 7) | ((sc1 - i) >> 31) | ((lim1 - iters) >> 31);
+      pc = (int)fast_umin((uint32_t)next + (stop << 8), (uint32_t)kTape2);
+    }
+    next = rpc;
+    r.m = m;
+    r.h0 = h0; r.h1 = h1; r.pc = next;
+    r.i = i; r.iters = iters; r.extra = extra;
+    r.done = next >= kTape2 || i >= stepcount;
+    r.bsrc = bsrc; r.bq = bq; r.bnext = bnext; r.binc = binc;
+    r.fsrc = fsrc; r.fq = fq; r.fnext = fnext; r.finc = finc;
+    r.bacc = bacc != 0; r.bwrite = bwrite != 0; r.bcmds = bcmds;
+    r.aq = aq; r.ah0 = ah0; r.ah1 = ah1;
+    r.ai = ai; r.aiters = aiters; r.aw0 = aw0; r.aw1 = aw1; r.afail = afail;
+  }
+
+  // Results of an ended run: ops (the reference's count of executed
+  // non-no-op steps), *steps and *work (commands executed).
+  __host__ __device__ __forceinline__ static uint32_t finish(const Run &r, uint32_t stepcount,
+                                                             uint32_t *steps, uint32_t *work) {
+    *steps = fast_umin(r.i, stepcount);
+    if (work) *work = r.iters;
+    return r.iters + r.extra;
+  }
+
+  // Evaluate (bff.inc.h:356-389) of the tape in `words`, budgeted: a run
+  // that would spend more than `budget` stops and reports *steps =
+  // kOverBudget.  The final tape is byte 0 of the words.
+  template <class W, class L>
+  __host__ __device__ static uint32_t evaluate(const W &w, const L &lut, const uint32_t *words,
+                                               uint32_t stepcount, uint32_t *steps,
+                                               uint32_t *work, uint32_t budget) {
+    Run r;
+    start_words(r, w, lut, words, stepcount);
+    advance<false>(r, w, lut, stepcount, budget);
+    if (!r.done) {
+      *steps = kOverBudget;
+      return 0;
+    }
+    return finish(r, stepcount, steps, work);
+  }
+};
+
+}  // namespace cubff
