@@ -29,7 +29,8 @@
 //     follow, and the cached jumps whose range the byte lies in are dropped;
 //   * a taken jump other than the cached backward and forward one: its
 //     target comes from the match byte (the scan of bff.inc.h:317-344
-//     done ahead of time) and is cached;
+//     done when the table is built; after a write that moves a bracket,
+//     the scan runs on 128-bit masks of '[' and ']') and is cached;
 //   * a taken backward jump of a loop whose body allows loop acceleration.
 //
 // Loop acceleration (exact, as in the budgeted interpreter of bff_fast.h):
@@ -81,6 +82,20 @@ struct LeanLds {
   }
 };
 
+// 32 words per lane in LDS (a lane's second tape copy), strided like
+// LeanLds.
+template <int LANES>
+struct LeanAnchorLds {
+  static constexpr int kStride = LANES + 1;
+  static constexpr int kRegionBytes = (kTape2 / 4) * kStride * 4;
+  lds_u32 *p;
+  __device__ __forceinline__ static LeanAnchorLds at(uint8_t *region, int lane) {
+    return LeanAnchorLds{(lds_u32 *)region + lane};
+  }
+  __device__ __forceinline__ uint32_t word(uint32_t j) const { return p[j * kStride]; }
+  __device__ __forceinline__ void set_word(uint32_t j, uint32_t v) const { p[j * kStride] = v; }
+};
+
 template <int MAP, bool HEADS>
 struct BffLean {
   using C = BffClass<MAP>;
@@ -122,11 +137,41 @@ struct BffLean {
     int bsrc, bq, bnext, binc, fsrc, fq, fnext, finc;
     bool bacc, bwrite;
     uint32_t bcmds;
+    // Whether the match bytes are the tape's (until a bracket is written).
+    uint32_t mvalid;
     // Loop anchor: its `]`, heads, steps and commands after it, the bytes
     // under the heads; failed comparisons in a row.
     int aq, ah0, ah1;
     uint32_t ai, aiters, aw0, aw1, afail;
+    // Cycle anchor (complete runs): its `]` (-1 none yet, -2 no more
+    // checks), heads, steps and ops after it, window, and the weighted sum
+    // of the tape changes since.
+    int cq, ch0, ch1;
+    uint32_t ci, cops, cwin, cdelta, cfirst;
   };
+
+  // The match of the bracket at pc from the masks: backward from a ']'
+  // (tb), forward from a '['; kNoMatch if there is none.
+  __host__ __device__ static uint32_t scan_match(const BffMasks &m, int pc, bool tb) {
+    int depth = 1, p = tb ? pc - 1 : pc + 1, q;
+    for (;;) {
+      if (tb) {
+        const int ql = m.open.prev_upto(p), qe = m.close.prev_upto(p);
+        q = fast_imax(ql, qe);
+        if (q < 0) break;
+        depth += (q == qe) ? 1 : -1;
+        p = q - 1;
+      } else {
+        const int ql = m.open.next_from(p), qe = m.close.next_from(p);
+        q = fast_imin(ql, qe);
+        if (q >= kTape2) break;
+        depth += (q == ql) ? 1 : -1;
+        p = q + 1;
+      }
+      if (depth == 0) return (uint32_t)q;
+    }
+    return kNoMatch;
+  }
 
   // Matches of every bracket (byte 3), from the masks.
   template <class W>
@@ -151,42 +196,54 @@ struct BffLean {
     }
   }
 
-  // Words, masks and matches of a tape whose 32 words (byte 4w + j is
-  // byte j of words[w]) are in registers, and the initial state.  lut(v)
-  // is code_of(v) from a table.
+  // Words, masks and matches of a tape whose raw bytes are in the words
+  // of positions 0..31 (bytes 4j..4j+3 of the tape are the four bytes of
+  // word j), and the initial state.  lut(v) is code_of(v) from a table.
+  // Groups of four positions are built from the top down: raw word j is
+  // read before positions 4j..4j+3 are written, and the raw words those
+  // positions held (4j..4j+3 > j for j > 0) were read earlier.
   template <class W, class L>
-  __host__ __device__ __forceinline__ static void start_words(Run &r, const W &w, const L &lut,
-                                                              const uint32_t *words,
-                                                              uint32_t stepcount) {
-    uint32_t mv[4] = {0, 0, 0, 0}, wr[4] = {0, 0, 0, 0}, op[4] = {0, 0, 0, 0},
-             cl[4] = {0, 0, 0, 0};
+  __host__ __device__ __forceinline__ static void start_raw(Run &r, const W &w, const L &lut,
+                                                            uint32_t stepcount) {
+    BffMasks m{};
     uint32_t next = kTape2;
+    uint32_t first_word = 0;
+#pragma unroll 2
+    for (int j = kTape2 / 4 - 1; j >= 0; j--) {
+      const uint32_t raw = w.word(j);
+      if (j == 0) first_word = raw;
+      uint32_t code[4];
 #pragma unroll
-    for (int p = kTape2 - 1; p >= 0; p--) {
-      const uint32_t v = (words[p >> 2] >> (8 * (p & 3))) & 0xff;
-      const uint32_t code = lut(v);
-      const uint32_t oh = 1u << (code >> 4);  // class as one bit
-      mv[p >> 5] |= ((oh >> kClsMove) & 1) << (p & 31);
-      wr[p >> 5] |= ((oh >> kClsWrite) & 1) << (p & 31);
-      op[p >> 5] |= ((oh >> kClsOpen) & 1) << (p & 31);
-      cl[p >> 5] |= ((oh >> kClsClose) & 1) << (p & 31);
-      w.set_word(p, v | (code << 8) | (next << 16) | (kNoMatch << 24));
-      next = (code >> 4) != kClsNone ? (uint32_t)p : next;
+      for (int b = 0; b < 4; b++) code[b] = lut((raw >> (8 * b)) & 0xff);
+      uint32_t nib[5] = {0, 0, 0, 0, 0};  // per class: the four positions' bits
+#pragma unroll
+      for (int b = 3; b >= 0; b--) {
+        const int p = 4 * j + b;
+        const uint32_t cls = code[b] >> 4;
+        w.set_word(p, ((raw >> (8 * b)) & 0xff) | (code[b] << 8) | (next << 16) | (kNoMatch << 24));
+        next = cls != kClsNone ? (uint32_t)p : next;
+        nib[kClsMove] |= (uint32_t)(cls == kClsMove) << b;
+        nib[kClsWrite] |= (uint32_t)(cls == kClsWrite) << b;
+        nib[kClsOpen] |= (uint32_t)(cls == kClsOpen) << b;
+        nib[kClsClose] |= (uint32_t)(cls == kClsClose) << b;
+      }
+      // Positions 4j..4j+3 enter each mask at the bottom.
+      const auto push = [](Mask128 &x, uint32_t n) {
+        x.hi = (x.hi << 4) | (x.lo >> 60);
+        x.lo = (x.lo << 4) | n;
+      };
+      push(m.move, nib[kClsMove]);
+      push(m.write, nib[kClsWrite]);
+      push(m.open, nib[kClsOpen]);
+      push(m.close, nib[kClsClose]);
     }
-    const auto m64 = [](const uint32_t *x) {
-      return Mask128{x[0] | ((uint64_t)x[1] << 32), x[2] | ((uint64_t)x[3] << 32)};
-    };
-    BffMasks &m = r.m;
-    m.move = m64(mv);
-    m.write = m64(wr);
-    m.open = m64(op);
-    m.close = m64(cl);
     m.cmd = Mask128{m.move.lo | m.write.lo | m.open.lo | m.close.lo,
                     m.move.hi | m.write.hi | m.open.hi | m.close.hi};
     // Position 128: where a lane that has stopped parks.
     w.set_word(kTape2, (kNoop << 8) | ((uint32_t)kTape2 << 16) | (kNoMatch << 24));
+    r.m = m;
     if ((m.open.lo | m.open.hi | m.close.lo | m.close.hi) != 0) match_all(w, m);
-    init(r, words[0] & 0xff, (words[0] >> 8) & 0xff, stepcount);
+    init(r, first_word & 0xff, (first_word >> 8) & 0xff, stepcount);
   }
 
   __host__ __device__ __forceinline__ static void init(Run &r, uint32_t b0, uint32_t b1,
@@ -211,8 +268,35 @@ struct BffLean {
     r.fsrc = -1; r.fq = kTape2; r.fnext = 0; r.finc = 0;
     r.bacc = r.bwrite = false;
     r.bcmds = 0;
+    r.mvalid = 1;
     r.aq = -1; r.ah0 = 0; r.ah1 = 0;
     r.ai = 0; r.aiters = 0; r.aw0 = 0; r.aw1 = 0; r.afail = 0;
+    r.cq = -1; r.ch0 = 0; r.ch1 = 0;
+    r.ci = 0; r.cops = 0; r.cwin = kCycleFirstWin; r.cdelta = 0; r.cfirst = 1;
+  }
+
+  // Full-state cycle skip of complete runs (exact, as in bff_fast.h): from
+  // step kCycleStart on, the state right after a taken backward jump is
+  // compared with an anchor whose tape is kept in a second per-lane region:
+  // pc, heads, a weighted sum of the tape changes since the anchor (zero
+  // when the tape is back to the anchor's; the weights only filter) and,
+  // when those match, every tape byte.  A match means the run repeats with
+  // that period: whole periods are counted without being run.  The first
+  // anchor stays for kCycleFirstWin steps, later ones follow Brent's scheme
+  // with windows kCycleWin, 2 kCycleWin, ...
+  static constexpr uint32_t kCycleStart = 128;
+  static constexpr uint32_t kCycleFirstWin = 4096;
+  static constexpr uint32_t kCycleWin = 1024;
+
+  // A second region of 32 words per lane: the anchor's tape.
+  struct NoAnchor {
+    __host__ __device__ uint32_t word(uint32_t) const { return 0; }
+    __host__ __device__ void set_word(uint32_t, uint32_t) const {}
+  };
+  template <class W>
+  __host__ __device__ __forceinline__ static uint32_t tape_word(const W &w, uint32_t j) {
+    return (w.word(4 * j) & 0xff) | ((w.word(4 * j + 1) & 0xff) << 8) |
+           ((w.word(4 * j + 2) & 0xff) << 16) | (w.word(4 * j + 3) << 24);
   }
 
   // Whether `c` holds in any lane of the wavefront (the lane itself on the
@@ -238,9 +322,10 @@ struct BffLean {
   // loop: a per-lane loop exit measured ~1.6x slower per command); the
   // rare cases are one branch, which a wavefront skips when none of its
   // lanes needs it.
-  template <bool RESUME, class W, class L>
+  template <bool RESUME, bool CYCLE = false, class W, class L, class A = NoAnchor>
   __host__ __device__ __forceinline__ static void advance(Run &r, const W &w, const L &lut,
-                                                          uint32_t stepcount, uint32_t budget) {
+                                                          uint32_t stepcount, uint32_t budget,
+                                                          const A &anchor = A{}) {
     BffMasks m = r.m;
     int pc = r.done ? kTape2 : r.pc, h0 = r.h0, h1 = r.h1;
     int rpc = r.done ? kTape2 : r.pc;
@@ -249,9 +334,11 @@ struct BffLean {
     int fsrc = r.fsrc, fq = r.fq, fnext = r.fnext, finc = r.finc;
     uint32_t bacc = r.bacc ? 1u : 0u;
     uint32_t bwrite = r.bwrite ? 1u : 0u;
-    uint32_t bcmds = r.bcmds;
+    uint32_t bcmds = r.bcmds, mvalid = r.mvalid;
     int aq = r.aq, ah0 = r.ah0, ah1 = r.ah1;
     uint32_t ai = r.ai, aiters = r.aiters, aw0 = r.aw0, aw1 = r.aw1, afail = r.afail;
+    int cq = r.cq, ch0 = r.ch0, ch1 = r.ch1;
+    uint32_t ci = r.ci, cops = r.cops, cwin = r.cwin, cdelta = r.cdelta, cfirst = r.cfirst;
     // i >= stepcount and iters >= lim as sign bits (all stay below 2^31);
     // the tape reads of orbit walks lower lim.
     const uint32_t sc1 = stepcount - 1;
@@ -274,6 +361,12 @@ struct BffLean {
       // The code moves with a copied byte; an increment or decrement
       // lowers a non-command byte's bound by one.
       w.set_low(waddr, vnew | ((wsrc & 0xff00) - (pm ? 0x100u : 0u)));
+      if constexpr (CYCLE) {
+        // Weight (2p + 1) * 16411 < 2^23 of the written position p: the
+        // product with a byte change stays a 24-bit multiply.
+        const int d = (int)vnew - (int)(wold & 0xff);
+        cdelta += (uint32_t)(d * (int)((2u * (uint32_t)waddr + 1u) * 16411u));
+      }
       const uint32_t cold = (wold >> 12) & 7, csrc = (wsrc >> 12) & 7;
       // An increment or decrement is rare when its byte is a command or
       // may be one step from one (code < 2 or >= 16).
@@ -289,7 +382,14 @@ struct BffLean {
       next = taken ? (tb ? bnext : fnext) : (int)nx;
       uint32_t inc = taken ? (uint32_t)(tb ? binc : finc) : nx - (uint32_t)pc;
       const uint32_t miss_b = taken ? (uint32_t)(pc ^ jsrc) : 0u;
-      const bool rare = wrare | (miss_b != 0) | (tb & (bacc != 0));
+      bool rare = wrare | (miss_b != 0) | (tb & (bacc != 0));
+      if constexpr (CYCLE) {
+        // A cycle check is due after a taken backward jump once the anchor
+        // state may have recurred or its window has passed.
+        const bool due = ((pc == cq) & (h0 == ch0) & (h1 == ch1) & (cdelta == 0)) |
+                         ((cq == -1) | (i + 1 - ci >= cwin));
+        rare = rare | (tb & (i + 1 >= kCycleStart) & (cq != -2) & due);
+      }
       {
         if (__builtin_expect(rare, 0)) {
           if (wrare) {
@@ -313,8 +413,9 @@ struct BffLean {
                 if (is) w.set_byte(waddr, 2, (uint32_t)after);
               }
               if (cold >= kClsOpen || cnew >= kClsOpen) {
-                // The brackets moved: every match may have changed.
-                match_all(w, m);
+                // The brackets moved: every match may have changed.  From
+                // here on jumps find their match in the masks.
+                mvalid = 0;
                 bsrc = -1;
                 fsrc = -1;
               }
@@ -329,10 +430,12 @@ struct BffLean {
             }
           } else {
             if (miss_b != 0) {
-              // The jump's target from the match byte: the next command
-              // after the match (none: past the tape), or for an unmatched
-              // bracket the end of the run after this step.
-              const uint32_t q = wp >> 24;
+              // The jump's target: the next command after its match (none:
+              // past the tape), or for an unmatched bracket the end of the
+              // run after this step.  The match is the match byte while
+              // that is valid, else the bracket scan of bff.inc.h:317-344
+              // done on the masks.
+              const uint32_t q = mvalid ? (wp >> 24) : scan_match(m, pc, tb);
               if (q == kNoMatch) {
                 next = kTape2;
                 inc = 1;
@@ -363,6 +466,7 @@ struct BffLean {
                 finc = (int)inc;
               }
             }
+            bool skipped = false;
             if (tb && bacc) {
               const uint32_t ib = i + 1;  // steps after this `]`
               if (pc == aq && iters + 1 == aiters + bcmds &&
@@ -400,6 +504,7 @@ struct BffLean {
                 h0 = (h0 + (int)(reps * d0)) & (kTape2 - 1);
                 h1 = (h1 + (int)(reps * d1)) & (kTape2 - 1);
                 aq = -1;
+                skipped = true;
               } else {
                 // A new anchor; a loop whose state changed over three
                 // iterations in a row is left alone.
@@ -412,6 +517,34 @@ struct BffLean {
                 aiters = iters + 1;
                 aw0 = v0;
                 aw1 = v1;
+              }
+            }
+            if constexpr (CYCLE) {
+              const uint32_t ib = i + 1;  // steps after this `]`
+              if (tb && !skipped && next < kTape2 && ib >= kCycleStart && cq != -2) {
+                bool same = pc == cq && h0 == ch0 && h1 == ch1 && cdelta == 0;
+                for (uint32_t j = 0; same && j < kTape2 / 4; j++)
+                  same = tape_word(w, j) == anchor.word(j);
+                if (same) {
+                  const uint32_t len = ib - ci;
+                  const uint32_t reps = (stepcount - ib) / len;
+                  i += reps * len;
+                  extra += reps * (iters + 1 + extra - cops);
+                  cq = -2;  // fewer than one period of steps is left
+                  aq = -1;  // the loop anchor's counts predate the skip
+                } else if (cq < 0 || ib - ci >= cwin) {
+                  if (cq >= 0) {
+                    cwin = cfirst ? kCycleWin : (cwin < (1u << 29) ? cwin * 2 : cwin);
+                    cfirst = 0;
+                  }
+                  for (uint32_t j = 0; j < kTape2 / 4; j++) anchor.set_word(j, tape_word(w, j));
+                  cq = pc;
+                  ch0 = h0;
+                  ch1 = h1;
+                  ci = ib;
+                  cops = iters + 1 + extra;
+                  cdelta = 0;
+                }
               }
             }
           }
@@ -431,9 +564,11 @@ struct BffLean {
     r.done = next >= kTape2 || i >= stepcount;
     r.bsrc = bsrc; r.bq = bq; r.bnext = bnext; r.binc = binc;
     r.fsrc = fsrc; r.fq = fq; r.fnext = fnext; r.finc = finc;
-    r.bacc = bacc != 0; r.bwrite = bwrite != 0; r.bcmds = bcmds;
+    r.bacc = bacc != 0; r.bwrite = bwrite != 0; r.bcmds = bcmds; r.mvalid = mvalid;
     r.aq = aq; r.ah0 = ah0; r.ah1 = ah1;
     r.ai = ai; r.aiters = aiters; r.aw0 = aw0; r.aw1 = aw1; r.afail = afail;
+    r.cq = cq; r.ch0 = ch0; r.ch1 = ch1;
+    r.ci = ci; r.cops = cops; r.cwin = cwin; r.cdelta = cdelta; r.cfirst = cfirst;
   }
 
   // Results of an ended run: ops (the reference's count of executed
@@ -445,15 +580,27 @@ struct BffLean {
     return r.iters + r.extra;
   }
 
-  // Evaluate (bff.inc.h:356-389) of the tape in `words`, budgeted: a run
-  // that would spend more than `budget` stops and reports *steps =
-  // kOverBudget.  The final tape is byte 0 of the words.
-  template <class W, class L>
-  __host__ __device__ static uint32_t evaluate(const W &w, const L &lut, const uint32_t *words,
-                                               uint32_t stepcount, uint32_t *steps,
-                                               uint32_t *work, uint32_t budget) {
+  // A complete run (no budget) with the full-state cycle skip; anchor is a
+  // second region of 32 words per lane.
+  template <class W, class L, class A>
+  __host__ __device__ static uint32_t evaluate_full(const W &w, const A &anchor, const L &lut,
+                                                    uint32_t stepcount, uint32_t *steps,
+                                                    uint32_t *work) {
     Run r;
-    start_words(r, w, lut, words, stepcount);
+    start_raw(r, w, lut, stepcount);
+    advance<false, true>(r, w, lut, stepcount, 0xffffffffu, anchor);
+    return finish(r, stepcount, steps, work);
+  }
+
+  // Evaluate (bff.inc.h:356-389) of the tape whose raw bytes are in the
+  // words of positions 0..31 (see start_raw), budgeted: a run that would
+  // spend more than `budget` stops and reports *steps = kOverBudget.  The
+  // final tape is byte 0 of the words.
+  template <class W, class L>
+  __host__ __device__ static uint32_t evaluate(const W &w, const L &lut, uint32_t stepcount,
+                                               uint32_t *steps, uint32_t *work, uint32_t budget) {
+    Run r;
+    start_raw(r, w, lut, stepcount);
     advance<false>(r, w, lut, stepcount, budget);
     if (!r.done) {
       *steps = kOverBudget;
@@ -462,5 +609,106 @@ struct BffLean {
     return finish(r, stepcount, steps, work);
   }
 };
+
+}  // namespace cubff
+
+namespace cubff {
+
+// The languages BffLean runs (the standard jump logic), with their map.
+template <int LANG> struct LeanOf { static constexpr bool kHas = false; };
+template <> struct LeanOf<kBff> { static constexpr bool kHas = true; using T = BffLean<kMapAscii, true>; };
+template <> struct LeanOf<kBffNoheads> { static constexpr bool kHas = true; using T = BffLean<kMapAscii, false>; };
+template <> struct LeanOf<kBffNoheads4bit> { static constexpr bool kHas = true; using T = BffLean<kMap4bit, false>; };
+template <> struct LeanOf<kBffPerm> { static constexpr bool kHas = true; using T = BffLean<kMapPerm, true>; };
+
+#if defined(__HIPCC__)
+// A workgroup's code table (BffLean::code_of for every byte value); every
+// thread of the block calls it before a barrier.
+template <int LANG>
+__device__ __forceinline__ void fill_lean_lut(uint8_t *lut) {
+  if constexpr (LeanOf<LANG>::kHas)
+    for (uint32_t v = threadIdx.x; v < 256; v += blockDim.x)
+      lut[v] = (uint8_t)LeanOf<LANG>::T::code_of(v);
+}
+
+// A soup row into half h of a lane's raw tape (words 16h .. 16h + 15 of
+// positions 0..31, see BffLean::start_raw), with 16-byte loads, or with
+// 8-byte device-coherent loads.
+template <int LANES>
+__device__ __forceinline__ void load_raw_row(const LeanLds<LANES> &w, int half, const uint8_t *src) {
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const uint4 v = s4[q];
+    w.set_word(16 * half + 4 * q, v.x);
+    w.set_word(16 * half + 4 * q + 1, v.y);
+    w.set_word(16 * half + 4 * q + 2, v.z);
+    w.set_word(16 * half + 4 * q + 3, v.w);
+  }
+}
+template <int LANES>
+__device__ __forceinline__ void load_raw_row_coherent(const LeanLds<LANES> &w, int half,
+                                                      const uint8_t *src) {
+  uint64_t *s = reinterpret_cast<uint64_t *>(const_cast<uint8_t *>(src));
+#pragma unroll
+  for (int q = 0; q < kTape / 8; q++) {
+    const uint64_t v = __hip_atomic_load(s + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w.set_word(16 * half + 2 * q, (uint32_t)v);
+    w.set_word(16 * half + 2 * q + 1, (uint32_t)(v >> 32));
+  }
+}
+// Mutation of common_language.h:172-180 on bytes [b0, b1) of a raw tape:
+// byte b of slot k is replaced by r & 0xFF when ((r >> 8) & (2^30 - 1)) <
+// P, r = SM((N*S(e) + k)*128 + b).
+template <int LANES>
+__device__ __forceinline__ void mutate_raw(const LeanLds<LANES> &w, uint64_t n, uint64_t seed_value,
+                                           uint64_t k, uint32_t prob, int b0 = 0,
+                                           int b1 = kTape2) {
+  const uint64_t base = (n * seed_value + k) * (uint64_t)kTape2;
+  for (int b = b0; b < b1; b++) {
+    const uint64_t r = splitmix64(base + b);
+    if (((uint32_t)(r >> 8) & 0x3fffffffu) < prob) w.set_byte(b >> 2, b & 3, (uint32_t)r & 0xff);
+  }
+}
+// The raw tape as the final one, for an interaction that is not
+// evaluated: byte 0 of position p's word is tape byte p.
+template <int LANES>
+__device__ __forceinline__ void raw_as_final(const LeanLds<LANES> &w) {
+  uint32_t raw[kTape2 / 4];
+#pragma unroll
+  for (int j = 0; j < kTape2 / 4; j++) raw[j] = w.word(j);
+#pragma unroll
+  for (int p = 0; p < kTape2; p++) w.set_word(p, (raw[p >> 2] >> (8 * (p & 3))) & 0xff);
+}
+// Half h of a lane's final tape (byte 0 of its words) written to a soup row
+// with 16-byte stores, or with 8-byte device-coherent stores.
+template <int LANES>
+__device__ __forceinline__ void store_lean_row(const LeanLds<LANES> &w, int half, uint8_t *dst) {
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    uint32_t v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int p = kTape * half + 16 * q + 4 * j;
+      v[j] = (w.word(p) & 0xff) | ((w.word(p + 1) & 0xff) << 8) | ((w.word(p + 2) & 0xff) << 16) |
+             (w.word(p + 3) << 24);
+    }
+    d4[q] = make_uint4(v[0], v[1], v[2], v[3]);
+  }
+}
+template <int LANES>
+__device__ __forceinline__ void store_lean_row_coherent(const LeanLds<LANES> &w, int half,
+                                                        uint8_t *dst) {
+  uint64_t *d = reinterpret_cast<uint64_t *>(dst);
+#pragma unroll
+  for (int q = 0; q < kTape / 8; q++) {
+    uint64_t v = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v |= (uint64_t)(w.word(kTape * half + 8 * q + j) & 0xff) << (8 * j);
+    __hip_atomic_store(d + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+#endif
 
 }  // namespace cubff

@@ -53,6 +53,7 @@
 #include "device_common.h"
 #include "kernels.h"
 #include "bff_fast.h"
+#include "bff_lean.h"
 #include "languages.h"
 #include "tape_io.h"
 
@@ -295,6 +296,155 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer(
     // entry takes over); one already done counts as satisfied.  Then drop
     // the creation count with the satisfied ones: at zero, the entry is
     // runnable.
+    uint32_t sat = 0;
+    if (d1 != kNone) {
+      DeferItem *d = a.items + d1;
+      uint32_t *slot = ld_agent(&d->p1) == p1 ? &d->succ1 : &d->succ2;
+      sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             kLinkDone;
+    }
+    if (d2 != kNone) {
+      DeferItem *d = a.items + d2;
+      uint32_t *slot = ld_agent(&d->p1) == p2 ? &d->succ1 : &d->succ2;
+      sat += __hip_atomic_exchange(slot, at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+             kLinkDone;
+    }
+    if (__hip_atomic_fetch_sub(&it->pending, sat + 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == sat + 1u)
+      push_ready(a, at);
+  }
+  const uint64_t wsum = wave_sum_u64(ops);
+  const uint64_t wsteps = wave_sum_u64(steps);
+  if ((lane & 63) == 0 && wsum) atomicAdd(&block_ops, (unsigned long long)wsum);
+  if ((lane & 63) == 0 && wsteps) atomicAdd(&block_steps, (unsigned long long)wsteps);
+  __syncthreads();
+  if (lane == 0 && block_ops) atomicAdd(&a.counters[0], block_ops);
+  if (lane == 0 && block_steps) atomicAdd(&a.counters[2], block_steps);
+  if (blockIdx.x == 0 && lane == 0) atomicAdd(&a.counters[1], (unsigned long long)*num_indices_p);
+}
+
+// k_epoch_defer for the languages BffLean runs: the two programs are
+// gathered and mutated in registers, the word table is built from them in
+// LDS, and the budgeted run and the write-back go through it.  The
+// deferral protocol is k_epoch_defer's.
+template <int LANG>
+__global__ __launch_bounds__(kEpochLanes) void k_epoch_defer_lean(
+    DeferArgs a, const uint32_t *__restrict__ shuf, const uint64_t *__restrict__ num_indices_p,
+    uint32_t e_rel) {
+  using F = typename LeanOf<LANG>::T;
+  using W = LeanLds<kEpochLanes>;
+  __shared__ __align__(16) uint8_t lds[W::kRegionBytes];
+  __shared__ uint8_t lut[256];
+  __shared__ unsigned long long block_ops, block_steps;
+  __shared__ uint32_t block_budget, block_wait;
+  const int lane = threadIdx.x;
+  fill_lean_lut<LANG>(lut);
+  if (lane == 0) {
+    block_ops = 0;
+    block_steps = 0;
+    // While the resolver is behind, wait for owing entries for as long as
+    // it takes instead of deferring behind them: no chain grows, and only
+    // interactions past the budget, which depend on nothing listed, are
+    // added (the resolver runs those side by side).  Complete runs here
+    // would go without the cycle skip, which the resolver's have.  Without
+    // a resolver (abort raised) nothing is given up.
+    const uint32_t backlog = ld_agent(&a.ctl->tail) - ld_agent(&a.ctl->resolved);
+    const bool abort = ld_agent(&a.ctl->abort) != 0;
+    block_budget = abort ? 0xffffffffu : a.budget;
+    block_wait = abort || backlog > a.max_backlog;
+  }
+  __syncthreads();
+  const uint32_t budget = block_budget;
+  const uint64_t k = (uint64_t)blockIdx.x * kEpochLanes + lane;
+  const W w = W::at(lds, lane);
+  const LdsByteLut blut{(const lds_u8 *)lut};
+  uint32_t ops = 0, steps = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone;
+  const bool eval = k < *num_indices_p;  // common_language.h:185
+  if (k < a.slots) {
+    p1 = shuf[2 * k];
+    p2 = shuf[2 * k + 1];
+    o1 = ld_agent(a.owner + p1);
+    o2 = ld_agent(a.owner + p2);
+    const bool wait = block_wait != 0;
+    const uint64_t deadline = wall_clock64() + a.owe_ticks;
+    if (o1 < kResolved &&
+        (ld_agent(&a.items[o1].done) == a.gen ||
+         (wait ? wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks)
+               : wait_eq_until(a.ctl, &a.items[o1].done, a.gen, deadline))))
+      o1 = kResolved;
+    if (o2 < kResolved &&
+        (ld_agent(&a.items[o2].done) == a.gen ||
+         (wait ? wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks)
+               : wait_eq_until(a.ctl, &a.items[o2].done, a.gen, deadline))))
+      o2 = kResolved;
+  }
+  after_flag();
+  bool defer = false;
+  const uint64_t seed_value = epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel);
+  if (k < a.slots) {
+    defer = o1 < kResolved || o2 < kResolved;  // a program is still owed
+    if (!defer) {
+      if (o1 == kResolved) load_raw_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
+      else load_raw_row(w, 0, a.soup + (uint64_t)p1 * kTape);
+      if (o2 == kResolved) load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
+      else load_raw_row(w, 1, a.soup + (uint64_t)p2 * kTape);
+      mutate_raw(w, a.n, seed_value, k, a.prob);
+      if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, budget);
+      else raw_as_final(w);  // not evaluated: the mutated tape goes back as it is
+      if (steps == kOverBudget) {
+        defer = true;
+        ops = steps = 0;
+      } else {
+        store_lean_row(w, 0, a.soup + (uint64_t)p1 * kTape);
+        store_lean_row(w, 1, a.soup + (uint64_t)p2 * kTape);
+        if (o1 != kNone) st_agent(a.owner + p1, kNone);
+        if (o2 != kNone) st_agent(a.owner + p2, kNone);
+        if (e_rel + 1 == a.epochs) {
+          a.pair_ops[k] = ops;
+          a.pair_steps[k] = steps;
+        }
+      }
+    }
+  }
+  const uint32_t at = wave_alloc_u32(&a.ctl->tail, defer ? 1u : 0u);
+  // The resolver left before anything was deferred: run the interaction
+  // here to the end, from the soup's unchanged bytes.
+  const bool gone = defer && at >= kDeferGone;
+  if (gone) {
+    load_raw_row(w, 0, a.soup + (uint64_t)p1 * kTape);
+    load_raw_row(w, 1, a.soup + (uint64_t)p2 * kTape);
+    mutate_raw(w, a.n, seed_value, k, a.prob);
+    if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, 0xffffffffu);
+    else raw_as_final(w);
+    store_lean_row(w, 0, a.soup + (uint64_t)p1 * kTape);
+    store_lean_row(w, 1, a.soup + (uint64_t)p2 * kTape);
+    if (e_rel + 1 == a.epochs) {
+      a.pair_ops[k] = ops;
+      a.pair_steps[k] = steps;
+    }
+    defer = false;
+  }
+  const bool put = defer && at < a.capacity;
+  if (defer && !put) st_agent(&a.ctl->abort, 1);
+  const uint32_t d1 = o1 < kResolved ? o1 : kNone;
+  const uint32_t d2 = (o2 < kResolved && p2 != p1) ? o2 : kNone;
+  DeferItem *it = a.items + (put ? at : 0u);
+  if (put) {
+    st_agent(&it->e, e_rel | (eval ? 0u : kNoEval));
+    st_agent(&it->k, (uint32_t)k);
+    st_agent(&it->p1, p1);
+    st_agent(&it->p2, p2);
+    st_agent(&it->dep1, d1);
+    st_agent(&it->dep2, d2);
+    st_agent(&it->succ1, kNone);
+    st_agent(&it->succ2, kNone);
+    st_agent(&it->pending, (uint32_t)(d1 != kNone) + (uint32_t)(d2 != kNone) + 1u);
+  }
+  if (__ballot(put)) publish_point();
+  if (put) {
+    st_agent(&it->ready, a.gen);
+    st_agent(a.owner + p1, at);
+    st_agent(a.owner + p2, at);
     uint32_t sat = 0;
     if (d1 != kNone) {
       DeferItem *d = a.items + d1;
@@ -579,9 +729,15 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
   // anchor, each 16-byte aligned.
   constexpr int kSecond = (Tape::kRegionBytes + 15) / 16 * 16;
   constexpr int kThird = kSecond + (second_region_bytes<LANG, 1>() + 15) / 16 * 16;
-  __shared__ __align__(16) uint8_t lds[kThird + (kFast ? LdsTape<1>::kRegionBytes : 0)];
+  // The languages BffLean runs use its word table instead.
+  constexpr bool kLean = LeanOf<LANG>::kHas;
+  constexpr int kLeanAnchor = (LeanLds<1>::kRegionBytes + 15) / 16 * 16;
+  constexpr int kBytes = kLean ? kLeanAnchor + LeanAnchorLds<1>::kRegionBytes
+                               : kThird + (kFast ? LdsTape<1>::kRegionBytes : 0);
+  __shared__ __align__(16) uint8_t lds[kBytes];
   __shared__ uint8_t lut[kFast ? 256 : 1];
-  fill_fast_lut<LANG>(lut);
+  if constexpr (kLean) fill_lean_lut<LANG>(lut);
+  else fill_fast_lut<LANG>(lut);
   __syncthreads();
   if (threadIdx.x != 0) return;
   // The resolver's one-lane wavefronts share SIMDs with the epoch kernel's
@@ -657,28 +813,48 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     const uint32_t e = ew & ~kNoEval, k = ld_agent(&it->k), p1 = ld_agent(&it->p1),
                    p2 = ld_agent(&it->p2);
     const bool eval = !(ew & kNoEval);
-    load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
-    load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
-    uint32_t s = 0;
-    if constexpr (kBatch) {
-      const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
-      s = q.s;
-      mutate(t, q.n, q.seed_value, q.kl, q.prob);
-    } else {
-      mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
-    }
-    uint32_t steps = 0, ops = 0;
-    if (eval) {
-      if constexpr (kFast)
-        ops = FastOf<LANG>::T::evaluate_full(t, LinTape<1>::at(lds + kSecond, 0),
-                                             LdsTape<1>::at(lds + kThird, 0),
+    uint32_t s = 0, steps = 0, ops = 0;
+    if constexpr (kLean) {
+      const auto w = LeanLds<1>::at(lds, 0);
+      load_raw_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
+      load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
+      if constexpr (kBatch) {
+        const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
+        s = q.s;
+        mutate_raw(w, q.n, q.seed_value, q.kl, q.prob);
+      } else {
+        mutate_raw(w, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+      }
+      if (eval)
+        ops = LeanOf<LANG>::T::evaluate_full(w, LeanAnchorLds<1>::at(lds + kLeanAnchor, 0),
                                              LdsByteLut{(const lds_u8 *)lut}, kEvalSteps, &steps,
                                              nullptr);
       else
-        ops = L::evaluate(t, aux, kEvalSteps, &steps);
+        raw_as_final(w);
+      store_lean_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
+      store_lean_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
+    } else {
+      load_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+      load_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
+      if constexpr (kBatch) {
+        const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
+        s = q.s;
+        mutate(t, q.n, q.seed_value, q.kl, q.prob);
+      } else {
+        mutate(t, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+      }
+      if (eval) {
+        if constexpr (kFast)
+          ops = FastOf<LANG>::T::evaluate_full(t, LinTape<1>::at(lds + kSecond, 0),
+                                               LdsTape<1>::at(lds + kThird, 0),
+                                               LdsByteLut{(const lds_u8 *)lut}, kEvalSteps,
+                                               &steps, nullptr);
+        else
+          ops = L::evaluate(t, aux, kEvalSteps, &steps);
+      }
+      store_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+      store_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
     }
-    store_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
-    store_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
     unsigned long long *ctr = a.counters + (kBatch ? 4 * (uint64_t)s : 0);
     if (ops) atomicAdd(&ctr[0], (unsigned long long)ops);
     if (steps) atomicAdd(&ctr[2], (unsigned long long)steps);
@@ -735,8 +911,12 @@ inline unsigned blocks_of(uint64_t n, unsigned b) { return (unsigned)((n + b - 1
 template <int L>
 hipError_t epoch_defer_lang(hipStream_t st, const DeferArgs &a, const uint32_t *shuf,
                             const uint64_t *num_indices, uint32_t e_rel) {
-  hipLaunchKernelGGL(k_epoch_defer<L>, dim3(blocks_of(a.slots, kEpochLanes)),
-                     dim3(kEpochLanes), 0, st, a, shuf, num_indices, e_rel);
+  if constexpr (LeanOf<L>::kHas)
+    hipLaunchKernelGGL(k_epoch_defer_lean<L>, dim3(blocks_of(a.slots, kEpochLanes)),
+                       dim3(kEpochLanes), 0, st, a, shuf, num_indices, e_rel);
+  else
+    hipLaunchKernelGGL(k_epoch_defer<L>, dim3(blocks_of(a.slots, kEpochLanes)),
+                       dim3(kEpochLanes), 0, st, a, shuf, num_indices, e_rel);
   return hipGetLastError();
 }
 
