@@ -173,20 +173,36 @@ def batched_config():
     return ["bff_noheads"] * S, ns, list(range(S)), probs
 
 
+AUTOTUNE_ROUNDS = 3
+
+
+def batched_start(warmup):
+    """First timed epoch of the batched legs: after the warm-up and the
+    longest launch-shape sweep (rounds x 12 shapes x two-epoch runs), so
+    that every leg times the same epochs."""
+    return warmup + AUTOTUNE_ROUNDS * 12 * 2
+
+
 def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
     langs, ns, seeds, probs = spec if spec is not None else batched_config()
     with capi.Batch(langs, ns) as b:
         b.init(seeds, probs)
         tuned = None
         # Warm-up first (the learned schedule trains on its first epoch), then
-        # the launch-shape sweep on live epochs, then the timed epochs.
+        # the launch-shape sweep on live epochs, then untimed epochs up to the
+        # common first timed epoch, then the timed epochs.
         b.run_epochs(0, warmup, sched)
         b.sync()
         e0 = warmup
         if autotune:
-            wg, tpl, table = b.autotune(e0, 3, sched)
+            wg, tpl, table = b.autotune(e0, AUTOTUNE_ROUNDS, sched)
             tuned = {"wg": wg, "tapes_per_lane": tpl, "ms_per_epoch": table}
             e0 = b.next_epoch()
+        start = batched_start(warmup)
+        if e0 < start:
+            b.run_epochs(e0, start - e0, sched)
+            e0 = start
+        b.sync()
         b.reset_counters()
         tot, it = b.run_epochs_timed(e0, epochs, sched)
         ops = steps = 0
@@ -195,11 +211,19 @@ def run_batched(capi, sched, epochs, warmup, autotune, spec=None):
             ops += o
             steps += st
         wg, tpl = b.get_launch()
+        # The next epochs replayed from the captured epoch graph (interpreter
+        # rounds; a replayed epoch does not defer).
+        b.reset_counters()
+        gtot, _ = b.run_epochs_timed(e0 + epochs, epochs, sched, use_graph=True)
+        gops = sum(b.read_counters(i)[0] for i in range(len(ns)))
+        graph = {"first_epoch": e0 + epochs, "epochs_timed": epochs,
+                 "ms_per_epoch": gtot / epochs, "instr_per_s": gops / (gtot / 1e3)}
         out = {"instr_per_s": ops / (tot / 1e3), "ms_per_epoch": tot / epochs,
                "first_epoch": e0,
                "interp_ms_per_epoch": it / epochs, "epochs_timed": epochs,
                "total_programs": sum(ns), "soups": len(ns), "wg": wg,
-               "tapes_per_lane": tpl, "steps": steps, "ops": ops, "autotune": tuned}
+               "tapes_per_lane": tpl, "steps": steps, "ops": ops, "autotune": tuned,
+               "graph_replay": graph}
         if sched == capi.SCHED_LEARNED:
             tr, loss, buckets = b.sched_stats()
             out["sched"] = {"trainings": tr, "last_loss": loss,

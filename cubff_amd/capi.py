@@ -118,7 +118,7 @@ def load():
         "cubff_batch_set_launch": (_i, [_vp, _i, ctypes.c_uint32]),
         "cubff_batch_get_launch": (_i, [_vp, ctypes.POINTER(_i), _u32p]),
         "cubff_batch_run_epochs": (_i, [_vp, _u64, _u64, _i, _i]),
-        "cubff_batch_run_epochs_timed": (_i, [_vp, _u64, _u64, _i,
+        "cubff_batch_run_epochs_timed": (_i, [_vp, _u64, _u64, _i, _i,
                                               ctypes.POINTER(ctypes.c_float),
                                               ctypes.POINTER(ctypes.c_float)]),
         "cubff_batch_autotune": (_i, [_vp, _u64, _u64, _i, ctypes.POINTER(_i), _u32p,
@@ -449,10 +449,14 @@ class Batch:
         _bcheck(load().cubff_batch_run_epochs(self._h, first_epoch, count, sched,
                                               int(use_graph)))
 
-    def run_epochs_timed(self, first_epoch: int, count: int, sched: int = SCHED_PLAIN):
+    def run_epochs_timed(self, first_epoch: int, count: int, sched: int = SCHED_PLAIN,
+                         use_graph: bool = False):
+        """(total ms, interpreter ms) of `count` epochs; with use_graph the
+        captured epoch is replayed and the interpreter time is -1."""
         a, b = ctypes.c_float(0), ctypes.c_float(0)
         _bcheck(load().cubff_batch_run_epochs_timed(self._h, first_epoch, count, sched,
-                                                    ctypes.byref(a), ctypes.byref(b)))
+                                                    int(use_graph), ctypes.byref(a),
+                                                    ctypes.byref(b)))
         return float(a.value), float(b.value)
 
     def autotune(self, first_epoch: int, epochs_per_config: int, sched: int = SCHED_PLAIN):

@@ -266,10 +266,12 @@ int cubff_batch_get_launch(cubff_batch *b, int *wg, uint32_t *tapes_per_lane);
 int cubff_batch_run_epochs(cubff_batch *b, uint64_t first_epoch, uint64_t count,
                            int sched, int use_graph);
 /* As above with HIP events: device time of the whole run and of the
- * interpreter launches alone.  Blocks until done. */
+ * interpreter launches alone (-1 with use_graph: the replayed epoch is one
+ * graph launch; its capture happens before the timed region).  Blocks until
+ * done. */
 int cubff_batch_run_epochs_timed(cubff_batch *b, uint64_t first_epoch,
-                                 uint64_t count, int sched, float *total_ms,
-                                 float *interp_ms);
+                                 uint64_t count, int sched, int use_graph,
+                                 float *total_ms, float *interp_ms);
 /* Runs epochs_per_config live epochs under each of the 12 launch shapes
  * (workgroup 64/128/256 x 1-4 tapes per lane), in rounds of one epoch per
  * shape (two when runs go through deferring segments, which then run
