@@ -9,6 +9,7 @@
  */
 #include "cubff_oracle.h"
 
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -531,6 +532,33 @@ uint64_t orc_pairing_2d(uint32_t *s, uint64_t n, const uint64_t *row_off,
 }
 
 /* MutateAndRunPrograms: common_language.h:157-193. */
+/* Programs that sit in two or more different slots of one epoch's pairing
+ * s[0 .. 2*(n/2)).  Slots of one epoch run in parallel and write their
+ * programs back, so such a program is written by several slots at once
+ * (common_language.h:165-192 relies on s being a permutation; a 2-D pattern
+ * line "a a" puts one program in both halves of a single slot, which is no
+ * race: p2's half wins).  Debug builds (ORC_DEBUG_CHECKS) check every epoch
+ * with it. */
+uint64_t orc_pairing_repeats(const uint32_t *s, uint64_t n) {
+  uint64_t slots = n / 2, repeats = 0;
+  uint64_t *last = (uint64_t *)malloc(n * sizeof(uint64_t));
+  if (!last) return 0;
+  for (uint64_t p = 0; p < n; p++) last[p] = UINT64_MAX;
+  for (uint64_t k = 0; k < slots; k++) {
+    for (int h = 0; h < 2; h++) {
+      const uint32_t p = s[2 * k + h];
+      if (p >= n) continue;
+      if (last[p] != UINT64_MAX && last[p] != k) repeats++;
+      last[p] = k;
+    }
+  }
+  free(last);
+  return repeats;
+}
+
+static uint64_t g_debug_repeats = 0;
+uint64_t orc_debug_repeats_seen(void) { return g_debug_repeats; }
+
 uint64_t orc_mutate_and_run(int lang, uint8_t *soup, const uint32_t *s,
                             uint64_t n, uint64_t num_indices,
                             uint64_t seed_value, uint32_t mutation_prob,
@@ -538,6 +566,16 @@ uint64_t orc_mutate_and_run(int lang, uint8_t *soup, const uint32_t *s,
                             int nthreads) {
   uint64_t total = 0;
   long long slots = (long long)(n / 2);
+#ifdef ORC_DEBUG_CHECKS
+  {
+    const uint64_t rep = orc_pairing_repeats(s, n);
+    if (rep) {
+      g_debug_repeats += rep;
+      fprintf(stderr, "orc debug: the epoch's pairing puts %llu programs in more than one slot "
+                      "(their slots race on the write back)\n", (unsigned long long)rep);
+    }
+  }
+#endif
 #ifdef _OPENMP
   if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : total) \

@@ -8,9 +8,9 @@
  * implementation.  Nothing in the product (cubff_amd/, include/) may link,
  * import or call it.
  *
- * Every function cites the reference file:line it restates.  How each one is
- * pinned (golden vectors, the compiled reference in oracle/_ref, closed forms)
- * is listed in oracle/README.md.
+ * Every function cites the reference file:line it restates.  Each one is
+ * pinned by golden vectors recorded from the compiled reference in
+ * oracle/_ref, closed forms or invariants (tests/test_cpu_oracle.py).
  */
 #ifndef CUBFF_ORACLE_H_
 #define CUBFF_ORACLE_H_
@@ -74,6 +74,12 @@ uint64_t orc_mutate_and_run(int lang, uint8_t *soup, const uint32_t *s,
                             uint64_t seed_value, uint32_t mutation_prob,
                             uint32_t *pair_ops, uint32_t *pair_steps,
                             int nthreads);
+
+/* Programs in two or more different slots of an epoch's pairing (0 for a
+ * permutation); debug builds (-DORC_DEBUG_CHECKS) check every epoch of
+ * orc_mutate_and_run with it and count what they find. */
+uint64_t orc_pairing_repeats(const uint32_t *s, uint64_t num_programs);
+uint64_t orc_debug_repeats_seen(void);
 
 /* Self-replication score for programs [0, count). */
 void orc_check_selfrep(int lang, const uint8_t *soup, uint64_t num_programs,

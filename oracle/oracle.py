@@ -15,7 +15,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcubff_oracle.so")
+# CUBFF_ORACLE_LIB names another build of the same code, such as the
+# AddressSanitizer / UBSan build with the debug checks (make -C oracle san;
+# the process then needs the sanitizer runtimes preloaded).
+LIB_PATH = os.environ.get("CUBFF_ORACLE_LIB") or os.path.join(HERE, "libcubff_oracle.so")
 
 LANGS = [
     "bff", "bff_noheads", "bff8", "bff8_noheads", "bff_noheads_4bit",
@@ -63,6 +66,10 @@ def lib():
         L.orc_mutate_and_run.restype = u64
         L.orc_mutate_and_run.argtypes = [i32, _u8p, _u32p, u64, u64, u64, u32,
                                          _u32p, _u32p, i32]
+        L.orc_pairing_repeats.restype = u64
+        L.orc_pairing_repeats.argtypes = [_u32p, u64]
+        L.orc_debug_repeats_seen.restype = u64
+        L.orc_debug_repeats_seen.argtypes = []
         L.orc_check_selfrep.restype = None
         L.orc_check_selfrep.argtypes = [i32, _u8p, u64, u64, u64, _u64p, i32]
         _lib = L
@@ -156,6 +163,17 @@ def mutate_and_run(lang: str, soup: np.ndarray, s: np.ndarray, num_indices: int,
                                    n, num_indices, seed_value, mutation_prob,
                                    _p(po, _u32p), _p(ps, _u32p), nthreads)
     return int(tot), po, ps
+
+
+def pairing_repeats(s: np.ndarray, n: int) -> int:
+    """Programs that sit in two or more different slots of a pairing."""
+    sa = np.ascontiguousarray(s, dtype=np.uint32)
+    return int(lib().orc_pairing_repeats(_p(sa, _u32p), n))
+
+
+def debug_repeats_seen() -> int:
+    """Repeats the debug build's per-epoch check has counted (0 otherwise)."""
+    return int(lib().orc_debug_repeats_seen())
 
 
 def check_selfrep(lang: str, soup: np.ndarray, seed_value: int,

@@ -17,7 +17,9 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-BIN = os.path.join(ROOT, "cubff_amd", "bin")
+# CUBFF_HOST_BIN names another build of the host layer (the sanitizer build
+# of tests/test_cpu_sanitize.py).
+BIN = os.environ.get("CUBFF_HOST_BIN") or os.path.join(ROOT, "cubff_amd", "bin")
 MAIN = os.path.join(BIN, "main")
 RESET = "\x1b[0;38:5:35;48:5:232m"
 
