@@ -144,11 +144,11 @@ def cpu_baseline(lang: str, n: int, seed: int, prob: int, first: int, count: int
             "sample": f"oracle OpenMP port, epochs {first}..{first + e - 1} of the same workload"}
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_defer_bff_noheads_2p17.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_k_epoch_defer_lean_bff_noheads_2p17.md")
 
 
 def traffic_per_launch():
-    """HBM bytes per k_epoch launch from the committed PMC summary (FETCH_SIZE
+    """HBM bytes per epoch-kernel launch from the committed PMC summary (FETCH_SIZE
     + WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
     plain copy of known size), or None when it has not been measured."""
     try:
@@ -364,6 +364,41 @@ def run_large_soup(capi, epochs):
         "max_selfrep_score": max(best) if best else None}
 
 
+PROBE = os.path.join(ROOT, "cubff_amd", "lib", "interp_probe")
+
+
+def epoch_tapes(capi, lang, n, seed, prob, epoch):
+    """The 128-byte tapes of epoch `epoch`'s pairs (before that epoch's
+    mutation), the soup run there on the GPU."""
+    import numpy as np
+    with capi.Soup(lang, n) as g:
+        g.init_programs(capi.epoch_seed(seed, 0))
+        g.run_epochs(capi.PAIR_SHUFFLE, seed, 0, epoch, prob)
+        soup = g.read_soup().reshape(n, 64)
+        g.run_epochs(capi.PAIR_SHUFFLE, seed, epoch, 1, prob)
+        s, _ = g.read_shuffle()
+    return np.ascontiguousarray(np.concatenate([soup[s[0::2]], soup[s[1::2]]], axis=1))
+
+
+def interp_probe(capi, lang, n, seed, prob, epoch, budget=768):
+    """Per-command cost of the BFF interpreters on the pairs of a real epoch:
+    their tapes go to cubff_amd/lib/interp_probe, which times each
+    interpreter per wavefront with the shader clock."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(PROBE):
+        return None
+    tapes = epoch_tapes(capi, lang, n, seed, prob, epoch)
+    with tempfile.NamedTemporaryFile(suffix=".tapes") as f:
+        f.write(tapes.tobytes())
+        f.flush()
+        out = subprocess.run([PROBE, f.name, str(budget)], capture_output=True, text=True,
+                             timeout=120, check=True).stdout
+    r = json.loads(out)
+    r["workload"] = f"{lang}, {n} programs, seed {seed}: the {n // 2} pairs of epoch {epoch}"
+    return r
+
+
 def run_other_configs(capi, epochs, large_epochs):
     """The other benchmark configurations, each on one GPU."""
     from cubff_amd.patterns import grid_pattern
@@ -432,6 +467,12 @@ def main():
     ap.add_argument("--pipeline", choices=["auto", "loop", "dataflow", "defer"], default="auto",
                     help="multi-epoch path of the single-soup run (auto: the library's choice "
                          "for the language)")
+    ap.add_argument("--interp-probe", type=int, default=1,
+                    help="time the BFF interpreters per command on a real epoch's pairs "
+                         "(cubff_amd/lib/interp_probe; 0 = skip)")
+    ap.add_argument("--write-epoch-tapes", default="",
+                    help="only write the tapes of epoch 60's pairs of the single-soup workload "
+                         "to this file (input of cubff_amd/lib/interp_probe)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(four launches), the reference for HBM byte counters")
@@ -446,6 +487,11 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": nbytes, "launches": 4}))
+        return
+    if args.write_epoch_tapes:
+        from cubff_amd import capi
+        with open(args.write_epoch_tapes, "wb") as f:
+            f.write(epoch_tapes(capi, args.lang, args.num, args.seed, 1 << 18, 60).tobytes())
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the epoch runs on a single MI355X and "
@@ -491,6 +537,10 @@ def main():
     lane_ops = work / (basis_ms / 1e3)
     algo_bytes = (n // 2) * BYTES_PER_SLOT[path]
     achieved = algo_bytes / (basis_ms / 1e3 / K)
+    # The BFF languages with the standard jump logic run the word-table
+    # interpreter's epoch kernel; the others the byte-tape one.
+    defer_kernel = ("k_epoch_defer_lean" if lang in ("bff", "bff_noheads", "bff_noheads_4bit",
+                                                     "bff_perm") else "k_epoch_defer")
     line = {
         "metric": METRIC,
         "value": ops / dt,
@@ -511,7 +561,7 @@ def main():
             "lang": lang, "num_programs": n, "seed": seed,
             "parallelism": "single soup, one lane per pair slot",
             "path": {"loop": "one launch per epoch (k_epoch)",
-                     "defer": "k_epoch_defer per epoch (command budget 768) + resolver "
+                     "defer": f"{defer_kernel} per epoch (command budget 768) + resolver "
                               "kernel on a second stream",
                      "dataflow": "multi-epoch dataflow kernel (k_flow)"}[path],
         },
@@ -525,10 +575,13 @@ def main():
             "peak": PEAK_LANE_OPS / 1e12,
             "unit": "T int32 lane-ops/s",
             "frac": lane_ops / PEAK_LANE_OPS,
-            "traffic": traffic_per_launch(),
+            # Measured for the default workload only (bff_noheads, 2^17 programs,
+            # deferring epoch kernel).
+            "traffic": (traffic_per_launch() if (lang, n, path) == ("bff_noheads", 131072, "defer")
+                        else None),
             "kernel": {"loop": "k_epoch (gather + mutate + evaluate + scatter)",
-                       "defer": "k_epoch_defer (gather + mutate + evaluate up to the budget "
-                                "+ scatter)",
+                       "defer": f"{defer_kernel} (gather + mutate + evaluate up to the "
+                                "budget + scatter)",
                        "dataflow": "k_flow"}[path],
             "kernel_ms_avg": kern_ms / K,
             "time_basis": ("epoch span (epoch kernel + concurrent resolver + drain), "
@@ -544,6 +597,8 @@ def main():
         "deferred_per_epoch": deferred / K,
         "loop_one_launch_per_epoch": loop,
     }
+    if args.interp_probe and lang.startswith("bff"):
+        line["interp_probe"] = interp_probe(capi, lang, n, seed, prob, 60)
     if args.batched_epochs > 0:
         plain = run_batched(capi, capi.SCHED_PLAIN, args.batched_epochs, 8, autotune=True)
         learned = run_batched(capi, capi.SCHED_LEARNED, args.batched_epochs, 8, autotune=True)
