@@ -592,17 +592,67 @@ struct BffLean {
     return finish(r, stepcount, steps, work);
   }
 
+  // What a run stopped by its budget needs, besides its tape bytes, to go
+  // on from a fresh table: the next command's position and the heads
+  // (pos = pc | h0 << 8 | h1 << 16), steps, commands and accelerated ones.
+  struct Resume {
+    uint32_t pos, i, iters, extra;
+  };
+  __host__ __device__ __forceinline__ static Resume saved(const Run &r) {
+    return Resume{(uint32_t)r.pc | ((uint32_t)r.h0 << 8) | ((uint32_t)r.h1 << 16), r.i, r.iters,
+                  r.extra};
+  }
+  // A run that goes on from `s` over the tape whose raw bytes are in the
+  // words of positions 0..31: the table, masks and matches are built from
+  // the bytes (as start_raw does), the jump caches and accelerations start
+  // empty (they only save work), and the next command, heads and counts
+  // are the stopped run's.
+  template <class W, class L>
+  __host__ __device__ __forceinline__ static void start_resume(Run &r, const W &w, const L &lut,
+                                                               uint32_t stepcount,
+                                                               const Resume &s) {
+    start_raw(r, w, lut, stepcount);
+    resume_from(r, s, stepcount);
+  }
+  // The second half of start_resume, on a run start_raw began.
+  __host__ __device__ __forceinline__ static void resume_from(Run &r, const Resume &s,
+                                                              uint32_t stepcount) {
+    r.pc = (int)(s.pos & 0xff);
+    r.h0 = (int)((s.pos >> 8) & 0xff);
+    r.h1 = (int)((s.pos >> 16) & 0xff);
+    r.i = s.i;
+    r.iters = s.iters;
+    r.extra = s.extra;
+    r.done = !(r.i < stepcount && r.pc < kTape2);
+  }
+
+  // The complete run (with the full-state cycle skip) of a run stopped by
+  // its budget, from `s` and its tape's raw words.
+  template <class W, class L, class A>
+  __host__ __device__ static uint32_t evaluate_resumed(const W &w, const A &anchor, const L &lut,
+                                                       uint32_t stepcount, const Resume &s,
+                                                       uint32_t *steps, uint32_t *work) {
+    Run r;
+    start_resume(r, w, lut, stepcount, s);
+    advance<false, true>(r, w, lut, stepcount, 0xffffffffu, anchor);
+    return finish(r, stepcount, steps, work);
+  }
+
   // Evaluate (bff.inc.h:356-389) of the tape whose raw bytes are in the
   // words of positions 0..31 (see start_raw), budgeted: a run that would
   // spend more than `budget` stops and reports *steps = kOverBudget.  The
   // final tape is byte 0 of the words.
+  // With `stopped`, a run stopped by the budget also leaves there what
+  // start_resume needs to go on from it.
   template <class W, class L>
   __host__ __device__ static uint32_t evaluate(const W &w, const L &lut, uint32_t stepcount,
-                                               uint32_t *steps, uint32_t *work, uint32_t budget) {
+                                               uint32_t *steps, uint32_t *work, uint32_t budget,
+                                               Resume *stopped = nullptr) {
     Run r;
     start_raw(r, w, lut, stepcount);
     advance<false>(r, w, lut, stepcount, budget);
     if (!r.done) {
+      if (stopped) *stopped = saved(r);
       *steps = kOverBudget;
       return 0;
     }

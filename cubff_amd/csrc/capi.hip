@@ -49,6 +49,12 @@ struct cubff_ctx {
   // Pairing buffers drawn ahead of the current one, with their pair counts.
   uint32_t *shuf_ahead[2] = {nullptr, nullptr};
   uint64_t *nidx_ahead[2] = {nullptr, nullptr};
+  // Fisher-Yates pairings of several epochs drawn at once: scratch and
+  // results for fyb_epochs epochs, and a device scalar holding n / 2.
+  FyScratch fyb{};
+  uint32_t *shuf_many = nullptr;
+  uint64_t *nidx_half = nullptr;
+  uint32_t fyb_epochs = 0;
   unsigned long long *hist = nullptr;  // byte histogram, 256 entries
   uint32_t *owner = nullptr;
   DeferItem *ditems = nullptr;
@@ -125,6 +131,8 @@ int ensure_2d_scratch(cubff_ctx *c) {
 constexpr uint64_t kFlowMaxEpochs = 64;
 constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
+// Scratch and results of Fisher-Yates pairings drawn several epochs at once.
+constexpr uint64_t kFyBatchBytes = 256ULL << 20;
 constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
 constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
 // How long an epoch kernel's lane waits for the deferred interaction that
@@ -192,6 +200,41 @@ int ensure_flow_scratch(cubff_ctx *c, uint64_t epochs) {
   return CUBFF_OK;
 }
 
+// Epochs whose Fisher-Yates pairings are drawn in one set of launches: up
+// to 32, within kFyBatchBytes of scratch and results (32 B per program and
+// epoch).
+uint32_t fy_batch_epochs(uint64_t n) {
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(32, kFyBatchBytes / (32 * n + 4)));
+}
+
+void free_fy_batch(cubff_ctx *c) {
+  for (uint32_t *p : {c->fyb.j, c->fyb.cnt, c->fyb.base, c->fyb.cursor, c->fyb.items,
+                      c->fyb.first, c->fyb.succ, c->shuf_many})
+    dev_free(p);
+  dev_free(c->nidx_half);
+  c->fyb = FyScratch{};
+  c->shuf_many = nullptr;
+  c->nidx_half = nullptr;
+  c->fyb_epochs = 0;
+}
+
+int ensure_fy_batch(cubff_ctx *c, uint32_t epochs) {
+  if (c->fyb_epochs >= epochs) return CUBFF_OK;
+  free_fy_batch(c);
+  const uint64_t m = c->n * epochs;
+  int r;
+  if ((r = dev_alloc(&c->fyb.j, m)) || (r = dev_alloc(&c->fyb.cnt, m + epochs)) ||
+      (r = dev_alloc(&c->fyb.base, m)) || (r = dev_alloc(&c->fyb.cursor, m)) ||
+      (r = dev_alloc(&c->fyb.items, m)) || (r = dev_alloc(&c->fyb.first, m)) ||
+      (r = dev_alloc(&c->fyb.succ, m)) || (r = dev_alloc(&c->shuf_many, m)) ||
+      (r = dev_alloc(&c->nidx_half, 1))) {
+    free_fy_batch(c);
+    return r;
+  }
+  c->fyb_epochs = epochs;
+  return CUBFF_OK;
+}
+
 int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint32_t *dst,
               hipStream_t st = nullptr, uint64_t *num_indices = nullptr) {
   if (!st) st = c->stream;
@@ -216,14 +259,17 @@ int pair_into(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t epoch, uint
   }
 }
 
-// Pairings drawn two epochs ahead on a third stream into three buffers (the
-// context's shuf and shuf_ahead, with their pair counts): no pairing depends
-// on the soup, and beside an epoch kernel and the resolver the pairing
-// kernels run several times slower than alone, so one epoch of lead is not
-// always enough.  Epoch j's kernel waits for its pairing; the pairing of
-// epoch j+3 waits for epoch j's kernel, which reads the same buffer.  Epoch
-// j's kernel is queued before pairing j+2 is drawn, since the 2-D pairing
-// blocks the host while its rounds converge.
+// Pairings of a run of epochs, one of two ways.  Fisher-Yates pairings are
+// drawn `chunk` epochs at a time (one set of five launches per chunk) on the
+// epoch stream, before the first epoch of each chunk: drawn one epoch at a
+// time beside the epoch kernels instead, the pairing kernels (small and
+// latency-bound) slowed those kernels by more than they cost alone.  Other
+// modes are drawn two epochs ahead on a third stream into three buffers
+// (the context's shuf and shuf_ahead, with their pair counts): no pairing
+// depends on the soup.  Epoch j's kernel waits for its pairing; the pairing
+// of epoch j+3 waits for epoch j's kernel, which reads the same buffer.
+// Epoch j's kernel is queued before pairing j+2 is drawn, since the 2-D
+// pairing blocks the host while its rounds converge.
 struct PairAhead {
   static constexpr uint32_t kBufs = 3;  // pairings are drawn kBufs - 1 epochs ahead
   cubff_ctx *c;
@@ -232,9 +278,18 @@ struct PairAhead {
   uint32_t *buf[kBufs] = {};
   uint64_t *nidx[kBufs] = {};  // pairs evaluated, per buffer
   hipEvent_t drawn[kBufs] = {}, used[kBufs] = {};
+  uint32_t chunk = 0;  // > 0: Fisher-Yates pairings drawn this many epochs at a time
 
-  int init() {
+  int init(uint32_t epochs) {
     int r;
+    if (mode == CUBFF_PAIR_SHUFFLE) {
+      // The scratch is sized for the longest chunk at once: freeing it later
+      // would wait for every stream, the deferring runs' resolver included.
+      if ((r = ensure_fy_batch(c, fy_batch_epochs(c->n)))) return r;
+      chunk = std::min(fy_batch_epochs(c->n), std::max(epochs, 1u));
+      HIP_TRY(launch_set_u64(c->stream, c->nidx_half, c->n / 2));
+      return CUBFF_OK;
+    }
     if (!c->stream3) {
       hipError_t e = hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking);
       if (e != hipSuccess)
@@ -259,6 +314,11 @@ struct PairAhead {
     HIP_TRY(hipStreamWaitEvent(c->stream3, used[0], 0));
     return CUBFF_OK;
   }
+  // The pairing epoch first + j reads, and its pair count.
+  uint32_t *shuf(uint32_t j) const {
+    return chunk ? c->shuf_many + (uint64_t)(j % chunk) * c->n : buf[j % kBufs];
+  }
+  uint64_t *count(uint32_t j) const { return chunk ? c->nidx_half : nidx[j % kBufs]; }
   // Pairing of epoch first + j into buffer j % kBufs, after the kernel that
   // last read that buffer (epoch j - kBufs).
   int draw(uint32_t j) {
@@ -273,27 +333,40 @@ struct PairAhead {
   // Draws of the first epochs, before any kernel.
   int start(uint32_t epochs) {
     int r;
-    for (uint32_t j = 0; j + 1 < kBufs && j < epochs; j++)
+    for (uint32_t j = 0; !chunk && j + 1 < kBufs && j < epochs; j++)
       if ((r = draw(j))) return r;
     return CUBFF_OK;
   }
   // After epoch j's kernel is queued: the pairing kBufs - 1 epochs on.
   int next(uint32_t j, uint32_t epochs) {
     const uint32_t k = j + kBufs - 1;
-    return k < epochs ? draw(k) : CUBFF_OK;
+    return !chunk && k < epochs ? draw(k) : CUBFF_OK;
   }
-  int before_kernel(uint32_t j) {
+  int before_kernel(uint32_t j, uint32_t epochs) {
+    if (chunk) {
+      if (j % chunk == 0)
+        HIP_TRY(launch_pair_shuffle_many(c->stream, c->fyb, c->shuf_many, c->n, seed, first + j,
+                                         std::min(chunk, epochs - j)));
+      return CUBFF_OK;
+    }
     HIP_TRY(hipStreamWaitEvent(c->stream, drawn[j % kBufs], 0));
     return CUBFF_OK;
   }
   int after_kernel(uint32_t j) {
-    HIP_TRY(hipEventRecord(used[j % kBufs], c->stream));
+    if (!chunk) HIP_TRY(hipEventRecord(used[j % kBufs], c->stream));
     return CUBFF_OK;
   }
-  // The last epoch's pairing becomes the context's current one; the other
-  // buffers stay with the context.
-  void finish(uint32_t epochs) {
-    if (!epochs) return;
+  // The last epoch's pairing becomes the context's current one (copied out
+  // of the chunk, or its buffer swapped in; the other buffers stay with the
+  // context).
+  int finish(uint32_t epochs) {
+    if (!epochs) return CUBFF_OK;
+    if (chunk) {
+      HIP_TRY(hipMemcpyAsync(c->shuf, shuf(epochs - 1), c->n * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(launch_set_u64(c->stream, c->num_indices, c->n / 2));
+      return CUBFF_OK;
+    }
     const uint32_t last = (epochs - 1) % kBufs;
     c->shuf = buf[last];
     c->num_indices = nidx[last];
@@ -301,6 +374,7 @@ struct PairAhead {
       c->shuf_ahead[i - 1] = buf[(last + i) % kBufs];
       c->nidx_ahead[i - 1] = nidx[(last + i) % kBufs];
     }
+    return CUBFF_OK;
   }
   ~PairAhead() {
     for (uint32_t i = 0; i < kBufs; i++) {
@@ -315,7 +389,7 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
                      uint32_t epochs, uint32_t prob, float *kern_ms) {
   int r;
   PairAhead pa{c, mode, params_seed, first};
-  if ((r = pa.init())) return r;
+  if ((r = pa.init(epochs))) return r;
   std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
   auto cleanup = [&]() {
     for (auto &x : kev)
@@ -331,11 +405,10 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   const uint64_t sm = splitmix64(params_seed);
   r = pa.start(epochs);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
-    if ((r = pa.before_kernel(j))) break;
+    if ((r = pa.before_kernel(j, epochs))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
     if (e == hipSuccess)
-      e = launch_epoch(c->stream, c->lang, c->soup, pa.buf[j % PairAhead::kBufs], c->n,
-                       pa.nidx[j % PairAhead::kBufs],
+      e = launch_epoch(c->stream, c->lang, c->soup, pa.shuf(j), c->n, pa.count(j),
                        epoch_seed_pre(sm, first + j), prob, c->pair_ops, c->pair_steps,
                        c->counters);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
@@ -355,8 +428,7 @@ int run_loop_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   cleanup();
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("epoch run failed: ") + hipGetErrorString(e));
-  pa.finish(epochs);
-  return CUBFF_OK;
+  return pa.finish(epochs);
 }
 
 // Epochs [first, first + epochs) as one dataflow launch; leaves the last
@@ -368,8 +440,18 @@ int run_flow_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t firs
   int r;
   if ((r = ensure_flow_scratch(c, epochs))) return r;
   const uint64_t n = c->n;
-  for (uint32_t j = 0; j < epochs; j++)
-    if ((r = pair_into(c, mode, params_seed, first + j, c->flow_shuf + (uint64_t)j * n))) return r;
+  if (mode == CUBFF_PAIR_SHUFFLE) {
+    // Fisher-Yates pairings several epochs per set of launches.
+    const uint32_t chunk = std::min(fy_batch_epochs(n), epochs);
+    if ((r = ensure_fy_batch(c, fy_batch_epochs(n)))) return r;
+    for (uint32_t j = 0; j < epochs; j += chunk)
+      HIP_TRY(launch_pair_shuffle_many(c->stream, c->fyb, c->flow_shuf + (uint64_t)j * n, n,
+                                       params_seed, first + j, std::min(chunk, epochs - j)));
+  } else {
+    for (uint32_t j = 0; j < epochs; j++)
+      if ((r = pair_into(c, mode, params_seed, first + j, c->flow_shuf + (uint64_t)j * n)))
+        return r;
+  }
   HIP_TRY(launch_flow_pairof(c->stream, c->flow_shuf, c->flow_pairof, n, epochs));
   int khz = 0;
   HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
@@ -482,6 +564,14 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
     if (e == hipSuccess) e = hipEventCreate(&x);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&started, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&resolved, hipEventDisableTiming);
+  // Pairing buffers are allocated before the resolver starts: it waits for
+  // the epoch kernels, and an allocation that waits for the device would
+  // keep them from being queued.
+  PairAhead pa{c, mode, params_seed, first};
+  if (e == hipSuccess && (r = pa.init(epochs))) {
+    cleanup();
+    return r;
+  }
   if (e == hipSuccess) e = launch_fill_u32(c->stream, c->owner, kNone, n);
   if (e == hipSuccess) e = launch_defer_start(c->stream, c->dctl);
   // Until this segment ends well, every queue slot counts as used.
@@ -493,15 +583,12 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
     cleanup();
     return fail(CUBFF_EHIP, std::string("deferred run setup failed: ") + hipGetErrorString(e));
   }
-  PairAhead pa{c, mode, params_seed, first};
-  r = pa.init();
-  if (r == CUBFF_OK) r = pa.start(epochs);
+  r = pa.start(epochs);
   for (uint32_t j = 0; r == CUBFF_OK && e == hipSuccess && j < epochs; j++) {
-    if ((r = pa.before_kernel(j))) break;
+    if ((r = pa.before_kernel(j, epochs))) break;
     if (kern_ms) e = hipEventRecord(kev[2 * j], c->stream);
     if (e == hipSuccess)
-      e = launch_epoch_defer(c->stream, c->lang, a, pa.buf[j % PairAhead::kBufs],
-                             pa.nidx[j % PairAhead::kBufs], j);
+      e = launch_epoch_defer(c->stream, c->lang, a, pa.shuf(j), pa.count(j), j);
     if (e == hipSuccess && kern_ms) e = hipEventRecord(kev[2 * j + 1], c->stream);
     if (e == hipSuccess) r = pa.after_kernel(j);
     if (r == CUBFF_OK && e == hipSuccess) r = pa.next(j, epochs);
@@ -531,7 +618,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   cleanup();
   if (r) return r;
   if (e != hipSuccess) return fail(CUBFF_EHIP, std::string("deferred run failed: ") + hipGetErrorString(e));
-  pa.finish(epochs);
+  if ((r = pa.finish(epochs))) return r;
   // The resolver may leave early when it cannot run beside the epoch
   // kernels (the list then reads kDeferGone or more and holds nothing);
   // the run is complete as long as every deferred entry was run.
@@ -752,6 +839,7 @@ int cubff_ctx_destroy(cubff_ctx *c) {
   dev_free(c->fy.items);
   dev_free(c->fy.first);
   dev_free(c->fy.succ);
+  free_fy_batch(c);
   dev_free(c->g2.order);
   dev_free(c->g2.eu);
   dev_free(c->g2.ew);

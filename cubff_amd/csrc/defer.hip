@@ -360,40 +360,60 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer_lean(
   const LdsByteLut blut{(const lds_u8 *)lut};
   uint32_t ops = 0, steps = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone;
   const bool eval = k < *num_indices_p;  // common_language.h:185
+  const uint64_t deadline = wall_clock64() + a.owe_ticks;
   if (k < a.slots) {
     p1 = shuf[2 * k];
     p2 = shuf[2 * k + 1];
     o1 = ld_agent(a.owner + p1);
     o2 = ld_agent(a.owner + p2);
-    const bool wait = block_wait != 0;
-    const uint64_t deadline = wall_clock64() + a.owe_ticks;
-    if (o1 < kResolved &&
-        (ld_agent(&a.items[o1].done) == a.gen ||
-         (wait ? wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks)
-               : wait_eq_until(a.ctl, &a.items[o1].done, a.gen, deadline))))
-      o1 = kResolved;
-    if (o2 < kResolved &&
-        (ld_agent(&a.items[o2].done) == a.gen ||
-         (wait ? wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks)
-               : wait_eq_until(a.ctl, &a.items[o2].done, a.gen, deadline))))
-      o2 = kResolved;
+    if (o1 < kResolved && ld_agent(&a.items[o1].done) == a.gen) o1 = kResolved;
+    if (o2 < kResolved && ld_agent(&a.items[o2].done) == a.gen) o2 = kResolved;
   }
   after_flag();
   bool defer = false;
   const uint64_t seed_value = epoch_seed_pre(a.sm_seed, a.first_epoch + e_rel);
-  if (k < a.slots) {
-    defer = o1 < kResolved || o2 < kResolved;  // a program is still owed
-    if (!defer) {
+  // Two passes: interactions whose programs are both free run in the first;
+  // those that owe a program wait for it after that (so the wait overlaps
+  // the other lanes' runs: a wavefront's time is the longer of the two, not
+  // their sum) and run in the second, or are deferred if it is still owed
+  // at the deadline.  Wavefronts with nothing owed skip the second pass.
+  bool todo = k < a.slots;
+  // A run stopped by its budget is handed to the resolver as it stands:
+  // its tape goes to the programs' rows (which it owns) and `rs` to the
+  // list entry, and the resolver goes on from there.
+  bool resumable = false;
+  typename F::Resume rs{};
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      if (!F::any_lane(todo)) break;
+      if (todo) {
+        const bool wait = block_wait != 0;
+        if (o1 < kResolved && (wait ? wait_eq(a.ctl, &a.items[o1].done, a.gen, a.spin_ticks)
+                                    : wait_eq_until(a.ctl, &a.items[o1].done, a.gen, deadline)))
+          o1 = kResolved;
+        if (o2 < kResolved && (wait ? wait_eq(a.ctl, &a.items[o2].done, a.gen, a.spin_ticks)
+                                    : wait_eq_until(a.ctl, &a.items[o2].done, a.gen, deadline)))
+          o2 = kResolved;
+        defer = o1 < kResolved || o2 < kResolved;  // a program is still owed
+        todo = !defer;
+      }
+      after_flag();
+    }
+    if (todo && o1 >= kResolved && o2 >= kResolved) {
+      todo = false;
       if (o1 == kResolved) load_raw_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
       else load_raw_row(w, 0, a.soup + (uint64_t)p1 * kTape);
       if (o2 == kResolved) load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
       else load_raw_row(w, 1, a.soup + (uint64_t)p2 * kTape);
       mutate_raw(w, a.n, seed_value, k, a.prob);
-      if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, budget);
+      if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, budget, &rs);
       else raw_as_final(w);  // not evaluated: the mutated tape goes back as it is
       if (steps == kOverBudget) {
-        defer = true;
         ops = steps = 0;
+        defer = true;
+        // Two halves of one row cannot both be kept: such a run starts over.
+        resumable = p1 != p2;
       } else {
         store_lean_row(w, 0, a.soup + (uint64_t)p1 * kTape);
         store_lean_row(w, 1, a.soup + (uint64_t)p2 * kTape);
@@ -430,7 +450,15 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer_lean(
   const uint32_t d2 = (o2 < kResolved && p2 != p1) ? o2 : kNone;
   DeferItem *it = a.items + (put ? at : 0u);
   if (put) {
-    st_agent(&it->e, e_rel | (eval ? 0u : kNoEval));
+    st_agent(&it->e, e_rel | (eval ? 0u : kNoEval) | (resumable ? kResume : 0u));
+    if (resumable) {
+      store_lean_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
+      store_lean_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
+      st_agent(&it->ops, rs.pos);
+      st_agent(&it->steps, rs.i);
+      st_agent(&it->pad[0], rs.iters);
+      st_agent(&it->pad[1], rs.extra);
+    }
     st_agent(&it->k, (uint32_t)k);
     st_agent(&it->p1, p1);
     st_agent(&it->p2, p2);
@@ -810,27 +838,43 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
     // it depends on was done before it was.
     after_flag();
     const uint32_t ew = ld_agent(&it->e);
-    const uint32_t e = ew & ~kNoEval, k = ld_agent(&it->k), p1 = ld_agent(&it->p1),
+    const uint32_t e = ew & ~(kNoEval | kResume), k = ld_agent(&it->k), p1 = ld_agent(&it->p1),
                    p2 = ld_agent(&it->p2);
     const bool eval = !(ew & kNoEval);
     uint32_t s = 0, steps = 0, ops = 0;
     if constexpr (kLean) {
+      using F = typename LeanOf<LANG>::T;
       const auto w = LeanLds<1>::at(lds, 0);
+      const auto anchor = LeanAnchorLds<1>::at(lds + kLeanAnchor, 0);
+      const LdsByteLut blut{(const lds_u8 *)lut};
       load_raw_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
       load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
-      if constexpr (kBatch) {
-        const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
-        s = q.s;
-        mutate_raw(w, q.n, q.seed_value, q.kl, q.prob);
-      } else {
-        mutate_raw(w, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+      // An entry the epoch kernel's budget stopped goes on from its tape,
+      // which is in the rows, and the state in the entry; others start
+      // from the mutated programs.
+      const bool resume = !kBatch && (ew & kResume);
+      if (!resume) {
+        if constexpr (kBatch) {
+          const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
+          s = q.s;
+          mutate_raw(w, q.n, q.seed_value, q.kl, q.prob);
+        } else {
+          mutate_raw(w, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
+        }
       }
-      if (eval)
-        ops = LeanOf<LANG>::T::evaluate_full(w, LeanAnchorLds<1>::at(lds + kLeanAnchor, 0),
-                                             LdsByteLut{(const lds_u8 *)lut}, kEvalSteps, &steps,
-                                             nullptr);
-      else
+      if (eval) {
+        typename F::Run r;
+        F::start_raw(r, w, blut, kEvalSteps);
+        if (resume)
+          F::resume_from(r,
+                         typename F::Resume{ld_agent(&it->ops), ld_agent(&it->steps),
+                                            ld_agent(&it->pad[0]), ld_agent(&it->pad[1])},
+                         kEvalSteps);
+        F::template advance<false, true>(r, w, blut, kEvalSteps, 0xffffffffu, anchor);
+        ops = F::finish(r, kEvalSteps, &steps, nullptr);
+      } else {
         raw_as_final(w);
+      }
       store_lean_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
       store_lean_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
     } else {

@@ -80,6 +80,10 @@ constexpr uint32_t kLinkDone = 0xfffffffdu;
 
 constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
 constexpr uint32_t kNoEval = 0x80000000u;    // DeferItem::e flag
+// DeferItem::e flag of a single soup's entry stopped by its command budget:
+// its tape is in its programs' rows, and ops, steps, pad[0], pad[1] hold the
+// stopped run's position and heads, steps, commands and accelerated ones.
+constexpr uint32_t kResume = 0x40000000u;
 // DeferCtl::tail once the resolver has left without anything deferred: it
 // sets tail from 0 to this value in one compare-and-swap, so an allocation
 // either lands before (the resolver stays) or returns at least kDeferGone
@@ -164,6 +168,12 @@ hipError_t launch_pair_fixed(hipStream_t st, uint32_t *s, uint64_t n,
                              uint64_t epoch);
 hipError_t launch_pair_cyclic(hipStream_t st, uint32_t *s, uint64_t n,
                               uint64_t epoch);
+// The Fisher-Yates pairings of epochs epoch .. epoch + epochs - 1 into
+// s[y * n ..] (y = 0 .. epochs - 1); w's arrays hold `epochs` times their
+// one-epoch sizes.
+hipError_t launch_pair_shuffle_many(hipStream_t st, const FyScratch &w, uint32_t *s,
+                                    uint64_t n, uint64_t params_seed, uint64_t epoch,
+                                    uint32_t epochs);
 hipError_t launch_pair_shuffle(hipStream_t st, const FyScratch &w, uint32_t *s,
                                uint64_t n, uint64_t params_seed, uint64_t epoch);
 hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,

@@ -256,27 +256,34 @@ __global__ void k_pair_cyclic(uint32_t *__restrict__ s, uint64_t n,
 // bucket element above t.  The a()-chains strictly increase, so chasing them
 // terminates.
 
+// Every kernel below draws gridDim.y epochs at once: blockIdx.y = y selects
+// epoch `epoch + y`, whose arrays start y * n entries in (cnt: y * (n + 1),
+// its last entry the bucket allocator's counter).
 __global__ void k_fy_draw(uint32_t *__restrict__ j, uint32_t *__restrict__ cnt,
                           uint64_t n, uint64_t sm_seed, uint64_t epoch) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const uint64_t r = splitmix64(epoch_seed_pre(sm_seed, epoch * n + t));
+  const uint64_t y = blockIdx.y;
+  j += y * n;
+  cnt += y * (n + 1);
+  const uint64_t r = splitmix64(epoch_seed_pre(sm_seed, (epoch + y) * n + t));
   const uint32_t jt = (uint32_t)(r % (t + 1));
   j[t] = jt;
   atomicAdd(&cnt[jt], 1u);
 }
 
-__global__ void k_fy_alloc(const uint32_t *__restrict__ cnt,
+__global__ void k_fy_alloc(uint32_t *__restrict__ cnt,
                            uint32_t *__restrict__ base,
-                           uint32_t *__restrict__ cursor,
-                           unsigned int *__restrict__ alloc, uint64_t n) {
+                           uint32_t *__restrict__ cursor, uint64_t n) {
   const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t y = blockIdx.y;
   const bool in = p < n;
-  const uint32_t c = in ? cnt[p] : 0u;
-  const uint32_t b = wave_alloc_u32(alloc, c);  // all lanes take part
+  const uint32_t c = in ? cnt[y * (n + 1) + p] : 0u;
+  // all lanes take part
+  const uint32_t b = wave_alloc_u32(cnt + y * (n + 1) + n, c);
   if (!in) return;
-  base[p] = b;
-  cursor[p] = 0;
+  base[y * n + p] = b;
+  cursor[y * n + p] = 0;
 }
 
 __global__ void k_fy_place(const uint32_t *__restrict__ j,
@@ -285,8 +292,9 @@ __global__ void k_fy_place(const uint32_t *__restrict__ j,
                            uint32_t *__restrict__ items, uint64_t n) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const uint32_t jt = j[t];
-  items[base[jt] + atomicAdd(&cursor[jt], 1u)] = (uint32_t)t;
+  const uint64_t o = (uint64_t)blockIdx.y * n;
+  const uint32_t jt = j[o + t];
+  items[o + base[o + jt] + atomicAdd(&cursor[o + jt], 1u)] = (uint32_t)t;
 }
 
 __global__ void k_fy_sort(const uint32_t *__restrict__ cnt,
@@ -296,20 +304,21 @@ __global__ void k_fy_sort(const uint32_t *__restrict__ cnt,
                           uint32_t *__restrict__ succ, uint64_t n) {
   uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const uint32_t c = cnt[p];
+  const uint64_t o = (uint64_t)blockIdx.y * n;
+  const uint32_t c = cnt[blockIdx.y * (n + 1) + p];
   if (c == 0) {
-    first[p] = kNone;
+    first[o + p] = kNone;
     return;
   }
-  uint32_t *b = items + base[p];
+  uint32_t *b = items + o + base[o + p];
   for (uint32_t x = 1; x < c; x++) {  // insertion sort; buckets are tiny
     const uint32_t v = b[x];
     uint32_t y = x;
     for (; y > 0 && b[y - 1] > v; y--) b[y] = b[y - 1];
     b[y] = v;
   }
-  first[p] = b[0];
-  for (uint32_t x = 0; x < c; x++) succ[b[x]] = x + 1 < c ? b[x + 1] : kNone;
+  first[o + p] = b[0];
+  for (uint32_t x = 0; x < c; x++) succ[o + b[x]] = x + 1 < c ? b[x + 1] : kNone;
 }
 
 __device__ __forceinline__ uint32_t fy_in(uint32_t t,
@@ -329,6 +338,10 @@ __global__ void k_fy_final(const uint32_t *__restrict__ j,
                            uint32_t *__restrict__ s, uint64_t n) {
   uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  const uint64_t o = (uint64_t)blockIdx.y * n;
+  j += o;
+  first += o;
+  succ += o;
   const uint32_t jt = j[t];
   uint32_t v;
   if (jt == (uint32_t)t) {
@@ -337,7 +350,7 @@ __global__ void k_fy_final(const uint32_t *__restrict__ j,
     const uint32_t c = succ[t];
     v = c == kNone ? jt : fy_in(c, j, first, succ);
   }
-  s[t] = v;
+  s[o + t] = v;
 }
 
 // 2-D greedy neighbour pairing (common_language.h:435-463).  Visiting
@@ -586,24 +599,25 @@ hipError_t launch_pair_cyclic(hipStream_t st, uint32_t *s, uint64_t n,
   return hipGetLastError();
 }
 
+hipError_t launch_pair_shuffle_many(hipStream_t st, const FyScratch &w, uint32_t *s,
+                                    uint64_t n, uint64_t params_seed, uint64_t epoch,
+                                    uint32_t epochs) {
+  const uint64_t sm_seed = splitmix64(params_seed);
+  const dim3 g(blocks_for(n, 256), epochs);
+  hipError_t e;
+  if ((e = launch_fill_u32(st, w.cnt, 0, (n + 1) * epochs)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_fy_draw, g, dim3(256), 0, st, w.j, w.cnt, n, sm_seed, epoch);
+  hipLaunchKernelGGL(k_fy_alloc, g, dim3(256), 0, st, w.cnt, w.base, w.cursor, n);
+  hipLaunchKernelGGL(k_fy_place, g, dim3(256), 0, st, w.j, w.base, w.cursor, w.items, n);
+  hipLaunchKernelGGL(k_fy_sort, g, dim3(256), 0, st, w.cnt, w.base, w.items, w.first, w.succ, n);
+  hipLaunchKernelGGL(k_fy_final, g, dim3(256), 0, st, w.j, w.first, w.succ, s, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_pair_shuffle(hipStream_t st, const FyScratch &w, uint32_t *s,
                                uint64_t n, uint64_t params_seed,
                                uint64_t epoch) {
-  const uint64_t sm_seed = splitmix64(params_seed);
-  const unsigned g = blocks_for(n, 256);
-  hipError_t e;
-  if ((e = launch_fill_u32(st, w.cnt, 0, n + 1)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_fy_draw, dim3(g), dim3(256), 0, st, w.j, w.cnt, n,
-                     sm_seed, epoch);
-  hipLaunchKernelGGL(k_fy_alloc, dim3(g), dim3(256), 0, st, w.cnt, w.base,
-                     w.cursor, w.cnt + n, n);
-  hipLaunchKernelGGL(k_fy_place, dim3(g), dim3(256), 0, st, w.j, w.base,
-                     w.cursor, w.items, n);
-  hipLaunchKernelGGL(k_fy_sort, dim3(g), dim3(256), 0, st, w.cnt, w.base,
-                     w.items, w.first, w.succ, n);
-  hipLaunchKernelGGL(k_fy_final, dim3(g), dim3(256), 0, st, w.j, w.first,
-                     w.succ, s, n);
-  return hipGetLastError();
+  return launch_pair_shuffle_many(st, w, s, n, params_seed, epoch, 1);
 }
 
 hipError_t launch_pair_2d(hipStream_t st, const FyScratch &w,

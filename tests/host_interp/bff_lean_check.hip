@@ -104,7 +104,8 @@ bool tables_consistent(const uint32_t *wd, const L &lut) {
 
 template <int MAP, bool HEADS>
 int check_tape(const char *name, int orc_lang, const uint8_t *tape, uint32_t stepcount,
-               uint32_t budget, uint64_t slice_seed, int r, uint64_t *over, uint64_t *cap_runs) {
+               uint32_t budget, uint64_t slice_seed, int r, uint64_t *over, uint64_t *cap_runs,
+               uint64_t *resumed) {
   using F = BffLean<MAP, HEADS>;
   static const HostCodes<MAP, HEADS> lut;
   uint8_t ref[128];
@@ -123,6 +124,43 @@ int check_tape(const char *name, int orc_lang, const uint8_t *tape, uint32_t ste
                             : F::evaluate(HostWords{wd}, lut, stepcount, &steps, &work, budget);
   if (steps == kOverBudget) {
     (*over)++;
+    // A run stopped by its budget goes on to the end from its tape bytes
+    // and saved state on a table built afresh (as a deferred interaction
+    // does): the oracle's results, with and without the cycle skip.
+    typename F::Run run;
+    uint32_t ws[129];
+    for (int j = 0; j < 32; j++) ws[j] = words[j];
+    F::start_raw(run, HostWords{ws}, lut, stepcount);
+    F::template advance<false>(run, HostWords{ws}, lut, stepcount, budget);
+    if (run.done) {
+      printf("%s tape %d: over budget once, not the second time\n", name, r);
+      return 1;
+    }
+    const typename F::Resume sv = F::saved(run);
+    uint8_t mid[128];
+    for (int p = 0; p < 128; p++) mid[p] = (uint8_t)ws[p];
+    uint32_t wr[129];
+    to_words(mid, wr);
+    uint32_t steps3 = 0, ops3;
+    uint32_t anchor3[32];
+    if (r & 1) {
+      ops3 = F::evaluate_resumed(HostWords{wr}, HostAnchor{anchor3}, lut, stepcount, sv, &steps3,
+                                 nullptr);
+    } else {
+      typename F::Run r3;
+      F::start_resume(r3, HostWords{wr}, lut, stepcount, sv);
+      F::template advance<false>(r3, HostWords{wr}, lut, stepcount, 0xffffffffu);
+      ops3 = F::finish(r3, stepcount, &steps3, nullptr);
+    }
+    uint8_t out3[128];
+    for (int p = 0; p < 128; p++) out3[p] = (uint8_t)wr[p];
+    if (ops3 != ref_ops || steps3 != ref_steps || memcmp(out3, ref, 128) != 0) {
+      printf("%s tape %d cap %u budget %u: resumed on a new table ops %u/%llu steps %u/%llu "
+             "tape %s\n", name, r, stepcount, budget, ops3, (unsigned long long)ref_ops, steps3,
+             (unsigned long long)ref_steps, memcmp(out3, ref, 128) ? "differs" : "same");
+      return 1;
+    }
+    (*resumed)++;
     return 0;
   }
   uint8_t out[128];
@@ -173,19 +211,21 @@ int check_tape(const char *name, int orc_lang, const uint8_t *tape, uint32_t ste
 template <int MAP, bool HEADS>
 int check(const char *name, int orc_lang, int count, uint64_t seed) {
   uint8_t tape[128];
-  uint64_t over = 0, cap_runs = 0;
+  uint64_t over = 0, cap_runs = 0, resumed = 0;
   for (int r = 0; r < count; r++) {
     make_tape(MAP, seed * 1000003 + r, tape);
     const uint64_t x = sm(seed ^ (0x9e37ull * (r + 1)));
     const uint32_t caps[4] = {8192, (uint32_t)(x % 300), 1 + (uint32_t)(x >> 20) % 3000, 0};
     const uint32_t stepcount = caps[(x >> 40) & 3];
     const uint32_t budget = ((x >> 44) & 3) == 0 ? 0xffffffffu : 1 + (uint32_t)((x >> 46) % 400);
-    if (check_tape<MAP, HEADS>(name, orc_lang, tape, stepcount, budget, x, r, &over, &cap_runs))
+    if (check_tape<MAP, HEADS>(name, orc_lang, tape, stepcount, budget, x, r, &over, &cap_runs,
+                               &resumed))
       return 1;
   }
-  printf("%s: %d tapes ok (%llu at the step limit, %llu over budget; complete runs with the "
-         "cycle skip every other tape, resumed runs every other pair)\n", name, count,
-         (unsigned long long)cap_runs, (unsigned long long)over);
+  printf("%s: %d tapes ok (%llu at the step limit, %llu over budget and resumed on a new "
+         "table; complete runs with the cycle skip every other tape, resumed runs every other "
+         "pair)\n", name, count, (unsigned long long)cap_runs, (unsigned long long)over);
+  if (over != resumed) return 1;
   return 0;
 }
 
@@ -209,12 +249,12 @@ int main(int argc, char **argv) {
     uint8_t buf[128];
     while (fread(buf, 1, 128, f) == 128) all.insert(all.end(), buf, buf + 128);
     fclose(f);
-    uint64_t over = 0, cap_runs = 0;
+    uint64_t over = 0, cap_runs = 0, resumed = 0;
     const int n = (int)(all.size() / 128);
     for (int r = 0; r < n && !bad; r++)
       for (uint32_t budget : {0xffffffffu, 768u})
         bad |= check_tape<kMapAscii, false>("file", ORC_BFF_NOHEADS, &all[128 * (size_t)r], 8192,
-                                            budget, sm(r), r, &over, &cap_runs);
+                                            budget, sm(r), r, &over, &cap_runs, &resumed);
     printf("file: %d tapes ok (%llu at the step limit, %llu over budget)\n", n,
            (unsigned long long)cap_runs, (unsigned long long)over);
   }

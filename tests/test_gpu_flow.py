@@ -169,6 +169,31 @@ def test_defer_equals_epoch_loop_at_2p17(capi):
                pipeline=capi.FLOW_DEFER)
 
 
+def test_defer_resolver_serves_every_run(capi):
+    # Runs of different lengths one after another (a short one, then one
+    # with several chunks of Fisher-Yates pairings): each run's resolver
+    # takes deferred interactions (none of them falls back to running every
+    # interaction in the epoch kernels because its resolver left early), and
+    # the soup equals the epoch-by-epoch loop's.
+    n, seed, P = 131072, 2, 1 << 18
+    print("defer runs of 6, 40 and 70 epochs at", n, flush=True)
+    out = []
+    for pipeline in (capi.FLOW_DEFER, capi.FLOW_OFF):
+        with capi.Soup("bff_noheads", n) as g:
+            g.set_pipeline(pipeline)
+            g.init_programs(O.epoch_seed(seed, 0))
+            first = 0
+            for count in (6, 40, 70):
+                d0 = g.flow_stats()[2]
+                g.run_epochs(capi.PAIR_SHUFFLE, seed, first, count, P)
+                first += count
+                if pipeline == capi.FLOW_DEFER:
+                    assert g.flow_stats()[2] > d0, count
+            out.append((_sha(g.read_soup()), g.read_counters(), _sha(g.read_pair_ops()[0]),
+                        _sha(g.read_shuffle()[0])))
+    assert out[0] == out[1]
+
+
 def test_defer_segment_boundary_at_2p18(capi):
     # 2^18 programs: deferring segments of 256 epochs (their entry lists
     # hold 16 bytes per program and epoch, up to 1 GiB), so 300 epochs cross
