@@ -561,7 +561,7 @@ def main():
             "lang": lang, "num_programs": n, "seed": seed,
             "parallelism": "single soup, one lane per pair slot",
             "path": {"loop": "one launch per epoch (k_epoch)",
-                     "defer": f"{defer_kernel} per epoch (command budget 768) + resolver "
+                     "defer": f"{defer_kernel} per epoch (command budget 512) + resolver "
                               "kernel on a second stream",
                      "dataflow": "multi-epoch dataflow kernel (k_flow)"}[path],
         },

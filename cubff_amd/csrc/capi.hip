@@ -13,8 +13,10 @@
 
 using namespace cubff;
 
-// Command budget of an interaction in the deferring epoch kernel.
-constexpr uint32_t kDeferBudget = 768;
+// Command budget of an interaction in the deferring epoch kernel: runs
+// past it are resumed by the resolver (2^17 bff_noheads: 384-768 within 1%
+// of each other, 512 best).
+constexpr uint32_t kDeferBudget = 512;
 
 struct cubff_ctx {
   int lang = 0;

@@ -130,7 +130,7 @@ int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims, uint64_t
  * without copying the soup. */
 int cubff_byte_histogram(cubff_ctx *ctx, uint64_t *counts);
 /* Command budget of an interaction in CUBFF_FLOW_DEFER epoch kernels
- * (default 768). */
+ * (default 512). */
 int cubff_set_defer_budget(cubff_ctx *ctx, uint32_t budget);
 /* How long the resolver of CUBFF_FLOW_DEFER runs waits, with nothing
  * deferred and no epoch kernel finishing, before it leaves (default 0.1 s;
