@@ -15,10 +15,10 @@
 //                       the cost of one lane's command, nothing else issuing,
 //   kernel_ms           the launch (best of three).
 // Interpreters: the word-table interpreter (bff_lean.h) budgeted as in the
-// deferring epoch kernel and complete (without its cycle skip: at 256 lanes
-// the LDS holds no anchor copies), and the byte-tape interpreters it
-// replaced there (bff_fast.h budgeted, languages.h complete with its cycle
-// skip).
+// deferring epoch kernel, complete without its cycle skip (256 lanes: the
+// LDS holds no anchor copies) and complete with it as the resolver runs it
+// (128 lanes), and the byte-tape interpreters it replaced there (bff_fast.h
+// budgeted, languages.h complete with its cycle skip).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kLanes = 256;
 
-enum Kind { kLeanBudget, kLeanFull, kFastBudget, kLangFull };
+enum Kind { kLeanBudget, kLeanFull, kFastBudget, kLangFull, kLeanCycle };
 
 __global__ __launch_bounds__(kLanes) void k_probe_lean(const uint8_t *tapes, uint32_t n,
                                                        uint32_t budget, int full,
@@ -64,6 +64,35 @@ __global__ __launch_bounds__(kLanes) void k_probe_lean(const uint8_t *tapes, uin
   if (k < n) F::evaluate(w, blut, kEvalSteps, &steps, &work, full ? 0xffffffffu : budget);
   const uint64_t c1 = __builtin_amdgcn_s_memtime();
   if (k < n) work_o[k] = steps == kOverBudget ? budget : work;
+  if ((lane & 63) == 0) cyc_o[k / 64] = c1 - c0;
+}
+
+// The complete run with the full-state cycle skip, as the resolver runs it
+// (128 lanes per workgroup: the LDS then also holds the anchors).
+__global__ __launch_bounds__(kLanes / 2) void k_probe_lean_cycle(const uint8_t *tapes, uint32_t n,
+                                                                uint32_t *work_o, uint64_t *cyc_o) {
+  constexpr int kL = kLanes / 2;
+  using F = BffLean<kMapAscii, false>;
+  using W = LeanLds<kL>;
+  using A = LeanAnchorLds<kL>;
+  __shared__ __align__(16) uint8_t lds[W::kRegionBytes + A::kRegionBytes];
+  __shared__ uint8_t lut[256];
+  fill_lean_lut<kBffNoheads>(lut);
+  __syncthreads();
+  const int lane = threadIdx.x;
+  const uint32_t k = blockIdx.x * kL + lane;
+  const W w = W::at(lds, lane);
+  const A anchor = A::at(lds + W::kRegionBytes, lane);
+  const LdsByteLut blut{(const lds_u8 *)lut};
+  if (k < n) {
+    load_raw_row(w, 0, tapes + (uint64_t)k * kTape2);
+    load_raw_row(w, 1, tapes + (uint64_t)k * kTape2 + kTape);
+  }
+  uint32_t steps = 0, work = 0;
+  const uint64_t c0 = __builtin_amdgcn_s_memtime();
+  if (k < n) F::evaluate_full(w, anchor, blut, kEvalSteps, &steps, &work);
+  const uint64_t c1 = __builtin_amdgcn_s_memtime();
+  if (k < n) work_o[k] = work;
   if ((lane & 63) == 0) cyc_o[k / 64] = c1 - c0;
 }
 
@@ -114,6 +143,7 @@ struct Fit {
 
 Fit run(int kind, const uint8_t *d_t, uint32_t n, uint32_t budget, uint32_t *d_w, uint64_t *d_c) {
   const uint32_t blocks = (n + kLanes - 1) / kLanes, waves = blocks * 4;
+  const uint32_t half_blocks = (n + kLanes / 2 - 1) / (kLanes / 2);
   hipEvent_t a, b;
   CK(hipEventCreate(&a));
   CK(hipEventCreate(&b));
@@ -121,7 +151,10 @@ Fit run(int kind, const uint8_t *d_t, uint32_t n, uint32_t budget, uint32_t *d_w
   float best = 1e30f;
   for (int rep = 0; rep < 3; rep++) {
     CK(hipEventRecord(a, 0));
-    if (kind == kLeanBudget || kind == kLeanFull)
+    if (kind == kLeanCycle)
+      hipLaunchKernelGGL(k_probe_lean_cycle, dim3(half_blocks), dim3(kLanes / 2), 0, 0, d_t, n,
+                         d_w, d_c);
+    else if (kind == kLeanBudget || kind == kLeanFull)
       hipLaunchKernelGGL(k_probe_lean, dim3(blocks), dim3(kLanes), 0, 0, d_t, n, budget, full,
                          d_w, d_c);
     else
@@ -194,10 +227,11 @@ int main(int argc, char **argv) {
   CK(hipMalloc(&d_c, (size_t)((n + kLanes - 1) / kLanes) * 4 * 8));
   CK(hipMemcpy(d_t, h.data(), h.size(), hipMemcpyHostToDevice));
   CK(hipMemcpy(d_lone, lone.data(), lone.size(), hipMemcpyHostToDevice));
-  static const char *kNames[4] = {"word_table_budgeted", "word_table_complete_no_cycle_skip",
-                                  "byte_tape_budgeted", "byte_tape_complete_cycle_skip"};
+  static const char *kNames[5] = {"word_table_budgeted", "word_table_complete_no_cycle_skip",
+                                  "byte_tape_budgeted", "byte_tape_complete_cycle_skip",
+                                  "word_table_complete_cycle_skip"};
   printf("{\"tapes\": %u, \"budget\": %u", n, budget);
-  for (int kind = 0; kind < 4; kind++) {
+  for (int kind = 0; kind < 5; kind++) {
     const Fit all = run(kind, d_t, n, budget, d_w, d_c);
     const Fit one = run(kind, d_lone, n, budget, d_w, d_c);
     printf(", \"%s\": {\"cycles_per_command\": %.1f, \"kernel_ms\": %.4f, \"max_commands\": %u, "
