@@ -583,6 +583,9 @@ __global__ __launch_bounds__(LANES) void k_b_refill_defer(DeferArgs a, uint32_t 
   r.done = true;
   r.iters = 0;
   uint32_t st = kLaneFree, g = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone, budget = budget0;
+  // The lane's run was stopped by the budget (not deferred while waiting):
+  // the resolver goes on from its tape and state.
+  bool stopped = false;
   uint64_t deadline = 0;
   SlotSoup q{};
   bool exhausted = false;
@@ -651,8 +654,20 @@ __global__ __launch_bounds__(LANES) void k_b_refill_defer(DeferArgs a, uint32_t 
       const uint32_t d1 = o1 < kResolved ? o1 : kNone;
       const uint32_t d2 = (o2 < kResolved && p2 != p1) ? o2 : kNone;
       DeferItem *it = a.items + (put ? at : 0u);
+      // A stopped run's tape goes to its programs' rows (the entry owns
+      // them from here on; a slot pairing a program with itself starts over)
+      // and its state to the entry.
+      const bool handed = put && stopped && p1 != p2;
       if (put) {
-        st_agent(&it->e, e_rel | (q.eval ? 0u : kNoEval));
+        st_agent(&it->e, e_rel | (q.eval ? 0u : kNoEval) | (handed ? kResume : 0u));
+        if (handed) {
+          store_row_coherent(t, 0, a.soup + (uint64_t)p1 * kTape);
+          store_row_coherent(t, 1, a.soup + (uint64_t)p2 * kTape);
+          st_agent(&it->ops, (uint32_t)r.pc | ((uint32_t)r.h0 << 8) | ((uint32_t)r.h1 << 16));
+          st_agent(&it->steps, r.i);
+          st_agent(&it->pad[0], r.iters);
+          st_agent(&it->pad[1], r.extra);
+        }
         st_agent(&it->k, g);
         st_agent(&it->p1, p1);
         st_agent(&it->p2, p2);
@@ -697,6 +712,7 @@ __global__ __launch_bounds__(LANES) void k_b_refill_defer(DeferArgs a, uint32_t 
         exhausted = (uint64_t)base + nf >= a.slots;
         const uint64_t e = (uint64_t)base + __popcll(fm & ((1ull << wl) - 1));
         if (st == kLaneFree && e < a.slots) {
+          stopped = false;
           g = a.list[e];
           q = slot_soup(a, g, epoch, slot_base);
           p1 = a.tgt[2 * (uint64_t)g];
@@ -725,8 +741,12 @@ __global__ __launch_bounds__(LANES) void k_b_refill_defer(DeferArgs a, uint32_t 
     }
     if (st == kLaneRun) {
       if (!r.done) F::template advance<true>(r, t, nx, anchor, blut, kEvalSteps, kRunSlice);
-      if (r.done) st = kLaneDone;
-      else if (r.iters >= budget) st = kLaneDefer;
+      if (r.done) {
+        st = kLaneDone;
+      } else if (r.iters >= budget) {
+        st = kLaneDefer;
+        stopped = true;
+      }
     }
     // Only lanes waiting on owed programs: give the memory system a rest.
     if (!__any(st == kLaneRun) && __any(st == kLaneWait)) __builtin_amdgcn_s_sleep(2);
@@ -739,7 +759,7 @@ __global__ void k_b_defer_fixup(DeferArgs a, uint32_t listed) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= listed) return;
   const DeferItem &it = a.items[x];
-  if ((it.e & ~kNoEval) + 1 == a.epochs) {
+  if ((it.e & ~(kNoEval | kResume)) + 1 == a.epochs) {
     a.pair_ops[it.k] = it.ops;
     a.pair_steps[it.k] = it.steps;
   }
@@ -849,18 +869,16 @@ __global__ __launch_bounds__(64) void k_resolve(DeferArgs a) {
       const LdsByteLut blut{(const lds_u8 *)lut};
       load_raw_row_coherent(w, 0, a.soup + (uint64_t)p1 * kTape);
       load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
-      // An entry the epoch kernel's budget stopped goes on from its tape,
+      // An entry an epoch kernel's budget stopped goes on from its tape,
       // which is in the rows, and the state in the entry; others start
       // from the mutated programs.
-      const bool resume = !kBatch && (ew & kResume);
-      if (!resume) {
-        if constexpr (kBatch) {
-          const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
-          s = q.s;
-          mutate_raw(w, q.n, q.seed_value, q.kl, q.prob);
-        } else {
-          mutate_raw(w, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
-        }
+      const bool resume = (ew & kResume) != 0;
+      if constexpr (kBatch) {
+        const SlotSoup q = slot_soup(a, k, a.first_epoch + e, a.slot_base);
+        s = q.s;
+        if (!resume) mutate_raw(w, q.n, q.seed_value, q.kl, q.prob);
+      } else {
+        if (!resume) mutate_raw(w, a.n, epoch_seed_pre(a.sm_seed, a.first_epoch + e), k, a.prob);
       }
       if (eval) {
         typename F::Run r;

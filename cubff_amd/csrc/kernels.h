@@ -80,9 +80,10 @@ constexpr uint32_t kLinkDone = 0xfffffffdu;
 
 constexpr uint32_t kResolved = 0xfffffffeu;  // owner[p]: last written by the resolver
 constexpr uint32_t kNoEval = 0x80000000u;    // DeferItem::e flag
-// DeferItem::e flag of a single soup's entry stopped by its command budget:
-// its tape is in its programs' rows, and ops, steps, pad[0], pad[1] hold the
-// stopped run's position and heads, steps, commands and accelerated ones.
+// DeferItem::e flag of an entry stopped by its command budget: its tape is
+// in its programs' rows, and ops, steps, pad[0], pad[1] hold the stopped
+// run's position and heads, steps, commands and accelerated ones (a batched
+// entry's ops and steps then receive its results).
 constexpr uint32_t kResume = 0x40000000u;
 // DeferCtl::tail once the resolver has left without anything deferred: it
 // sets tail from 0 to this value in one compare-and-swap, so an allocation

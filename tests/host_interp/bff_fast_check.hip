@@ -1,6 +1,7 @@
 // CPU check of the budgeted BFF interpreter (cubff_amd/csrc/bff_fast.h)
 // against the oracle's Evaluate: the header's template compiled for the
-// host, run on random tapes at random step limits and command budgets.
+// host, run on random tapes at random step limits and command budgets, and
+// runs it stopped handed to the word-table interpreter (bff_lean.h).
 // Test infrastructure: built and run by tests/test_cpu_interp.py.
 //
 //   bff_fast_check <tapes per language> <seed>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "../../cubff_amd/csrc/bff_fast.h"
+#include "../../cubff_amd/csrc/bff_lean.h"
 #include "../../oracle/cubff_oracle.h"
 
 using namespace cubff;
@@ -29,6 +31,26 @@ struct HostBytes {
   void set_word(uint32_t w, uint32_t v) const {
     for (int b = 0; b < 4; b++) p[4 * w + b] = (uint8_t)(v >> (8 * b));
   }
+};
+
+// The word-table interpreter's per-position words and code table, in host
+// memory.
+struct LeanWords {
+  uint32_t *p;
+  uint32_t word(uint32_t pos) const { return p[pos]; }
+  void set_word(uint32_t pos, uint32_t w) const { p[pos] = w; }
+  void set_low(uint32_t pos, uint32_t v) const { p[pos] = (p[pos] & 0xffff0000u) | (v & 0xffff); }
+  void set_byte(uint32_t pos, int b, uint32_t v) const {
+    p[pos] = (p[pos] & ~(0xffu << (8 * b))) | ((v & 0xff) << (8 * b));
+  }
+};
+template <int MAP, bool HEADS>
+struct LeanCodes {
+  uint8_t t[256];
+  LeanCodes() {
+    for (int v = 0; v < 256; v++) t[v] = (uint8_t)BffLean<MAP, HEADS>::code_of(v);
+  }
+  uint32_t operator()(uint32_t v) const { return t[v & 255]; }
 };
 
 template <int MAP>
@@ -67,7 +89,7 @@ template <int MAP, bool HEADS, bool ACCEL>
 int check(const char *name, int orc_lang, int count, uint64_t seed) {
   HostLut<MAP> lut;
   uint8_t tape[129], ref[128], nxb[129];
-  uint64_t cap_runs = 0, over = 0, ops_sum = 0, full_runs = 0, resumed = 0;
+  uint64_t cap_runs = 0, over = 0, ops_sum = 0, full_runs = 0, resumed = 0, handed = 0;
   for (int r = 0; r < count; r++) {
     make_tape(MAP, seed * 1000003 + r, tape);
     const uint64_t x = sm(seed ^ (0x9e37ull * (r + 1)));
@@ -152,11 +174,49 @@ int check(const char *name, int orc_lang, int count, uint64_t seed) {
       }
       resumed++;
     }
+    // A run stopped after a command budget in slices of 8 (as the batched
+    // refill kernel runs it) goes on to the end in the word-table
+    // interpreter from its tape bytes and state (as the resolver does).
+    if (ACCEL) {
+      using F = BffFast<MAP, HEADS, ACCEL>;
+      using G = BffLean<MAP, HEADS>;
+      uint8_t t3[129], nx3[129];
+      make_tape(MAP, seed * 1000003 + r, t3);
+      typename F::Run run;
+      F::start(run, HostBytes{t3}, HostBytes{nx3}, lut, stepcount);
+      const uint32_t cut = 1 + (uint32_t)(sm(x + 5) % 300);
+      while (!run.done && run.iters < cut)
+        F::template advance<false>(run, HostBytes{t3}, HostBytes{nx3}, typename F::NoAux{}, lut,
+                                   stepcount, 8);
+      if (!run.done) {
+        const typename G::Resume sv{(uint32_t)run.pc | ((uint32_t)run.h0 << 8) |
+                                        ((uint32_t)run.h1 << 16),
+                                    run.i, run.iters, run.extra};
+        uint32_t wr[129];
+        for (int w = 0; w < 32; w++) wr[w] = HostBytes{t3}.word(w);
+        static const LeanCodes<MAP, HEADS> codes;
+        typename G::Run gr;
+        G::start_resume(gr, LeanWords{wr}, codes, stepcount, sv);
+        G::template advance<false>(gr, LeanWords{wr}, codes, stepcount, 0xffffffffu);
+        uint32_t steps3 = 0;
+        const uint32_t ops3 = G::finish(gr, stepcount, &steps3, nullptr);
+        uint8_t out3[128];
+        for (int q = 0; q < 128; q++) out3[q] = (uint8_t)wr[q];
+        if (ops3 != ref_ops || steps3 != ref_steps || memcmp(out3, ref, 128) != 0) {
+          printf("%s tape %d cap %u: handed to the word table after %u commands: ops %u/%llu "
+                 "steps %u/%llu tape %s\n", name, r, stepcount, cut, ops3,
+                 (unsigned long long)ref_ops, steps3, (unsigned long long)ref_steps,
+                 memcmp(out3, ref, 128) ? "differs" : "same");
+          return 1;
+        }
+        handed++;
+      }
+    }
   }
   printf("%s: %d tapes ok (%llu at the step limit, %llu over budget, %llu with the cycle skip, "
-         "%llu resumed in slices, %llu ops)\n", name, count, (unsigned long long)cap_runs,
-         (unsigned long long)over, (unsigned long long)full_runs, (unsigned long long)resumed,
-         (unsigned long long)ops_sum);
+         "%llu resumed in slices, %llu handed to the word table, %llu ops)\n", name, count,
+         (unsigned long long)cap_runs, (unsigned long long)over, (unsigned long long)full_runs,
+         (unsigned long long)resumed, (unsigned long long)handed, (unsigned long long)ops_sum);
   return 0;
 }
 
