@@ -137,6 +137,9 @@ struct BffLean {
     int bsrc, bq, bnext, binc, fsrc, fq, fnext, finc;
     bool bacc, bwrite;
     uint32_t bcmds;
+    // The backward one's body is straight and short enough to be run
+    // iteration by iteration without decoding (complete runs only).
+    bool bcomp;
     // Whether the match bytes are the tape's (until a bracket is written).
     uint32_t mvalid;
     // Loop anchor: its `]`, heads, steps and commands after it, the bytes
@@ -266,7 +269,7 @@ struct BffLean {
     r.done = !(r.i < stepcount && first < kTape2);
     r.bsrc = -1; r.bq = -1; r.bnext = 0; r.binc = 0;
     r.fsrc = -1; r.fq = kTape2; r.fnext = 0; r.finc = 0;
-    r.bacc = r.bwrite = false;
+    r.bacc = r.bwrite = r.bcomp = false;
     r.bcmds = 0;
     r.mvalid = 1;
     r.aq = -1; r.ah0 = 0; r.ah1 = 0;
@@ -309,6 +312,88 @@ struct BffLean {
 #endif
   }
 
+  // Straight loop bodies of up to kStraightMax commands run without
+  // decoding in complete runs.
+  static constexpr uint32_t kStraightMax = 8;
+
+  // Whole iterations of the straight loop (no bracket in its body) whose
+  // `]` at `close` (matching the `[` at q) was just taken: the nc body
+  // commands from pos0 run from a list of their kinds read once, without
+  // the per-command decode, next-command and jump work.  On entry I and IT
+  // are the steps and commands after the `]` (the next command is pos0);
+  // each iteration adds len = close - pos0 + binc steps and nc + 1
+  // commands.  It stops, with the state exact, where the general loop must
+  // take over: before an iteration that would reach the step cap or pass
+  // the command limit lim1 (at pos0), before a write into the loop's own
+  // code or one that changes a byte's class or may bring a byte to a
+  // command value (at that command), and at the `]` when the byte under
+  // head 0 is null (the exit) or the cycle skip's check is due there (the
+  // anchor state cq/ch0/ch1 with no change since, no anchor yet, or the
+  // anchor's window passed).  Returns the position of the next command;
+  // heads, I, IT and cdelta (the cycle skip's weighted change sum) follow.
+  template <class W>
+  __host__ __device__ static int run_straight(const W &w, int close, int q, int pos0,
+                                              uint32_t binc, uint32_t nc, uint32_t stepcount,
+                                              uint32_t lim1, int cq, int ch0, int ch1,
+                                              uint32_t ci, uint32_t cwin, int &h0, int &h1,
+                                              uint32_t &I, uint32_t &IT, uint32_t &cdelta) {
+    uint32_t kinds = 0, plo = 0, phi = 0;
+    int p = pos0;
+    for (uint32_t j = 0; j < nc; j++) {
+      const uint32_t wp = w.word(p);
+      kinds |= ((wp >> 8) & 15) << (4 * j);
+      if (j < 4) plo |= (uint32_t)p << (7 * j);
+      else phi |= (uint32_t)p << (7 * (j - 4));
+      p = (int)((wp >> 16) & 0xff);
+    }
+    if (p != close) return pos0;
+    const uint32_t len = (uint32_t)(close - pos0) + binc;
+    for (;;) {
+      if (I + len >= stepcount || (int)(lim1 - (IT + nc + 1)) < 0) return pos0;
+      for (uint32_t j = 0; j < nc; j++) {
+        const uint32_t k = (kinds >> (4 * j)) & 15;
+        if ((k - kPlus) < 4u) {
+          const bool c01 = k == kCopy01;
+          const int a = c01 ? h1 : h0;
+          const int pj = (int)((j < 4 ? plo >> (7 * j) : phi >> (7 * (j - 4))) & 127);
+          const uint32_t wold = w.word(a);
+          uint32_t low;
+          bool stop = a >= q && a <= close;
+          if (k == kPlus || k == kMinus) {
+            stop = stop || ((wold >> 8) & 0x7f) - 2 >= 14u;
+            low = (((wold & 0xff) + (k == kPlus ? 1u : 0xffu)) & 0xff) | ((wold & 0xff00) - 0x100u);
+          } else {
+            const uint32_t wsrc = w.word(c01 ? h0 : h1);
+            stop = stop || ((wsrc >> 12) & 7) != ((wold >> 12) & 7);
+            low = wsrc & 0xffff;
+          }
+          if (stop) {
+            I += (uint32_t)(pj - pos0);
+            IT += j;
+            return pj;
+          }
+          w.set_low(a, low);
+          const int d = (int)(low & 0xff) - (int)(wold & 0xff);
+          cdelta += (uint32_t)(d * (int)((2u * (uint32_t)a + 1u) * 16411u));
+        } else {
+          h0 = (h0 + (int)((kTH0 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
+          h1 = (h1 + (int)((kTH1 >> (2 * k)) & 3) - 1) & (kTape2 - 1);
+        }
+      }
+      const uint32_t ib = I + (uint32_t)(close - pos0) + 1;  // steps after the `]`
+      const bool due = cq != -2 && ib >= kCycleStart &&
+                       ((close == cq && h0 == ch0 && h1 == ch1 && cdelta == 0) || cq < 0 ||
+                        ib - ci >= cwin);
+      if (due || C::null(w.word(h0) & 0xff)) {
+        I += (uint32_t)(close - pos0);
+        IT += nc;
+        return close;
+      }
+      I += len;
+      IT += nc + 1;
+    }
+  }
+
   // Runs commands until the run ends (r.done) or `budget` (>= 1) has been
   // spent in this call: commands plus the tape reads of loop
   // acceleration's orbit walks.  With RESUME a later call goes on from the
@@ -333,6 +418,7 @@ struct BffLean {
     int bsrc = r.bsrc, bq = r.bq, bnext = r.bnext, binc = r.binc;
     int fsrc = r.fsrc, fq = r.fq, fnext = r.fnext, finc = r.finc;
     uint32_t bacc = r.bacc ? 1u : 0u;
+    uint32_t bcomp = r.bcomp ? 1u : 0u;
     uint32_t bwrite = r.bwrite ? 1u : 0u;
     uint32_t bcmds = r.bcmds, mvalid = r.mvalid;
     int aq = r.aq, ah0 = r.ah0, ah1 = r.ah1;
@@ -383,6 +469,7 @@ struct BffLean {
       uint32_t inc = taken ? (uint32_t)(tb ? binc : finc) : nx - (uint32_t)pc;
       const uint32_t miss_b = taken ? (uint32_t)(pc ^ jsrc) : 0u;
       bool rare = wrare | (miss_b != 0) | (tb & (bacc != 0));
+      if constexpr (CYCLE) rare = rare | (tb & (bcomp != 0) & (bacc == 0));
       if constexpr (CYCLE) {
         // A cycle check is due after a taken backward jump once the anchor
         // state may have recurred or its window has passed.
@@ -449,6 +536,7 @@ struct BffLean {
                 bnext = next;
                 binc = (int)inc;
                 bacc = 0;
+                bcomp = 0;
                 if (q != kNoMatch) {
                   const Mask128 body = mask_between((int)q, pc);
                   const bool straight = !mask_any(m.open, body) && !mask_any(m.close, body);
@@ -457,6 +545,7 @@ struct BffLean {
                   bacc = straight && !(bw && bmove) ? 1u : 0u;
                   const Mask128 cm{m.cmd.lo & body.lo, m.cmd.hi & body.hi};
                   bcmds = (uint32_t)(__builtin_popcountll(cm.lo) + __builtin_popcountll(cm.hi)) + 1;
+                  bcomp = straight && bcmds >= 2 && bcmds <= kStraightMax + 1 ? 1u : 0u;
                 }
                 afail = 0;
               } else {
@@ -532,6 +621,7 @@ struct BffLean {
                   extra += reps * (iters + 1 + extra - cops);
                   cq = -2;  // fewer than one period of steps is left
                   aq = -1;  // the loop anchor's counts predate the skip
+                  skipped = true;
                 } else if (cq < 0 || ib - ci >= cwin) {
                   if (cq >= 0) {
                     cwin = cfirst ? kCycleWin : (cwin < (1u << 29) ? cwin * 2 : cwin);
@@ -545,6 +635,17 @@ struct BffLean {
                   cops = iters + 1 + extra;
                   cdelta = 0;
                 }
+              }
+              // A straight loop that acceleration does not take: its next
+              // iterations run here without decoding.
+              if (tb && !skipped && bcomp && !bacc && bsrc == pc && next == bnext) {
+                uint32_t I = i + inc, IT = iters + 1;
+                const int out = run_straight(w, pc, bq, bnext, (uint32_t)binc, bcmds - 1, stepcount,
+                                             lim1, cq, ch0, ch1, ci, cwin, h0, h1, I, IT, cdelta);
+                next = out;
+                inc = I - i;
+                iters = IT - 1;
+                aq = -1;
               }
             }
           }
@@ -565,6 +666,7 @@ struct BffLean {
     r.bsrc = bsrc; r.bq = bq; r.bnext = bnext; r.binc = binc;
     r.fsrc = fsrc; r.fq = fq; r.fnext = fnext; r.finc = finc;
     r.bacc = bacc != 0; r.bwrite = bwrite != 0; r.bcmds = bcmds; r.mvalid = mvalid;
+    r.bcomp = bcomp != 0;
     r.aq = aq; r.ah0 = ah0; r.ah1 = ah1;
     r.ai = ai; r.aiters = aiters; r.aw0 = aw0; r.aw1 = aw1; r.afail = afail;
     r.cq = cq; r.ch0 = ch0; r.ch1 = ch1;
