@@ -325,14 +325,14 @@ struct BffLean {
   // commands.  It stops, with the state exact, where the general loop must
   // take over: before an iteration that would reach the step cap or pass
   // the command limit lim1 (at pos0), before a write into the loop's own
-  // code or one that changes a byte's class or may bring a byte to a
-  // command value (at that command), and at the `]` when the byte under
-  // head 0 is null (the exit) or the cycle skip's check is due there (the
-  // anchor state cq/ch0/ch1 with no change since, no anchor yet, or the
-  // anchor's window passed).  Returns the position of the next command;
-  // heads, I, IT and cdelta (the cycle skip's weighted change sum) follow.
-  template <class W>
-  __host__ __device__ static int run_straight(const W &w, int close, int q, int pos0,
+  // code or one that changes a byte's class (at that command), and at the
+  // `]` when the byte under head 0 is null (the exit) or the cycle skip's
+  // check is due there (the anchor state cq/ch0/ch1 with no change since,
+  // no anchor yet, or the anchor's window passed).  Returns the position
+  // of the next command; heads, I, IT and cdelta (the cycle skip's
+  // weighted change sum) follow.
+  template <class W, class L>
+  __host__ __device__ static int run_straight(const W &w, const L &lut, int close, int q, int pos0,
                                               uint32_t binc, uint32_t nc, uint32_t stepcount,
                                               uint32_t lim1, int cq, int ch0, int ch1,
                                               uint32_t ci, uint32_t cwin, int &h0, int &h1,
@@ -360,8 +360,17 @@ struct BffLean {
           uint32_t low;
           bool stop = a >= q && a <= close;
           if (k == kPlus || k == kMinus) {
-            stop = stop || ((wold >> 8) & 0x7f) - 2 >= 14u;
-            low = (((wold & 0xff) + (k == kPlus ? 1u : 0xffu)) & 0xff) | ((wold & 0xff00) - 0x100u);
+            // A comment byte whose bound is 1 or less gets its exact code
+            // (still a comment: the class stays); a command byte stops.
+            const uint32_t vnew = ((wold & 0xff) + (k == kPlus ? 1u : 0xffu)) & 0xff;
+            const uint32_t code = (wold >> 8) & 0x7f;
+            uint32_t cnew = code - 1;
+            if (code < 2) {
+              cnew = lut(vnew);
+              stop = stop || (cnew >> 4) != kClsNone;
+            }
+            stop = stop || code >= 16;
+            low = vnew | (cnew << 8);
           } else {
             const uint32_t wsrc = w.word(c01 ? h0 : h1);
             stop = stop || ((wsrc >> 12) & 7) != ((wold >> 12) & 7);
@@ -640,7 +649,7 @@ struct BffLean {
               // iterations run here without decoding.
               if (tb && !skipped && bcomp && !bacc && bsrc == pc && next == bnext) {
                 uint32_t I = i + inc, IT = iters + 1;
-                const int out = run_straight(w, pc, bq, bnext, (uint32_t)binc, bcmds - 1, stepcount,
+                const int out = run_straight(w, lut, pc, bq, bnext, (uint32_t)binc, bcmds - 1, stepcount,
                                              lim1, cq, ch0, ch1, ci, cwin, h0, h1, I, IT, cdelta);
                 next = out;
                 inc = I - i;
