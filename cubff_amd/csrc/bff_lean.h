@@ -416,10 +416,13 @@ struct BffLean {
   // loop: a per-lane loop exit measured ~1.6x slower per command); the
   // rare cases are one branch, which a wavefront skips when none of its
   // lanes needs it.
+  // On the device, a run also stops (as if its budget were spent) once the
+  // wall clock passes `deadline`, checked every 32 commands.
   template <bool RESUME, bool CYCLE = false, class W, class L, class A = NoAnchor>
   __host__ __device__ __forceinline__ static void advance(Run &r, const W &w, const L &lut,
                                                           uint32_t stepcount, uint32_t budget,
-                                                          const A &anchor = A{}) {
+                                                          const A &anchor = A{},
+                                                          uint64_t deadline = ~0ull) {
     BffMasks m = r.m;
     int pc = r.done ? kTape2 : r.pc, h0 = r.h0, h1 = r.h1;
     int rpc = r.done ? kTape2 : r.pc;
@@ -439,7 +442,14 @@ struct BffLean {
     const uint32_t sc1 = stepcount - 1;
     uint32_t lim1 = iters + fast_umin(budget, 1u << 30) - 1;
     int next = rpc;
+    uint32_t tick = 0;
     while (any_lane(pc < kTape2)) {
+#if defined(__HIP_DEVICE_COMPILE__)
+      if (((++tick) & 31) == 0 && wall_clock64() > deadline) lim1 = iters - 1;
+#else
+      (void)tick;
+      (void)deadline;
+#endif
       const uint32_t wp = w.word(pc);
       const uint32_t w0 = w.word(h0);
       const uint32_t w1 = w.word(h1);
@@ -753,15 +763,16 @@ struct BffLean {
   // words of positions 0..31 (see start_raw), budgeted: a run that would
   // spend more than `budget` stops and reports *steps = kOverBudget.  The
   // final tape is byte 0 of the words.
-  // With `stopped`, a run stopped by the budget also leaves there what
-  // start_resume needs to go on from it.
+  // With `stopped`, a run stopped by the budget (or the deadline) also
+  // leaves there what start_resume needs to go on from it.
   template <class W, class L>
   __host__ __device__ static uint32_t evaluate(const W &w, const L &lut, uint32_t stepcount,
                                                uint32_t *steps, uint32_t *work, uint32_t budget,
-                                               Resume *stopped = nullptr) {
+                                               Resume *stopped = nullptr,
+                                               uint64_t deadline = ~0ull) {
     Run r;
     start_raw(r, w, lut, stepcount);
-    advance<false>(r, w, lut, stepcount, budget);
+    advance<false>(r, w, lut, stepcount, budget, NoAnchor{}, deadline);
     if (!r.done) {
       if (stopped) *stopped = saved(r);
       *steps = kOverBudget;

@@ -141,6 +141,12 @@ constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop seg
 // owes one of its programs before deferring its own (~half an epoch kernel
 // at 2^17 programs).
 constexpr double kOweWaitS = 200e-6;
+// A run of the deferring epoch kernel still going this long after the
+// kernel's start stops as if its command budget were spent (the resolver
+// goes on from it): the rare runs whose commands mostly take the slow path
+// no longer hold their epoch (2^17 bff_noheads: 0.425 -> 0.401 ms per epoch;
+// 150 us and less defer too much).
+constexpr double kRunLimitS = 200e-6;
 // Deferring segments end by waiting for the resolver to finish the last
 // epochs' long interactions: long segments (bounded by kFlowBytes of list
 // entries, 16 bytes per program and epoch) amortize that wait.
@@ -553,6 +559,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.spin_ticks = (uint64_t)(c->flow_wait_s * khz * 1000.0);
   a.idle_ticks = (uint64_t)(std::min(c->resolver_idle_s, c->flow_wait_s) * khz * 1000.0);
   a.owe_ticks = (uint64_t)(std::min(kOweWaitS, c->flow_wait_s) * khz * 1000.0);
+  a.run_ticks = (uint64_t)(std::min(kRunLimitS, c->flow_wait_s) * khz * 1000.0);
   hipEvent_t started = nullptr, resolved = nullptr;
   std::vector<hipEvent_t> kev(kern_ms ? 2 * epochs : 0, nullptr);
   auto cleanup = [&]() {

@@ -360,7 +360,13 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer_lean(
   const LdsByteLut blut{(const lds_u8 *)lut};
   uint32_t ops = 0, steps = 0, p1 = 0, p2 = 0, o1 = kNone, o2 = kNone;
   const bool eval = k < *num_indices_p;  // common_language.h:185
-  const uint64_t deadline = wall_clock64() + a.owe_ticks;
+  const uint64_t t_start = wall_clock64();
+  const uint64_t deadline = t_start + a.owe_ticks;
+  // Runs stop at the budget or this long after the kernel's start,
+  // whichever comes first (a stopped run loses nothing: the resolver goes on
+  // from it).
+  const uint64_t run_deadline =
+      a.run_ticks && budget != 0xffffffffu ? t_start + a.run_ticks : ~0ull;
   if (k < a.slots) {
     p1 = shuf[2 * k];
     p2 = shuf[2 * k + 1];
@@ -407,7 +413,7 @@ __global__ __launch_bounds__(kEpochLanes) void k_epoch_defer_lean(
       if (o2 == kResolved) load_raw_row_coherent(w, 1, a.soup + (uint64_t)p2 * kTape);
       else load_raw_row(w, 1, a.soup + (uint64_t)p2 * kTape);
       mutate_raw(w, a.n, seed_value, k, a.prob);
-      if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, budget, &rs);
+      if (eval) ops = F::evaluate(w, blut, kEvalSteps, &steps, nullptr, budget, &rs, run_deadline);
       else raw_as_final(w);  // not evaluated: the mutated tape goes back as it is
       if (steps == kOverBudget) {
         ops = steps = 0;

@@ -106,6 +106,7 @@ struct DeferArgs {
   uint64_t spin_ticks;
   uint64_t idle_ticks;  // resolver: leave if nothing arrives and no epoch ends this long
   uint64_t owe_ticks;   // epoch kernel: wait this long for an owing entry before deferring
+  uint64_t run_ticks;   // epoch kernel: a run still going this long after the kernel's start is deferred (0: no limit)
   // Batched soups (num_soups > 0; batch.h): soup s owns programs
   // [prog_base[s], prog_base[s+1]) and pair slots [slot_base[s],
   // slot_base[s+1]); `k` of a list entry is a slot over all soups.  An
