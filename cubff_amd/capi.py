@@ -24,7 +24,7 @@ EXPORTS = [
     "cubff_write_soup", "cubff_read_soup", "cubff_set_shuffle",
     "cubff_set_interactions", "cubff_pair", "cubff_run_epoch",
     "cubff_run_epochs", "cubff_run_epochs_timed", "cubff_set_pipeline", "cubff_flow_stats", "cubff_set_defer_budget",
-    "cubff_set_resolver_idle",
+    "cubff_set_resolver_idle", "cubff_set_selfrep_chunk",
     "cubff_byte_histogram",
     "cubff_read_counters", "cubff_reset_counters",
     "cubff_read_pair_ops", "cubff_read_shuffle", "cubff_selfrep",
@@ -94,6 +94,7 @@ def load():
         "cubff_set_pipeline": (_i, [_vp, _i, ctypes.c_double]),
         "cubff_flow_stats": (_i, [_vp, _u64p, _u64p, _u64p]),
         "cubff_set_defer_budget": (_i, [_vp, ctypes.c_uint32]),
+        "cubff_set_selfrep_chunk": (_i, [_vp, ctypes.c_uint64]),
         "cubff_set_resolver_idle": (_i, [_vp, ctypes.c_double]),
         "cubff_byte_histogram": (_i, [_vp, _u64p]),
         "cubff_read_counters": (_i, [_vp, _u64p, _u64p, _u64p]),
@@ -342,6 +343,9 @@ class Soup:
 
     def set_defer_budget(self, budget: int):
         _check(load().cubff_set_defer_budget(self._h, budget))
+
+    def set_selfrep_chunk(self, programs: int):
+        _check(load().cubff_set_selfrep_chunk(self._h, programs))
 
     def set_resolver_idle(self, seconds: float):
         _check(load().cubff_set_resolver_idle(self._h, seconds))

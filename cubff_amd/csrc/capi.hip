@@ -68,6 +68,7 @@ struct cubff_ctx {
   uint32_t defer_budget = kDeferBudget;
   uint64_t defer_items = 0;  // interactions deferred since creation
   double resolver_idle_s = 0.1;  // the resolver leaves after this long with nothing deferred
+  uint64_t sr_chunk = kSelfrepChunk;  // programs per chunk of the levelled self-rep check
 };
 
 namespace {
@@ -135,7 +136,11 @@ constexpr uint64_t kFlowBytes = 1ULL << 30;
 constexpr double kFlowPatienceS = 2e-6;
 // Scratch and results of Fisher-Yates pairings drawn several epochs at once.
 constexpr uint64_t kFyBatchBytes = 256ULL << 20;
-constexpr unsigned kResolveWaves = 512;  // resolver wavefronts (one interaction each)
+// Resolver wavefronts (one interaction each); 256 measured 2.5% faster per
+// epoch than 512 and 1,024 at 2^17 bff_noheads, 128 within 1%.
+constexpr unsigned kResolveWaves = 256;
+// Entries awaiting the resolver past which epoch kernels defer nothing more.
+constexpr uint32_t kMaxBacklog = 1024;
 constexpr uint64_t kLoopSegmentEpochs = 64;  // epochs per synchronised loop segment
 // How long an epoch kernel's lane waits for the deferred interaction that
 // owes one of its programs before deferring its own (~half an epoch kernel
@@ -547,7 +552,7 @@ int run_defer_segment(cubff_ctx *c, int mode, uint64_t params_seed, uint64_t fir
   a.epochs = epochs;
   a.capacity = (uint32_t)(epochs * (n / 2));
   a.budget = c->defer_budget;
-  a.max_backlog = 2 * kResolveWaves;
+  a.max_backlog = kMaxBacklog;
   a.gen = ++c->defer_gen;
   if (a.gen == 0) a.gen = c->defer_gen = 1;  // 0 marks never-written entries
   a.prob = prob;
@@ -1040,6 +1045,22 @@ int cubff_set_resolver_idle(cubff_ctx *c, double seconds) {
   return CUBFF_OK;
 }
 
+int cubff_set_selfrep_chunk(cubff_ctx *c, uint64_t programs) {
+  int r;
+  if ((r = check_ctx(c))) return r;
+  if (programs > kSelfrepChunk)
+    return fail(CUBFF_EINVAL, "self-rep chunk exceeds 2^22 programs");
+  c->sr_chunk = programs ? programs : kSelfrepChunk;
+  // The next check allocates the scratch for the new size.
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  dev_free(c->sr.finals);
+  dev_free(c->sr.list[0]);
+  dev_free(c->sr.list[1]);
+  dev_free(c->sr.counts);
+  c->sr.chunk = 0;
+  return CUBFF_OK;
+}
+
 int cubff_set_defer_budget(cubff_ctx *c, uint32_t budget) {
   int r;
   if ((r = check_ctx(c))) return r;
@@ -1111,7 +1132,7 @@ int cubff_selfrep(cubff_ctx *c, uint64_t seed_value, uint64_t count,
   if (!c->selfrep && (r = dev_alloc(&c->selfrep, c->n))) return r;
   if (selfrep_levels_supported(c->lang)) {
     if (!c->sr.finals) {
-      const uint64_t chunk = std::min<uint64_t>(c->n, kSelfrepChunk);
+      const uint64_t chunk = std::min<uint64_t>(c->n, c->sr_chunk);
       if ((r = dev_alloc(&c->sr.finals, chunk * 13 * 128)) ||
           (r = dev_alloc(&c->sr.list[0], chunk * 13)) ||
           (r = dev_alloc(&c->sr.list[1], chunk * 13)) || (r = dev_alloc(&c->sr.counts, 2)))

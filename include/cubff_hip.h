@@ -129,6 +129,12 @@ int cubff_flow_stats(cubff_ctx *ctx, uint64_t *items, uint64_t *claims, uint64_t
  * uncommon byte metrics of the callback block (common_language.h:519-540)
  * without copying the soup. */
 int cubff_byte_histogram(cubff_ctx *ctx, uint64_t *counts);
+/* Programs per chunk of cubff_selfrep's levelled check (0: the default,
+ * 2^22, at most 2^22): its scratch holds 13 final tapes of 128 bytes per
+ * program of a chunk (7 GiB at 2^22).  Frees the scratch; the next check
+ * allocates it.  No reference counterpart: the reference's CheckSelfRep
+ * (common_language.h:203-277) keeps no per-chain state. */
+int cubff_set_selfrep_chunk(cubff_ctx *ctx, uint64_t programs);
 /* Command budget of an interaction in CUBFF_FLOW_DEFER epoch kernels
  * (default 512). */
 int cubff_set_defer_budget(cubff_ctx *ctx, uint32_t budget);

@@ -189,9 +189,10 @@ def test_selfrep_vs_oracle(capi, lang):
 
 @pytest.mark.parametrize("lang", ["bff_noheads", "bff_perm"])
 def test_selfrep_two_chunks_vs_oracle(capi, lang):
-    # More programs than one chunk of the levelled check (2^18), with copy
-    # loops planted around the chunk boundary and elsewhere so that chains
-    # reach every level (long runs, cycles, programs that copy themselves).
+    # More programs than one chunk of the levelled check (set to 2^18 here),
+    # with copy loops planted around the chunk boundary and elsewhere so that
+    # chains reach every level (long runs, cycles, programs that copy
+    # themselves); then the same count in one chunk.
     n, count = 1 << 19, (1 << 18) + 3000
     soup = O.init_programs(n, O.epoch_seed(11, 0))
     plants = [b"[[{.>]-]                ]", b"[<,]", b"[{.>>}]", b"[[[-]+]", b"{[.>]}" * 6,
@@ -207,11 +208,15 @@ def test_selfrep_two_chunks_vs_oracle(capi, lang):
     print("selfrep two chunks", lang, flush=True)
     with capi.Soup(lang, n) as g:
         g.write_soup(soup)
+        g.set_selfrep_chunk(1 << 18)
         got = g.selfrep(sv, count)
         again = g.selfrep(sv, 4096)
+        g.set_selfrep_chunk(0)
+        whole = g.selfrep(sv, count)
     want = O.check_selfrep(lang, soup, sv, count)
     assert np.array_equal(got, want)
     assert np.array_equal(again, want[:4096])
+    assert np.array_equal(whole, want)
 
 
 def test_eval_selfrep_single(capi):
