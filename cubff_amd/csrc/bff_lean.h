@@ -417,7 +417,10 @@ struct BffLean {
   // rare cases are one branch, which a wavefront skips when none of its
   // lanes needs it.
   // On the device, a run also stops (as if its budget were spent) once the
-  // wall clock passes `deadline`, checked every 32 commands.
+  // wall clock passes `deadline`, checked every 32 commands: the wavefront
+  // leaves the loop before its next command, where every lane's state is
+  // whole.  The check is laid out off the loop's straight path; in line it
+  // cost ~10% per command (a taken branch every command), off it ~5%.
   template <bool RESUME, bool CYCLE = false, class W, class L, class A = NoAnchor>
   __host__ __device__ __forceinline__ static void advance(Run &r, const W &w, const L &lut,
                                                           uint32_t stepcount, uint32_t budget,
@@ -445,7 +448,7 @@ struct BffLean {
     uint32_t tick = 0;
     while (any_lane(pc < kTape2)) {
 #if defined(__HIP_DEVICE_COMPILE__)
-      if (((++tick) & 31) == 0 && wall_clock64() > deadline) lim1 = iters - 1;
+      if (__builtin_expect(((++tick) & 31) == 0, 0) && wall_clock64() > deadline) break;
 #else
       (void)tick;
       (void)deadline;
